@@ -1,0 +1,1255 @@
+// Host side of libdtrt_mi355x: the C-ABI declared in include/dtrt.h.
+//
+// Objects copy their inputs (shape.cpp:14-32, camera.cpp:13-27). Scene creation
+// (scene.cpp:9-136) flattens the shapes, builds the BVH that replaces Embree, the
+// light, edge and primary-edge distributions, and uploads everything to HBM once.
+// A render is a sequence of kernel launches on the scene's stream.
+#include "launch.h"
+#include "../../include/dtrt.h"
+#include "../../include/dtrt_testing.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace dt;
+
+namespace {
+
+thread_local std::string g_err;
+int g_nder = -1;
+
+int nderNow() {
+    if (g_nder < 0) {
+        const char *e = std::getenv("DTRT_NDER");
+        int n = e ? std::atoi(e) : 4;
+        g_nder = (n == 1 || n == 2 || n == 3 || n == 4 || n == 8 || n == 16) ? n : 4;
+    }
+    return g_nder;
+}
+
+bool fail(const std::string &m) { g_err = m; return false; }
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);                      \
+            return false;                                                                \
+        }                                                                                \
+    } while (0)
+
+struct Obj {
+    int tag;
+    explicit Obj(int t) : tag(t) {}
+    virtual ~Obj() = default;
+};
+enum { T_CAMERA = 1, T_SHAPE, T_BSDF, T_PHASE, T_EMITTER, T_MEDIUM, T_SCENE };
+
+struct D3 { double x, y, z; };
+inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 cross3(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double norm3(D3 a) { return std::sqrt(dot3(a, a)); }
+
+bool invert4(const double m[4][4], double o[4][4]) {
+    double a[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) a[i][j] = j < 4 ? m[i][j] : (j - 4 == i);
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r) if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        if (a[p][c] == 0) return false;
+        for (int j = 0; j < 8; ++j) std::swap(a[p][j], a[c][j]);
+        double d = a[c][c];
+        for (int j = 0; j < 8; ++j) a[c][j] /= d;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) { double f = a[r][c]; for (int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j]; }
+    }
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) o[i][j] = a[i][j + 4];
+    return true;
+}
+
+// ------------------------------------------------------------------ objects
+struct Camera : Obj {
+    Camera() : Obj(T_CAMERA) {}
+    int w = 0, h = 0, ox = 0, oy = 0, cw = -1, ch = -1, med = -1, nder = 0;
+    float clip_near = 0;
+    double c2w[4][4], w2c[4][4], c2ndc[3][3], ndc2c[3][3];
+    D3 cpos, cs, ct, cn, cpos_d[DTRT_NDER_MAX], cs_d[DTRT_NDER_MAX], ct_d[DTRT_NDER_MAX], cn_d[DTRT_NDER_MAX];
+    bool cropped() const { return cw > 0 && ch > 0; }
+};
+
+struct EdgeH { int v0, v1, f0, f1; double len; };
+
+struct Shape : Obj {
+    Shape() : Obj(T_SHAPE) {}
+    int nv = 0, nf = 0, nder = 0, light_id = -1, bsdf_id = -1, med_int = -1, med_ext = -1;
+    std::vector<D3> v, vd, n, nd, fn, fnd;   // tangents entity-major: vd[i*nder + j]
+    std::vector<int> f;
+    std::vector<EdgeH> edges;
+    std::vector<double> edge_cdf;            // normalised, ne+1 entries
+    double edge_sum = 0;
+    bool hasNormals() const { return !n.empty(); }
+};
+
+struct Bsdf : Obj { Bsdf() : Obj(T_BSDF) { std::memset(&d, 0, sizeof(d)); } DBsdf d; };
+struct Phase : Obj { Phase() : Obj(T_PHASE) { std::memset(&d, 0, sizeof(d)); } DPhase d; };
+struct Emitter : Obj { Emitter() : Obj(T_EMITTER) { std::memset(&d, 0, sizeof(d)); } DEmitter d; };
+struct Medium : Obj {
+    Medium() : Obj(T_MEDIUM) { std::memset(&d, 0, sizeof(d)); }
+    DMedium d;
+    std::vector<float> density, albedo;
+};
+
+// shape.cpp:109-126 — face normals and their tangents (double)
+void computeFaceNormals(Shape &s) {
+    s.fn.assign(s.nf, D3{0, 0, 0});
+    s.fnd.assign((size_t)s.nf * s.nder, D3{0, 0, 0});
+    for (int k = 0; k < s.nf; ++k) {
+        int a = s.f[3 * k], b = s.f[3 * k + 1], c = s.f[3 * k + 2];
+        D3 e1 = s.v[b] - s.v[a], e2 = s.v[c] - s.v[a], cr = cross3(e1, e2);
+        double len = norm3(cr);
+        for (int j = 0; j < s.nder; ++j) {
+            D3 de1 = s.vd[(size_t)b * s.nder + j] - s.vd[(size_t)a * s.nder + j];
+            D3 de2 = s.vd[(size_t)c * s.nder + j] - s.vd[(size_t)a * s.nder + j];
+            D3 dc = cross3(de1, e2) + cross3(e1, de2);
+            if (len < 1e-7) s.fnd[(size_t)k * s.nder + j] = dc;
+            else {
+                D3 nn = cr * (1.0 / len);
+                s.fnd[(size_t)k * s.nder + j] = (dc - nn * dot3(nn, dc)) * (1.0 / len);
+            }
+        }
+        s.fn[k] = len < 1e-7 ? cr : cr * (1.0 / len);
+    }
+}
+
+// shape.cpp:254-300 — edges ordered by (min vertex, max vertex); crease or boundary only
+bool constructEdges(Shape &s) {
+    std::vector<std::array<int, 3>> ev;
+    ev.reserve((size_t)s.nf * 3);
+    for (int k = 0; k < s.nf; ++k)
+        for (int e = 0; e < 3; ++e) {
+            int a = s.f[3 * k + e], b = s.f[3 * k + (e + 1) % 3];
+            ev.push_back({std::min(a, b), std::max(a, b), k});
+        }
+    std::sort(ev.begin(), ev.end());
+    s.edges.clear();
+    for (size_t i = 0; i < ev.size();) {
+        size_t j = i;
+        while (j < ev.size() && ev[j][0] == ev[i][0] && ev[j][1] == ev[i][1]) ++j;
+        size_t cnt = j - i;
+        int a = ev[i][0], b = ev[i][1];
+        double len = norm3(s.v[a] - s.v[b]);
+        if (cnt > 2) return fail("shape: every edge can be shared by at most 2 faces");
+        if (cnt == 2) {
+            int f0 = ev[i][2], f1 = ev[i + 1][2];
+            if (f0 == f1) return fail("shape: duplicated faces");
+            double val = dot3(s.fn[f0], s.fn[f1]);
+            if (val < -1.0 + 1e-7) return fail("shape: inconsistent normal orientation");
+            if (val < 1.0 - 1e-7) s.edges.push_back({a, b, f0, f1, len});
+        } else {
+            s.edges.push_back({a, b, ev[i][2], -1, len});
+        }
+        i = j;
+    }
+    s.edge_cdf.assign(s.edges.size() + 1, 0.0);
+    for (size_t e = 0; e < s.edges.size(); ++e) s.edge_cdf[e + 1] = s.edge_cdf[e] + s.edges[e].len;
+    s.edge_sum = s.edge_cdf.back();
+    if (s.edge_sum > 0) {
+        for (double &c : s.edge_cdf) c /= s.edge_sum;
+        s.edge_cdf.back() = 1.0;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------- BVH build
+// Binned-SAH binary BVH; GPU nodes hold both children's boxes (64 B) and leaves
+// are encoded in the child slot as ~((first << 4) | count), count <= 15.
+struct BuildNode { float lo[3], hi[3]; int left = -1, right = -1, first = 0, count = 0; };
+
+struct BvhOut {
+    std::vector<float4> nodes, tris;
+    int depth = 0;
+};
+
+bool buildBvh(const std::vector<std::array<float, 9>> &tri, BvhOut &out) {
+    const int n = (int)tri.size();
+    std::vector<float> cen((size_t)n * 3), lo((size_t)n * 3), hi((size_t)n * 3);
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            float x0 = tri[i][a], x1 = tri[i][3 + a], x2 = tri[i][6 + a];
+            lo[3 * i + a] = std::min(x0, std::min(x1, x2));
+            hi[3 * i + a] = std::max(x0, std::max(x1, x2));
+            cen[3 * i + a] = 0.5f * (lo[3 * i + a] + hi[3 * i + a]);
+        }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::vector<BuildNode> bn;
+    bn.reserve(2 * (size_t)n + 1);
+    struct Task { int node, first, count, depth; };
+    bn.push_back(BuildNode());
+    std::vector<Task> st{{0, 0, n, 1}};
+    const int NB = 16, MAXLEAF = 15;
+    int maxDepth = 0;
+    auto area = [](const float *l, const float *h) {
+        float dx = std::max(0.f, h[0] - l[0]), dy = std::max(0.f, h[1] - l[1]), dz = std::max(0.f, h[2] - l[2]);
+        return dx * dy + dy * dz + dz * dx;
+    };
+    while (!st.empty()) {
+        Task t = st.back();
+        st.pop_back();
+        maxDepth = std::max(maxDepth, t.depth);
+        float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = t.first; i < t.first + t.count; ++i) {
+            int p = idx[i];
+            for (int a = 0; a < 3; ++a) {
+                nlo[a] = std::min(nlo[a], lo[3 * p + a]); nhi[a] = std::max(nhi[a], hi[3 * p + a]);
+                clo[a] = std::min(clo[a], cen[3 * p + a]); chi[a] = std::max(chi[a], cen[3 * p + a]);
+            }
+        }
+        BuildNode &nd = bn[t.node];
+        std::memcpy(nd.lo, nlo, sizeof(nlo));
+        std::memcpy(nd.hi, nhi, sizeof(nhi));
+        bool mustSplit = t.count > MAXLEAF;
+        if (t.count <= 4 && !mustSplit) { nd.first = t.first; nd.count = t.count; continue; }
+        // binned SAH over the three axes
+        int bestAxis = -1, bestSplit = -1;
+        float bestCost = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            float ext = chi[a] - clo[a];
+            if (!(ext > 0)) continue;
+            int cnt[NB] = {0};
+            float blo[NB][3], bhi[NB][3];
+            for (int b = 0; b < NB; ++b) for (int c = 0; c < 3; ++c) { blo[b][c] = INFINITY; bhi[b][c] = -INFINITY; }
+            for (int i = t.first; i < t.first + t.count; ++i) {
+                int p = idx[i];
+                int b = std::min(NB - 1, (int)((cen[3 * p + a] - clo[a]) / ext * NB));
+                cnt[b]++;
+                for (int c = 0; c < 3; ++c) { blo[b][c] = std::min(blo[b][c], lo[3 * p + c]); bhi[b][c] = std::max(bhi[b][c], hi[3 * p + c]); }
+            }
+            float rA[NB], rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int rC[NB], rc = 0;
+            for (int b = NB - 1; b > 0; --b) {
+                rc += cnt[b];
+                for (int c = 0; c < 3; ++c) { rl[c] = std::min(rl[c], blo[b][c]); rh[c] = std::max(rh[c], bhi[b][c]); }
+                rA[b] = rc ? area(rl, rh) : 0.f;
+                rC[b] = rc;
+            }
+            float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int lc = 0;
+            for (int b = 0; b < NB - 1; ++b) {
+                lc += cnt[b];
+                for (int c = 0; c < 3; ++c) { ll[c] = std::min(ll[c], blo[b][c]); lh[c] = std::max(lh[c], bhi[b][c]); }
+                if (lc == 0 || rC[b + 1] == 0) continue;
+                float cost = area(ll, lh) * lc + rA[b + 1] * rC[b + 1];
+                if (cost < bestCost) { bestCost = cost; bestAxis = a; bestSplit = b; }
+            }
+        }
+        float leafCost = area(nlo, nhi) * t.count;
+        int mid;
+        if (bestAxis < 0 || (!mustSplit && bestCost >= leafCost) || t.depth >= DTRT_BVH_STACK - 2) {
+            if (!mustSplit || (bestAxis < 0 && t.count <= MAXLEAF)) { nd.first = t.first; nd.count = t.count; continue; }
+            if (t.depth >= DTRT_BVH_STACK - 2) return fail("bvh: tree deeper than the traversal stack");
+            mid = t.first + t.count / 2;   // identical centroids: split by index
+        } else {
+            float ext = chi[bestAxis] - clo[bestAxis];
+            auto it = std::partition(idx.begin() + t.first, idx.begin() + t.first + t.count, [&](int p) {
+                return std::min(NB - 1, (int)((cen[3 * p + bestAxis] - clo[bestAxis]) / ext * NB)) <= bestSplit;
+            });
+            mid = (int)(it - idx.begin());
+            if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+        }
+        int l = (int)bn.size();
+        bn.push_back(BuildNode());
+        bn.push_back(BuildNode());
+        bn[t.node].left = l;
+        bn[t.node].right = l + 1;
+        st.push_back({l, t.first, mid - t.first, t.depth + 1});
+        st.push_back({l + 1, mid, t.first + t.count - mid, t.depth + 1});
+    }
+    // triangles in leaf order
+    out.tris.resize((size_t)n * 3);
+    for (int i = 0; i < n; ++i) {
+        const auto &T = tri[idx[i]];
+        float4 v0{T[0], T[1], T[2], 0.f};
+        int fid = idx[i];
+        std::memcpy(&v0.w, &fid, 4);
+        out.tris[3 * (size_t)i] = v0;
+        out.tris[3 * (size_t)i + 1] = float4{T[3] - T[0], T[4] - T[1], T[5] - T[2], 0.f};
+        out.tris[3 * (size_t)i + 2] = float4{T[6] - T[0], T[7] - T[1], T[8] - T[2], 0.f};
+    }
+    // GPU nodes: inner build nodes only; a leaf root gets a wrapper node
+    std::vector<int> gpuIndex(bn.size(), -1);
+    std::vector<int> order;
+    if (bn[0].left < 0) {
+        out.nodes.resize(4);
+        const BuildNode &r = bn[0];
+        int leaf = ~((r.first << 4) | r.count), empty = ~0;
+        int il, ir;
+        std::memcpy(&il, &leaf, 4);
+        std::memcpy(&ir, &empty, 4);
+        float fl, fr;
+        std::memcpy(&fl, &leaf, 4);
+        std::memcpy(&fr, &empty, 4);
+        out.nodes[0] = float4{r.lo[0], r.lo[1], r.lo[2], fl};
+        out.nodes[1] = float4{r.hi[0], r.hi[1], r.hi[2], fr};
+        out.nodes[2] = float4{1.f, 1.f, 1.f, 0.f};
+        out.nodes[3] = float4{-1.f, -1.f, -1.f, 0.f};
+        out.depth = 1;
+        return true;
+    }
+    std::vector<int> stack{0};
+    while (!stack.empty()) {   // DFS preorder numbering of inner nodes
+        int b = stack.back();
+        stack.pop_back();
+        gpuIndex[b] = (int)order.size();
+        order.push_back(b);
+        for (int c : {bn[b].right, bn[b].left})
+            if (bn[c].left >= 0) stack.push_back(c);
+    }
+    out.nodes.resize(order.size() * 4);
+    auto enc = [&](int c) {
+        const BuildNode &x = bn[c];
+        int v = x.left >= 0 ? gpuIndex[c] : ~((x.first << 4) | x.count);
+        float f;
+        std::memcpy(&f, &v, 4);
+        return f;
+    };
+    for (size_t g = 0; g < order.size(); ++g) {
+        const BuildNode &p = bn[order[g]], &L = bn[p.left], &R = bn[p.right];
+        out.nodes[4 * g] = float4{L.lo[0], L.lo[1], L.lo[2], enc(p.left)};
+        out.nodes[4 * g + 1] = float4{L.hi[0], L.hi[1], L.hi[2], enc(p.right)};
+        out.nodes[4 * g + 2] = float4{R.lo[0], R.lo[1], R.lo[2], 0.f};
+        out.nodes[4 * g + 3] = float4{R.hi[0], R.hi[1], R.hi[2], 0.f};
+    }
+    out.depth = maxDepth;
+    return true;
+}
+
+// ------------------------------------------------------------ primary edges
+// edge_manager.cpp:31-110 with line_clip.h:37-95, in double on the host.
+struct P2 { double x, y; double dx[DTRT_NDER_MAX], dy[DTRT_NDER_MAX]; };
+struct PEdge { P2 s0, s1; D3 p0, p1, c0, c1; };
+
+struct ADV { D3 v; D3 g[DTRT_NDER_MAX]; };
+
+int outcode(double x, double y, double xmin, double xmax, double ymin, double ymax) {
+    int c = 0;
+    if (x < xmin - 1e-7) c |= 1; else if (x > xmax + 1e-7) c |= 2;
+    if (y < ymin - 1e-7) c |= 4; else if (y > ymax + 1e-7) c |= 8;
+    return c;
+}
+P2 lerp2(const P2 &a, const P2 &b, double s, int nd) {
+    P2 r;
+    r.x = a.x + (b.x - a.x) * s;
+    r.y = a.y + (b.y - a.y) * s;
+    for (int j = 0; j < nd; ++j) { r.dx[j] = a.dx[j] + (b.dx[j] - a.dx[j]) * s; r.dy[j] = a.dy[j] + (b.dy[j] - a.dy[j]) * s; }
+    return r;
+}
+bool clipSeg(P2 a, P2 b, P2 &c0, P2 &c1, double xmin, double xmax, double ymin, double ymax, int nd) {
+    int o0 = outcode(a.x, a.y, xmin, xmax, ymin, ymax), o1 = outcode(b.x, b.y, xmin, xmax, ymin, ymax);
+    c0 = a;
+    c1 = b;
+    for (int it = 0; it < 64; ++it) {
+        if (!(o0 | o1)) return true;
+        if (o0 & o1) return false;
+        int oo = o0 ? o0 : o1;
+        P2 p;
+        if (oo & 8) p = lerp2(c0, c1, (ymax - c0.y) / (c1.y - c0.y), nd);
+        else if (oo & 4) p = lerp2(c0, c1, (ymin - c0.y) / (c1.y - c0.y), nd);
+        else if (oo & 2) p = lerp2(c0, c1, (xmax - c0.x) / (c1.x - c0.x), nd);
+        else p = lerp2(c0, c1, (xmin - c0.x) / (c1.x - c0.x), nd);
+        if (oo == o0) { c0 = p; o0 = outcode(c0.x, c0.y, xmin, xmax, ymin, ymax); }
+        else { c1 = p; o1 = outcode(c1.x, c1.y, xmin, xmax, ymin, ymax); }
+    }
+    return false;
+}
+double inTri(D3 b0, D3 c0, D3 b1, D3 c1, double t0) {   // utils.cpp:256-266 with a = origin
+    double cb = norm3(b0) / norm3(b1), cc = norm3(c0) / norm3(c1);
+    double bx = (1 - t0) * cb, by = t0 * cc;
+    return by / (bx + by);
+}
+
+void projectEdge(const Camera &cam, ADV a, ADV b, int nd, std::vector<PEdge> &out, std::vector<double> &w) {
+    const D3 n = cam.cn;
+    double z0 = dot3(a.v - cam.cpos, n), z1 = dot3(b.v - cam.cpos, n), cn = cam.clip_near;
+    auto addScaled = [&](ADV &x, const ADV &y, const ADV &z, double s) {   // x += s * (y - z)
+        x.v = x.v + (y.v - z.v) * s;
+        for (int j = 0; j < nd; ++j) x.g[j] = x.g[j] + (y.g[j] - z.g[j]) * s;
+    };
+    if (z0 < cn && z1 < cn) return;
+    if (z0 < cn) { ADV a0 = a; addScaled(a, b, a0, (cn - z0) / (z1 - z0)); }
+    else if (z1 < cn) { ADV b0 = b; addScaled(b, a, b0, (cn - z1) / (z0 - z1)); }
+    auto xfm = [&](D3 p) {
+        double hh[4];
+        for (int i = 0; i < 4; ++i) hh[i] = cam.w2c[i][0] * p.x + cam.w2c[i][1] * p.y + cam.w2c[i][2] * p.z + cam.w2c[i][3];
+        return D3{hh[0] / hh[3], hh[1] / hh[3], hh[2] / hh[3]};
+    };
+    D3 l0 = xfm(a.v), l1 = xfm(b.v);
+    // perspective division and camera-frame coordinates, with tangents from the
+    // vertex motion and the camera pose tangents
+    auto toPlane = [&](const ADV &p, D3 &pl, D3 *plg) {
+        D3 q = p.v - cam.cpos;
+        double z = dot3(q, n);
+        D3 r = q * (1.0 / z);
+        D3 loc{dot3(r, cam.cs), dot3(r, cam.ct), dot3(r, cam.cn)};
+        pl = loc;
+        for (int j = 0; j < nd; ++j) {
+            D3 dq = p.g[j] - cam.cpos_d[j];
+            double dz = dot3(dq, n) + dot3(q, cam.cn_d[j]);
+            D3 dr = (dq - r * dz) * (1.0 / z);
+            plg[j] = D3{dot3(dr, cam.cs) + dot3(r, cam.cs_d[j]), dot3(dr, cam.ct) + dot3(r, cam.ct_d[j]),
+                        dot3(dr, cam.cn) + dot3(r, cam.cn_d[j])};
+        }
+    };
+    D3 p0, p1, g0[DTRT_NDER_MAX], g1[DTRT_NDER_MAX];
+    toPlane(a, p0, g0);
+    toPlane(b, p1, g1);
+    double fov = cam.c2ndc[0][0], asp = double(cam.w) / double(cam.h);
+    P2 r0, r1;
+    r0.x = p0.x * fov * 0.5 + 0.5; r0.y = p0.y * fov * (-0.5) * asp + 0.5;
+    r1.x = p1.x * fov * 0.5 + 0.5; r1.y = p1.y * fov * (-0.5) * asp + 0.5;
+    for (int j = 0; j < nd; ++j) {
+        r0.dx[j] = g0[j].x * fov * 0.5; r0.dy[j] = g0[j].y * fov * (-0.5) * asp;
+        r1.dx[j] = g1[j].x * fov * 0.5; r1.dy[j] = g1[j].y * fov * (-0.5) * asp;
+    }
+    double xmin = 0, xmax = 1, ymin = 0, ymax = 1;
+    if (cam.cropped()) {
+        xmin = (float)cam.ox / cam.w; xmax = (float)(cam.ox + cam.cw) / cam.w;
+        ymin = (float)cam.oy / cam.h; ymax = (float)(cam.oy + cam.ch) / cam.h;
+    }
+    P2 s0, s1;
+    if (!clipSeg(r0, r1, s0, s1, xmin, xmax, ymin, ymax, nd)) return;
+    D3 pp0 = p0, pp1 = p1, c0 = l0, c1 = l1;
+    if (std::hypot(r0.x - s0.x, r0.y - s0.y) > 1e-7) {
+        D3 pc{(s0.x - 0.5) * 2.0 / fov, (s0.y - 0.5) * (-2.0) / (asp * fov), 1.0};
+        double t0 = norm3(pc - p0) / norm3(p1 - p0);
+        pp0 = p0 + (p1 - p0) * t0;
+        c0 = l0 + (l1 - l0) * inTri(p0, p1, l0, l1, t0);
+    }
+    if (std::hypot(r1.x - s1.x, r1.y - s1.y) > 1e-7) {
+        D3 pc{(s1.x - 0.5) * 2.0 / fov, (s1.y - 0.5) * (-2.0) / (asp * fov), 1.0};
+        double t0 = norm3(pc - p0) / norm3(p1 - p0);
+        pp1 = p0 + (p1 - p0) * t0;
+        c1 = l0 + (l1 - l0) * inTri(p0, p1, l0, l1, t0);
+    }
+    PEdge e;
+    e.s0 = s0; e.s1 = s1;
+    e.s0.x *= cam.w; e.s0.y *= cam.h; e.s1.x *= cam.w; e.s1.y *= cam.h;
+    for (int j = 0; j < nd; ++j) { e.s0.dx[j] *= cam.w; e.s0.dy[j] *= cam.h; e.s1.dx[j] *= cam.w; e.s1.dy[j] *= cam.h; }
+    e.p0 = pp0; e.p1 = pp1; e.c0 = c0; e.c1 = c1;
+    out.push_back(e);
+    w.push_back(std::hypot(e.s0.x - e.s1.x, e.s0.y - e.s1.y));
+}
+
+// shape.cpp:302-335 on the host (primary-edge selection from the camera position)
+int silhouetteHost(const Shape &s, const EdgeH &e, D3 p) {
+    if (e.f0 < 0 || e.f1 < 0) return 2;
+    bool ff[2];
+    int fs[2] = {e.f0, e.f1};
+    for (int k = 0; k < 2; ++k) {
+        const int *fi = &s.f[3 * fs[k]];
+        int o = fi[0];
+        if (o == e.v0 || o == e.v1) o = fi[1];
+        if (o == e.v0 || o == e.v1) o = fi[2];
+        D3 nn = k == 0 ? cross3(s.v[e.v0] - s.v[o], s.v[e.v1] - s.v[o]) : cross3(s.v[e.v1] - s.v[o], s.v[e.v0] - s.v[o]);
+        ff[k] = dot3(nn, p - s.v[o]) > 0.0;
+    }
+    if (ff[0] != ff[1]) return 2;
+    return s.hasNormals() ? 0 : 1;
+}
+
+// ------------------------------------------------------------------- scene
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Scene : Obj {
+    Scene() : Obj(T_SCENE) {}
+    ~Scene() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
+        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf}) if (b->p) (void)hipFree(b->p);
+        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int nder = 0;
+    Camera cam;
+    std::vector<Shape *> shapes;
+    std::vector<Bsdf *> bsdfs;
+    std::vector<Emitter *> emitters;
+    std::vector<Phase *> phases;
+    std::vector<Medium *> media;
+    DScene S;
+    std::vector<DevBuf> bufs;
+    DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev;
+    // last render
+    int64_t last_stats[10] = {0};
+    double kernel_ms[4] = {0};
+    int64_t kernel_launches[4] = {0};
+    uint64_t work_counts[12] = {0};
+    int edge_ebase_total = 0;
+
+    template <class T> const T *upload(const std::vector<T> &v) {
+        if (v.empty()) return nullptr;
+        DevBuf b;
+        b.bytes = v.size() * sizeof(T);
+        if (hipMalloc(&b.p, b.bytes) != hipSuccess) return nullptr;
+        if (hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(b.p); return nullptr; }
+        bufs.push_back(b);
+        return static_cast<const T *>(b.p);
+    }
+    bool ensure(DevBuf &b, size_t bytes) {
+        if (b.bytes >= bytes) return true;
+        if (b.p) { (void)hipStreamSynchronize(stream); (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+        HIPCHK(hipMalloc(&b.p, bytes));
+        b.bytes = bytes;
+        return true;
+    }
+    bool build();
+    bool initEdges(const float *weights);
+    bool render(const dtrt_render_options &o, float *out, bool out_is_device);
+};
+
+bool frameRotationReady() {
+    // utils.cpp:55 — rotation by 0.1f rad about normalize(0.1f, 0.2f, 0.3f), Rodrigues' formula
+    static std::mutex mu;
+    static std::vector<int> done;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail("no HIP device");
+    std::lock_guard<std::mutex> lk(mu);
+    if (std::find(done.begin(), done.end(), dev) != done.end()) return true;
+    double ax = 0.1f, ay = 0.2f, az = 0.3f, l = std::sqrt(ax * ax + ay * ay + az * az);
+    ax /= l; ay /= l; az /= l;
+    double th = (double)0.1f, c = std::cos(th), s = std::sin(th), t = 1 - c;
+    float m[9] = {(float)(c + ax * ax * t), (float)(ax * ay * t - az * s), (float)(ax * az * t + ay * s),
+                  (float)(ay * ax * t + az * s), (float)(c + ay * ay * t), (float)(ay * az * t - ax * s),
+                  (float)(az * ax * t - ay * s), (float)(az * ay * t + ax * s), (float)(c + az * az * t)};
+    HIPCHK(setFrameRotation(m));
+    done.push_back(dev);
+    return true;
+}
+
+bool Scene::build() {
+    if (!frameRotationReady()) return false;
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    std::memset(&S, 0, sizeof(S));
+    S.nder = nder;
+    S.nshapes = (int)shapes.size();
+    S.nlights = (int)emitters.size();
+    S.nmedia = (int)media.size();
+    // flatten geometry
+    std::vector<float> V, Vd, Nrm, Nd, FN, FNd;
+    std::vector<int> F, fshape;
+    std::vector<DShape> dsh;
+    std::vector<int4> edges;
+    std::vector<double> ecdf;
+    std::vector<std::array<float, 9>> tri;
+    int vbase = 0, fbase = 0, ebase = 0;
+    for (size_t si = 0; si < shapes.size(); ++si) {
+        const Shape &s = *shapes[si];
+        DShape d{};
+        d.vbase = vbase; d.fbase = fbase; d.nv = s.nv; d.nf = s.nf;
+        d.light_id = s.light_id; d.bsdf_id = s.bsdf_id; d.med_int = s.med_int; d.med_ext = s.med_ext;
+        d.has_normals = s.hasNormals();
+        d.ebase = ebase; d.ne = (int)s.edges.size();
+        dsh.push_back(d);
+        for (int i = 0; i < s.nv; ++i) {
+            V.insert(V.end(), {(float)s.v[i].x, (float)s.v[i].y, (float)s.v[i].z});
+            D3 nn = s.hasNormals() ? s.n[i] : D3{0, 0, 0};
+            Nrm.insert(Nrm.end(), {(float)nn.x, (float)nn.y, (float)nn.z});
+            for (int j = 0; j < nder; ++j) {
+                const D3 &g = s.vd[(size_t)i * nder + j];
+                Vd.insert(Vd.end(), {(float)g.x, (float)g.y, (float)g.z});
+                D3 h = s.hasNormals() ? s.nd[(size_t)i * nder + j] : D3{0, 0, 0};
+                Nd.insert(Nd.end(), {(float)h.x, (float)h.y, (float)h.z});
+            }
+        }
+        for (int k = 0; k < s.nf; ++k) {
+            int a = s.f[3 * k] + vbase, b = s.f[3 * k + 1] + vbase, c = s.f[3 * k + 2] + vbase;
+            F.insert(F.end(), {a, b, c});
+            fshape.push_back((int)si);
+            FN.insert(FN.end(), {(float)s.fn[k].x, (float)s.fn[k].y, (float)s.fn[k].z});
+            for (int j = 0; j < nder; ++j) {
+                const D3 &g = s.fnd[(size_t)k * nder + j];
+                FNd.insert(FNd.end(), {(float)g.x, (float)g.y, (float)g.z});
+            }
+            std::array<float, 9> t;
+            for (int q = 0; q < 3; ++q) {
+                const D3 &p = s.v[s.f[3 * k + q]];
+                t[3 * q] = (float)p.x; t[3 * q + 1] = (float)p.y; t[3 * q + 2] = (float)p.z;
+            }
+            tri.push_back(t);
+        }
+        for (const EdgeH &e : s.edges)
+            edges.push_back(int4{e.v0 + vbase, e.v1 + vbase, e.f0 + fbase, e.f1 >= 0 ? e.f1 + fbase : -1});
+        ecdf.insert(ecdf.end(), s.edge_cdf.begin(), s.edge_cdf.end());   // ne+1 entries at ebase + si
+        vbase += s.nv;
+        fbase += s.nf;
+        ebase += (int)s.edges.size();
+    }
+    S.NV = vbase;
+    S.NF = fbase;
+    BvhOut bvh;
+    if (!buildBvh(tri, bvh)) return false;
+    S.nnodes = (int)(bvh.nodes.size() / 4);
+    S.ntris = (int)tri.size();
+    S.nodes = upload(bvh.nodes);
+    S.tris = upload(bvh.tris);
+    S.V = upload(V); S.Vd = upload(Vd); S.Nrm = upload(Nrm); S.Nd = upload(Nd);
+    S.F = upload(F); S.face_shape = upload(fshape); S.FN = upload(FN); S.FNd = upload(FNd);
+    S.shapes = upload(dsh);
+    S.edges = upload(edges);
+    S.edge_cdf = upload(ecdf);
+    // materials
+    std::vector<DBsdf> db;
+    for (auto *b : bsdfs) db.push_back(b->d);
+    std::vector<DPhase> dp;
+    for (auto *p : phases) dp.push_back(p->d);
+    std::vector<DEmitter> de;
+    for (auto *e : emitters) de.push_back(e->d);
+    S.bsdfs = upload(db);
+    S.phases = upload(dp);
+    S.emitters = upload(de);
+    std::vector<DMedium> dm;
+    for (auto *m : media) {
+        DMedium d = m->d;
+        d.density = m->density.empty() ? nullptr : upload(m->density);
+        d.albedo_grid = m->albedo.empty() ? nullptr : upload(m->albedo);
+        dm.push_back(d);
+    }
+    S.media = upload(dm);
+    // lights (scene.cpp:45-98)
+    int nl = (int)emitters.size();
+    std::vector<double> lcdf(nl), pool;
+    std::vector<float> lpmf(nl), lareas(nl);
+    std::vector<int> abeg(nl);
+    double tot = 0;
+    std::vector<double> pmf(nl);
+    for (int l = 0; l < nl; ++l) {
+        const Shape &s = *shapes[emitters[l]->d.shape_id];
+        abeg[l] = (int)pool.size();
+        double sum = 0;
+        std::vector<double> c;
+        for (int k = 0; k < s.nf; ++k) {
+            const int *fi = &s.f[3 * k];
+            sum += 0.5 * norm3(cross3(s.v[fi[1]] - s.v[fi[0]], s.v[fi[2]] - s.v[fi[0]]));
+            c.push_back(sum);
+        }
+        for (double &x : c) x /= sum;
+        pool.insert(pool.end(), c.begin(), c.end());
+        lareas[l] = (float)sum;
+        const float *I = emitters[l]->d.I;
+        pmf[l] = sum * (0.212671f * I[0] + 0.715160f * I[1] + 0.072169f * I[2]) * M_PI;
+        tot += pmf[l];
+    }
+    if (nl > 0 && !(tot > 0)) return fail("scene: total light importance must be positive");
+    double run = 0;
+    for (int l = 0; l < nl; ++l) { pmf[l] /= tot; lpmf[l] = (float)pmf[l]; run += pmf[l]; lcdf[l] = run; }
+    S.light_cdf = upload(lcdf);
+    S.light_pmf = upload(lpmf);
+    S.light_areas = upload(lareas);
+    S.area_cdf_begin = upload(abeg);
+    S.area_cdf_pool = upload(pool);
+    // camera
+    DCamera &c = S.cam;
+    std::memset(&c, 0, sizeof(c));
+    c.width = cam.w; c.height = cam.h; c.ox = cam.ox; c.oy = cam.oy; c.cw = cam.cw; c.ch = cam.ch;
+    c.cropped = cam.cropped(); c.med_id = cam.med;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) c.c2w[4 * i + j] = (float)cam.c2w[i][j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c.ndc2c[3 * i + j] = (float)cam.ndc2c[i][j];
+    c.fov = (float)cam.c2ndc[0][0];
+    c.aspect = (float)cam.w / (float)cam.h;
+    c.clip_near = cam.clip_near;
+    auto put = [](float *dst, D3 v) { dst[0] = (float)v.x; dst[1] = (float)v.y; dst[2] = (float)v.z; };
+    put(c.cpos, cam.cpos); put(c.cs, cam.cs); put(c.ct, cam.ct); put(c.cn, cam.cn);
+    for (int j = 0; j < nder; ++j) { put(c.cpos_d[j], cam.cpos_d[j]); put(c.cs_d[j], cam.cs_d[j]); put(c.ct_d[j], cam.ct_d[j]); put(c.cn_d[j], cam.cn_d[j]); }
+    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
+    return initEdges(nullptr);
+}
+
+// bruteforce.cpp:4-20 and edge_manager.cpp:6-16
+bool Scene::initEdges(const float *weights) {
+    std::vector<double> w(shapes.size() + 1, 0.0);
+    double s = 0;
+    for (size_t i = 0; i < shapes.size(); ++i) {
+        bool ignore = bsdfs[shapes[i]->bsdf_id]->d.type == BSDF_NULL && shapes[i]->light_id < 0;
+        double x = (ignore ? 0.0 : shapes[i]->edge_sum) * (weights ? (double)weights[i] : 1.0);
+        if (x < -1e-7) return fail("initEdges: negative weight");
+        s += x;
+        w[i + 1] = w[i] + x;
+    }
+    S.edges_valid = s > 1e-7;
+    if (S.edges_valid) { for (double &x : w) x /= s; w.back() = 1.0; }
+    S.shape_edge_cdf = upload(w);
+    // primary edges
+    std::vector<PEdge> pe;
+    std::vector<double> pw;
+    for (const Shape *sh : shapes) {
+        if (!(bsdfs[sh->bsdf_id]->d.type != BSDF_NULL || sh->light_id >= 0)) continue;
+        for (const EdgeH &e : sh->edges) {
+            if (silhouetteHost(*sh, e, cam.cpos) <= 0) continue;
+            ADV a, b;
+            a.v = sh->v[e.v0];
+            b.v = sh->v[e.v1];
+            for (int j = 0; j < nder; ++j) { a.g[j] = sh->vd[(size_t)e.v0 * nder + j]; b.g[j] = sh->vd[(size_t)e.v1 * nder + j]; }
+            projectEdge(cam, a, b, nder, pe, pw);
+        }
+    }
+    S.npe = (int)pe.size();
+    std::vector<double> cdf(pe.size() + 1, 0.0);
+    for (size_t i = 0; i < pw.size(); ++i) cdf[i + 1] = cdf[i] + pw[i];
+    S.pe_sum = cdf.back();
+    if (S.pe_sum > 0) { for (double &x : cdf) x /= S.pe_sum; cdf.back() = 1.0; }
+    const int stride = 16 + 4 * nder;
+    std::vector<float> packed(pe.size() * stride, 0.f);
+    for (size_t i = 0; i < pe.size(); ++i) {
+        float *p = &packed[i * stride];
+        const PEdge &e = pe[i];
+        double v[16] = {e.s0.x, e.s0.y, e.s1.x, e.s1.y, e.p0.x, e.p0.y, e.p0.z, e.p1.x, e.p1.y, e.p1.z,
+                        e.c0.x, e.c0.y, e.c0.z, e.c1.x, e.c1.y, e.c1.z};
+        for (int k = 0; k < 16; ++k) p[k] = (float)v[k];
+        for (int j = 0; j < nder; ++j) {
+            p[16 + 4 * j] = (float)e.s0.dx[j]; p[17 + 4 * j] = (float)e.s0.dy[j];
+            p[18 + 4 * j] = (float)e.s1.dx[j]; p[19 + 4 * j] = (float)e.s1.dy[j];
+        }
+    }
+    S.pe_cdf = upload(cdf);
+    S.pe = packed.empty() ? nullptr : upload(packed);
+    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
+    return true;
+}
+
+constexpr size_t QUEUE_BUDGET = size_t(6) << 30;   // bytes of child records per chunk
+
+bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device) {
+    RenderParams P{};
+    P.seed = o.seed; P.spp = o.spp; P.max_bounces = o.max_bounces; P.sppe = o.sppe; P.sppse = std::max(0, o.sppse);
+    P.mode = o.mode >= 0 ? o.mode : 0x3F; P.ddist = o.ddist_coeff;
+    P.width = cam.w; P.height = cam.h; P.ox = cam.ox; P.oy = cam.oy; P.cw = cam.cw; P.ch = cam.ch; P.cropped = cam.cropped();
+    P.npix = P.cropped ? cam.cw * cam.ch : cam.w * cam.h;
+    if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
+    P.rec_stride = (int)((sizeof(ChildHdr) + 12 * nder + 15) / 16 * 16);
+    const size_t npix = P.npix, img = (size_t)(nder + 1) * npix * 3;
+    if (!ensure(acc, img * 4) || !ensure(acc_pe, (size_t)nder * npix * 3 * 4) || !ensure(qcount, 64) ||
+        !ensure(stats, 64 * 4) || !ensure(work, 16 * 8))
+        return false;
+    HIPCHK(hipMemsetAsync(acc.p, 0, img * 4, stream));
+    HIPCHK(hipMemsetAsync(acc_pe.p, 0, (size_t)nder * npix * 3 * 4, stream));
+    HIPCHK(hipMemsetAsync(stats.p, 0, 64 * 4, stream));
+    HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
+    float *dout = out;
+    if (!out_is_device) {
+        if (!ensure(outbuf, img * 4)) return false;
+        dout = static_cast<float *>(outbuf.p);
+        HIPCHK(hipMemcpyAsync(dout, out, img * 4, hipMemcpyHostToDevice, stream));
+    }
+    // child capacity per AD path: 2 sub-paths per edge sample and up to 100 Lins
+    // samples at every vertex (volpathAD.cpp:212-303, 334, 519-614)
+    const bool edges_on = P.sppse > 0 && (P.mode & (4 | 32));
+    const int lins = (P.mode & 16) ? (P.ddist > 0 ? 100 : 1) : 0;
+    const uint64_t perPath = std::max<uint64_t>(1, (uint64_t)std::max(P.max_bounces, 1) * ((edges_on ? 2 * P.sppse : 0) + lins));
+    const uint64_t total = (uint64_t)npix * std::max(0, P.spp);
+    uint64_t chunk = std::max<uint64_t>(64, QUEUE_BUDGET / (perPath * P.rec_stride) / 64 * 64);
+    chunk = std::min<uint64_t>(chunk, (uint64_t)1 << 22);
+    uint64_t pe_total = 0;
+    if (P.sppe > 0 && S.npe > 0) pe_total = (uint64_t)std::ceil(S.pe_sum * P.sppe / 128.0) * 128;
+    P.has_pe = pe_total > 0;
+    const uint64_t pe_chunk = std::min<uint64_t>(std::max<uint64_t>(64, QUEUE_BUDGET / (2 * (uint64_t)P.rec_stride) / 64 * 64),
+                                                 (uint64_t)1 << 23);
+    uint64_t cap = std::max(std::min(chunk, total) * perPath, std::min(pe_chunk, pe_total) * 2);
+    cap = std::max<uint64_t>(cap, 64);
+    if (cap > 0xFFFFFFF0ull) return fail("render: child queue too large");
+    P.qcap = (unsigned)cap;
+    if (!ensure(queue, (size_t)cap * P.rec_stride)) return false;
+    if (ev.empty()) {
+        ev.resize(2);
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned childGrid = (unsigned)std::max(1, ncu) * 32;
+    std::memset(kernel_ms, 0, sizeof(kernel_ms));
+    std::memset(kernel_launches, 0, sizeof(kernel_launches));
+    auto timed = [&](int slot, auto &&fn) -> bool {
+        HIPCHK(hipEventRecord(ev[0], stream));
+        HIPCHK(fn());
+        HIPCHK(hipEventRecord(ev[1], stream));
+        HIPCHK(hipEventSynchronize(ev[1]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        kernel_ms[slot] += ms;
+        kernel_launches[slot]++;
+        return true;
+    };
+    unsigned *qc = static_cast<unsigned *>(qcount.p);
+    unsigned *st = static_cast<unsigned *>(stats.p);
+    unsigned long long *wk = static_cast<unsigned long long *>(work.p);
+    float *A = static_cast<float *>(acc.p), *APE = static_cast<float *>(acc_pe.p);
+    char *Q = static_cast<char *>(queue.p);
+    for (uint64_t s0 = 0; s0 < total; s0 += chunk) {
+        uint64_t n = std::min(chunk, total - s0);
+        HIPCHK(hipMemsetAsync(qc, 0, 4, stream));
+        if (!timed(0, [&] { return launch_ad_paths(S, P, s0, n, A, Q, qc, st, wk, stream); })) return false;
+        if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
+    }
+    for (uint64_t s0 = 0; s0 < pe_total; s0 += pe_chunk) {
+        uint64_t n = std::min(pe_chunk, pe_total - s0);
+        HIPCHK(hipMemsetAsync(qc, 0, 4, stream));
+        if (!timed(2, [&] { return launch_primary(S, P, s0, n, pe_total, Q, qc, st, wk, stream); })) return false;
+        if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
+    }
+    if (!timed(3, [&] { return launch_finalize(S, P, A, APE, dout, st, stream); })) return false;
+    if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, img * 4, hipMemcpyDeviceToHost, stream));
+    unsigned hst[STAT_COUNT];
+    unsigned long long hw[12];
+    HIPCHK(hipMemcpyAsync(hst, st, sizeof(hst), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hw, wk, sizeof(hw), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int i = 0; i < 7; ++i) last_stats[i] = hst[i];
+    last_stats[7] = (int64_t)total;
+    last_stats[8] = 0;
+    last_stats[9] = (int64_t)pe_total;
+    for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
+    if (hst[STAT_QUEUE_OVERFLOW]) return fail("render: child queue overflow");
+    return true;
+}
+
+template <class T> T *as(dtrt_object *o, int tag) {
+    Obj *b = reinterpret_cast<Obj *>(o);
+    if (!b || b->tag != tag) { g_err = "wrong or null object handle"; return nullptr; }
+    return static_cast<T *>(b);
+}
+dtrt_object *handle(Obj *o) { return reinterpret_cast<dtrt_object *>(o); }
+
+void fillSpecD(float (*dst)[3], const float *src, int nd) {
+    for (int j = 0; j < nd; ++j) for (int k = 0; k < 3; ++k) dst[j][k] = src ? src[3 * j + k] : 0.f;
+}
+void fillScalarD(float *dst, const float *src, int nd) { for (int j = 0; j < nd; ++j) dst[j] = src ? src[j] : 0.f; }
+
+bool readVol(const char *fn, std::vector<float> &data, int *res, int &ch, float *mn, float *mx) {
+    FILE *f = std::fopen(fn, "rb");
+    if (!f) return fail(std::string("cannot open volume file ") + fn);
+    char hdr[3];
+    uint8_t ver = 0;
+    int type = 0;
+    bool ok = std::fread(hdr, 1, 3, f) == 3 && std::strncmp(hdr, "VOL", 3) == 0 && std::fread(&ver, 1, 1, f) == 1 &&
+              ver == 3 && std::fread(&type, 4, 1, f) == 1 && type == 1 && std::fread(res, 4, 3, f) == 3 &&
+              std::fread(&ch, 4, 1, f) == 1 && (ch == 1 || ch == 3) && res[0] > 1 && res[1] > 1 && res[2] > 1 &&
+              std::fread(mn, 4, 3, f) == 3 && std::fread(mx, 4, 3, f) == 3;
+    if (ok) {
+        size_t n = (size_t)res[0] * res[1] * res[2] * ch;
+        data.resize(n);
+        ok = std::fread(data.data(), 4, n, f) == n;
+    }
+    std::fclose(f);
+    if (!ok) return fail(std::string("invalid VOL v3 file ") + fn);
+    for (float &x : data) x = std::max(x, 0.0f);
+    return true;
+}
+
+// heterogeneous.h:13-106, gridvolume.h:49-141
+bool finishHetero(Medium &m, const float *to_world, float scale, const float *albedo3, int phase_id, const float *trans,
+                  const float *rot, const float *dscale, const float *dalb, const float *bmin, const float *bmax) {
+    DMedium &d = m.d;
+    int nd = nderNow();
+    d.type = MED_HETEROGENEOUS;
+    d.phase_id = phase_id;
+    double v2w[4][4], w2v[4][4];
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) v2w[i][j] = to_world ? to_world[4 * i + j] : (i == j);
+    if (std::fabs(v2w[3][0]) > 1e-7 || std::fabs(v2w[3][1]) > 1e-7 || std::fabs(v2w[3][2]) > 1e-7 || std::fabs(v2w[3][3] - 1) > 1e-7)
+        return fail("heterogeneous: invalid to-world transformation");
+    if (!invert4(v2w, w2v)) return fail("heterogeneous: singular to-world transformation");
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) d.w2v[4 * i + j] = (float)w2v[i][j];
+    float mx = 0;
+    for (float x : m.density) mx = std::max(mx, x);
+    if (!(mx > 0)) return fail("heterogeneous: density grid is all zero");
+    d.maxDensity = mx;
+    d.scale = scale;
+    d.maxInvDensity = (float)(1.0 / ((double)scale * mx));
+    for (int a = 0; a < 3; ++a) {
+        d.bmin[a] = bmin[a];
+        d.gscale[a] = (float)((double)(d.res[a] - 1) / ((double)bmax[a] - bmin[a]));
+        d.albedo[a] = albedo3 ? albedo3[a] : 0.f;
+    }
+    fillScalarD(d.scale_d, dscale, nd);
+    fillSpecD(d.albedo_d, dalb, nd);
+    fillSpecD(d.trans_d, trans, nd);
+    fillSpecD(d.rot_d, rot, nd);
+    double cx = 0.5 * ((double)bmin[0] + bmax[0]), cy = 0.5 * ((double)bmin[1] + bmax[1]), cz = 0.5 * ((double)bmin[2] + bmax[2]);
+    for (int i = 0; i < 3; ++i) d.center[i] = (float)(v2w[i][0] * cx + v2w[i][1] * cy + v2w[i][2] * cz + v2w[i][3]);
+    for (int i = 0; i < 3; ++i) { d.aabb_min[i] = INFINITY; d.aabb_max[i] = -INFINITY; }
+    for (int c = 0; c < 8; ++c) {
+        double p[3] = {c & 1 ? bmax[0] : bmin[0], c & 2 ? bmax[1] : bmin[1], c & 4 ? bmax[2] : bmin[2]};
+        for (int i = 0; i < 3; ++i) {
+            float w = (float)(v2w[i][0] * p[0] + v2w[i][1] * p[1] + v2w[i][2] * p[2] + v2w[i][3]);
+            d.aabb_min[i] = std::min(d.aabb_min[i], w);
+            d.aabb_max[i] = std::max(d.aabb_max[i], w);
+        }
+    }
+    return true;
+}
+
+
+template <class T> bool devTmp(std::vector<void *> &keep, T *&p, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { g_err = "hipMalloc failed"; return false; }
+    keep.push_back(q);
+    p = static_cast<T *>(q);
+    return true;
+}
+struct TmpFree {
+    std::vector<void *> v;
+    ~TmpFree() { for (void *p : v) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int dtrt_abi_version(void) { return 1; }
+int dtrt_set_nder(int n) {
+    if (!(n == 1 || n == 2 || n == 3 || n == 4 || n == 8 || n == 16)) { g_err = "nder must be one of 1, 2, 3, 4, 8, 16"; return -1; }
+    g_nder = n;
+    return 0;
+}
+int dtrt_nder(void) { return nderNow(); }
+double dtrt_angle_eps(void) { return (double)1e-4f; }
+const char *dtrt_last_error(void) { return g_err.c_str(); }
+void dtrt_destroy(dtrt_object *o) { delete reinterpret_cast<Obj *>(o); }
+
+dtrt_object *dtrt_camera_create(int w, int h, const float *c2w, const float *c2ndc, float clip_near, int med_id, const float *vel) {
+    if (!c2w || !c2ndc || w <= 0 || h <= 0) { g_err = "camera: bad arguments"; return nullptr; }
+    auto c = std::make_unique<Camera>();
+    int nd = nderNow();
+    c->w = w; c->h = h; c->clip_near = clip_near; c->med = med_id; c->nder = nd;
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) c->c2w[i][j] = c2w[4 * i + j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->c2ndc[i][j] = c2ndc[3 * i + j];
+    if (c->c2ndc[0][0] != c->c2ndc[1][1]) { g_err = "camera: cam_to_ndc[0][0] must equal cam_to_ndc[1][1]"; return nullptr; }
+    if (!invert4(c->c2w, c->w2c)) { g_err = "camera: singular cam_to_world"; return nullptr; }
+    double m4[4][4] = {{0}}, i4[4][4];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m4[i][j] = c->c2ndc[i][j];
+    m4[3][3] = 1;
+    if (!invert4(m4, i4)) { g_err = "camera: singular cam_to_ndc"; return nullptr; }
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->ndc2c[i][j] = i4[i][j];
+    c->cpos = D3{c->c2w[0][3], c->c2w[1][3], c->c2w[2][3]};
+    c->cs = D3{c->c2w[0][0], c->c2w[1][0], c->c2w[2][0]};
+    c->ct = D3{c->c2w[0][1], c->c2w[1][1], c->c2w[2][1]};
+    c->cn = D3{c->c2w[0][2], c->c2w[1][2], c->c2w[2][2]};
+    for (int j = 0; j < nd; ++j) {   // camera.cpp:92-111
+        D3 dx = vel ? D3{vel[3 * j], vel[3 * j + 1], vel[3 * j + 2]} : D3{0, 0, 0};
+        D3 dw = vel ? D3{vel[3 * nd + 3 * j], vel[3 * nd + 3 * j + 1], vel[3 * nd + 3 * j + 2]} : D3{0, 0, 0};
+        c->cpos_d[j] = dx;
+        c->cs_d[j] = cross3(dw, c->cs);
+        c->ct_d[j] = cross3(dw, c->ct);
+        c->cn_d[j] = cross3(dw, c->cn);
+    }
+    return handle(c.release());
+}
+int dtrt_camera_set_rect(dtrt_object *cam, int ox, int oy, int w, int h) {
+    Camera *c = as<Camera>(cam, T_CAMERA);
+    if (!c) return -1;
+    c->ox = ox; c->oy = oy; c->cw = w; c->ch = h;
+    if (c->cropped() && (ox < 0 || oy < 0 || ox + w > c->w || oy + h > c->h)) { g_err = "set_rect: crop outside the image"; return -1; }
+    return 0;
+}
+
+dtrt_object *dtrt_shape_create(const float *v, const int *idx, const float *uvs, const float *n, int nv, int nf, int light_id,
+                               int bsdf_id, int med_int, int med_ext, const float *vel) {
+    (void)uvs;   // no BSDF on the path reads texture coordinates
+    if (!v || !idx || nv <= 0 || nf <= 0) { g_err = "shape: bad arguments"; return nullptr; }
+    auto s = std::make_unique<Shape>();
+    int nd = nderNow();
+    s->nv = nv; s->nf = nf; s->nder = nd; s->light_id = light_id; s->bsdf_id = bsdf_id; s->med_int = med_int; s->med_ext = med_ext;
+    s->v.resize(nv);
+    for (int i = 0; i < nv; ++i) s->v[i] = D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    s->f.assign(idx, idx + 3 * (size_t)nf);
+    for (int x : s->f) if (x < 0 || x >= nv) { g_err = "shape: vertex index out of range"; return nullptr; }
+    if (n) {
+        s->n.resize(nv);
+        for (int i = 0; i < nv; ++i) {
+            D3 q{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+            s->n[i] = q * (1.0 / norm3(q));
+        }
+    }
+    s->vd.assign((size_t)nv * nd, D3{0, 0, 0});
+    s->nd.assign(n ? (size_t)nv * nd : 0, D3{0, 0, 0});
+    if (vel)
+        for (int j = 0; j < nd; ++j)
+            for (int i = 0; i < nv; ++i) {
+                const float *p = vel + ((size_t)j * nv + i) * 3;
+                s->vd[(size_t)i * nd + j] = D3{p[0], p[1], p[2]};
+                if (n) {   // shape.cpp:45 reads the same normal block for every lane
+                    const float *q = vel + (size_t)nv * 3 * nd + (size_t)i * 3;
+                    s->nd[(size_t)i * nd + j] = D3{q[0], q[1], q[2]};
+                }
+            }
+    computeFaceNormals(*s);
+    if (!constructEdges(*s)) return nullptr;
+    return handle(s.release());
+}
+
+dtrt_object *dtrt_bsdf_diffuse_create(const float *rgb, const float *d) {
+    if (!rgb) { g_err = "BSDF_diffuse: null reflectance"; return nullptr; }
+    auto b = std::make_unique<Bsdf>();
+    b->d.type = BSDF_DIFFUSE;
+    for (int k = 0; k < 3; ++k) b->d.refl[k] = rgb[k];
+    fillSpecD(b->d.refl_d, d, nderNow());
+    return handle(b.release());
+}
+dtrt_object *dtrt_bsdf_null_create(void) { auto b = std::make_unique<Bsdf>(); b->d.type = BSDF_NULL; return handle(b.release()); }
+dtrt_object *dtrt_bsdf_phong_create(const float *kd, const float *ks, float ex, const float *dkd, const float *dks, const float *dex) {
+    if (!kd || !ks) { g_err = "BSDF_Phong: null reflectance"; return nullptr; }
+    auto b = std::make_unique<Bsdf>();
+    b->d.type = BSDF_PHONG;
+    for (int k = 0; k < 3; ++k) { b->d.kd[k] = kd[k]; b->d.ks[k] = ks[k]; }
+    b->d.ex = ex;
+    double v0 = ((double)kd[0] + kd[1] + kd[2]) / 3, v1 = ((double)ks[0] + ks[1] + ks[2]) / 3;
+    b->d.specW = (float)(v1 / (v0 + v1));
+    int nd = nderNow();
+    fillSpecD(b->d.kd_d, dkd, nd);
+    fillSpecD(b->d.ks_d, dks, nd);
+    fillScalarD(b->d.ex_d, dex, nd);
+    return handle(b.release());
+}
+dtrt_object *dtrt_bsdf_roughdielectric_create(float alpha, float int_ior, float ext_ior, const float *dalpha, const float *deta) {
+    auto b = std::make_unique<Bsdf>();
+    b->d.type = BSDF_ROUGHDIELECTRIC;
+    b->d.alpha = alpha < 1e-4f ? 1e-4f : alpha;   // microfacet.h:11-14
+    b->d.eta = int_ior / ext_ior;
+    int nd = nderNow();
+    fillScalarD(b->d.alpha_d, dalpha, nd);
+    fillScalarD(b->d.eta_d, deta, nd);
+    return handle(b.release());
+}
+dtrt_object *dtrt_phase_hg_create(float g, const float *dg) {
+    auto p = std::make_unique<Phase>();
+    p->d.type = PHASE_HG;
+    p->d.g = g;
+    fillScalarD(p->d.g_d, dg, nderNow());
+    return handle(p.release());
+}
+dtrt_object *dtrt_phase_isotropic_create(void) { auto p = std::make_unique<Phase>(); p->d.type = PHASE_ISO; return handle(p.release()); }
+dtrt_object *dtrt_area_light_create(int shape_id, const float *I, int two_sided) {
+    if (!I) { g_err = "AreaLight: null intensity"; return nullptr; }
+    auto e = std::make_unique<Emitter>();
+    e->d.type = EMIT_AREA; e->d.shape_id = shape_id; e->d.two_sided = two_sided != 0;
+    for (int k = 0; k < 3; ++k) e->d.I[k] = I[k];
+    return handle(e.release());
+}
+dtrt_object *dtrt_area_light_ex_create(int shape_id, const float *I, float kappa, const float *dI, const float *dk) {
+    if (!I) { g_err = "AreaLightEx: null intensity"; return nullptr; }
+    auto e = std::make_unique<Emitter>();
+    e->d.type = EMIT_AREA_EX; e->d.shape_id = shape_id; e->d.kappa = kappa;
+    for (int k = 0; k < 3; ++k) e->d.I[k] = I[k];
+    int nd = nderNow();
+    fillSpecD(e->d.I_d, dI, nd);
+    fillScalarD(e->d.kappa_d, dk, nd);
+    return handle(e.release());
+}
+dtrt_object *dtrt_homogeneous_create(float sigma_t, const float *alb, int phase_id, const float *dsig, const float *dalb) {
+    if (!alb) { g_err = "Homogeneous: null albedo"; return nullptr; }
+    auto m = std::make_unique<Medium>();
+    m->d.type = MED_HOMOGENEOUS; m->d.phase_id = phase_id; m->d.sigma_t = sigma_t;
+    for (int k = 0; k < 3; ++k) m->d.albedo[k] = alb[k];
+    m->d.sampling_weight = std::max(alb[0], std::max(alb[1], alb[2]));
+    int nd = nderNow();
+    fillScalarD(m->d.sigma_t_d, dsig, nd);
+    fillSpecD(m->d.albedo_d, dalb, nd);
+    return handle(m.release());
+}
+dtrt_object *dtrt_heterogeneous_create(const char *file, const float *to_world, float scale, const float *albedo3,
+                                       const char *albedo_file, int phase_id, const float *trans, const float *rot,
+                                       const float *dscale, const float *dalb) {
+    if (!file) { g_err = "Heterogeneous: null density file"; return nullptr; }
+    auto m = std::make_unique<Medium>();
+    int ch = 0;
+    float mn[3], mx[3];
+    if (!readVol(file, m->density, m->d.res, ch, mn, mx)) return nullptr;
+    if (ch != 1) { g_err = "Heterogeneous: density grid must have one channel"; return nullptr; }
+    if (albedo_file) {
+        int ach = 0;
+        float amn[3], amx[3];
+        if (!readVol(albedo_file, m->albedo, m->d.ares, ach, amn, amx)) return nullptr;
+        if (ach != 3) { g_err = "Heterogeneous: albedo grid must have three channels"; return nullptr; }
+        for (int a = 0; a < 3; ++a) { m->d.amin[a] = amn[a]; m->d.agscale[a] = (float)((double)(m->d.ares[a] - 1) / ((double)amx[a] - amn[a])); }
+        albedo3 = nullptr;   // the reference leaves the constant albedo zero in this variant
+    }
+    if (!finishHetero(*m, to_world, scale, albedo3, phase_id, trans, rot, dscale, dalb, mn, mx)) return nullptr;
+    return handle(m.release());
+}
+dtrt_object *dtrt_heterogeneous_create_grid(const float *density, int rx, int ry, int rz, const float *bmin, const float *bmax,
+                                            const float *to_world, float scale, const float *albedo3, int phase_id,
+                                            const float *trans, const float *rot, const float *dscale, const float *dalb) {
+    if (!density || rx < 2 || ry < 2 || rz < 2 || !bmin || !bmax) { g_err = "grid: bad arguments"; return nullptr; }
+    auto m = std::make_unique<Medium>();
+    m->d.res[0] = rx; m->d.res[1] = ry; m->d.res[2] = rz;
+    m->density.assign(density, density + (size_t)rx * ry * rz);
+    for (float &x : m->density) x = std::max(x, 0.0f);
+    if (!finishHetero(*m, to_world, scale, albedo3, phase_id, trans, rot, dscale, dalb, bmin, bmax)) return nullptr;
+    return handle(m.release());
+}
+
+dtrt_object *dtrt_scene_create(dtrt_object *camera, dtrt_object *const *shapes, int ns, dtrt_object *const *bsdfs, int nb,
+                               dtrt_object *const *lights, int nl, dtrt_object *const *phases, int np,
+                               dtrt_object *const *media, int nm) {
+    try {
+        auto sc = std::make_unique<Scene>();
+        Camera *c = as<Camera>(camera, T_CAMERA);
+        if (!c) return nullptr;
+        sc->nder = nderNow();
+        if (c->nder != sc->nder) { g_err = "scene: camera was created with a different nder"; return nullptr; }
+        sc->cam = *c;
+        for (int i = 0; i < ns; ++i) { Shape *s = as<Shape>(shapes[i], T_SHAPE); if (!s) return nullptr; sc->shapes.push_back(s); }
+        for (int i = 0; i < nb; ++i) { Bsdf *b = as<Bsdf>(bsdfs[i], T_BSDF); if (!b) return nullptr; sc->bsdfs.push_back(b); }
+        for (int i = 0; i < nl; ++i) { Emitter *e = as<Emitter>(lights[i], T_EMITTER); if (!e) return nullptr; sc->emitters.push_back(e); }
+        for (int i = 0; i < np; ++i) { Phase *p = as<Phase>(phases[i], T_PHASE); if (!p) return nullptr; sc->phases.push_back(p); }
+        for (int i = 0; i < nm; ++i) { Medium *m = as<Medium>(media[i], T_MEDIUM); if (!m) return nullptr; sc->media.push_back(m); }
+        for (Shape *s : sc->shapes)
+            if (s->nder != sc->nder || s->bsdf_id < 0 || s->bsdf_id >= nb || s->light_id >= nl || s->med_int >= nm ||
+                s->med_ext >= nm) { g_err = "scene: a shape references an object that does not exist"; return nullptr; }
+        for (Emitter *e : sc->emitters)
+            if (e->d.shape_id < 0 || e->d.shape_id >= ns) { g_err = "scene: light shape id out of range"; return nullptr; }
+        for (Medium *m : sc->media)
+            if (m->d.phase_id < 0 || m->d.phase_id >= np) { g_err = "scene: medium phase id out of range"; return nullptr; }
+        if (c->med >= nm) { g_err = "scene: camera medium out of range"; return nullptr; }
+        if (!sc->build()) return nullptr;
+        return handle(sc.release());
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+int dtrt_scene_init_edges(dtrt_object *scene, const float *w) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !w) { if (s) g_err = "initEdges: null weights"; return -1; }
+    return s->initEdges(w) ? 0 : -1;
+}
+
+static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
+    if (integrator != 0) { g_err = "render: only VolPathTracerAD (integrator 0) is provided"; return -2; }
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !o || !img) { if (s) g_err = "render: null argument"; return -1; }
+    try {
+        return s->render(*o, img, dev) ? 0 : -1;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *image) {
+    return renderCommon(integrator, scene, o, image, false);
+}
+int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *image) {
+    return renderCommon(integrator, scene, o, image, true);
+}
+int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !out) return -1;
+    for (int i = 0; i < 10; ++i) out[i] = s->last_stats[i];
+    return 0;
+}
+
+// ------------------------------------------------------------- test / bench hooks
+int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, uint64_t *work12) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    for (int i = 0; i < 4; ++i) { ms4[i] = s->kernel_ms[i]; launches4[i] = s->kernel_launches[i]; }
+    for (int i = 0; i < 12; ++i) work12[i] = s->work_counts[i];
+    return 0;
+}
+int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    int64_t ne = 0;
+    for (Shape *sh : s->shapes) ne += (int64_t)sh->edges.size();
+    out8[0] = s->S.nnodes; out8[1] = s->S.ntris; out8[2] = ne; out8[3] = s->S.npe;
+    std::memcpy(&out8[4], &s->S.pe_sum, 8);
+    out8[5] = s->S.edges_valid; out8[6] = s->S.NV; out8[7] = s->S.NF;
+    return 0;
+}
+int dtrt_test_shape_edges(dtrt_object *shape, int *out_v0v1f0f1, double *out_len, int cap) {
+    Shape *s = as<Shape>(shape, T_SHAPE);
+    if (!s) return -1;
+    int n = (int)s->edges.size();
+    for (int i = 0; i < std::min(n, cap); ++i) {
+        out_v0v1f0f1[4 * i] = s->edges[i].v0; out_v0v1f0f1[4 * i + 1] = s->edges[i].v1;
+        out_v0v1f0f1[4 * i + 2] = s->edges[i].f0; out_v0v1f0f1[4 * i + 3] = s->edges[i].f1;
+        out_len[i] = s->edges[i].len;
+    }
+    return n;
+}
+
+int dtrt_test_trace(dtrt_object *scene, const float *rays8, int n, int *out_face, float *out_t) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || n < 0) return -1;
+    TmpFree tf;
+    float *dr, *dt_;
+    int *df;
+    if (!devTmp(tf.v, dr, (size_t)n * 8) || !devTmp(tf.v, df, n) || !devTmp(tf.v, dt_, n)) return -1;
+    if (hipMemcpy(dr, rays8, (size_t)n * 8 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (launch_trace(s->S, dr, n, df, dt_, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "trace failed"; return -1; }
+    (void)hipMemcpy(out_face, df, (size_t)n * 4, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(out_t, dt_, (size_t)n * 4, hipMemcpyDeviceToHost);
+    return 0;
+}
+int dtrt_test_li(dtrt_object *scene, const float *rays6, int n, int depth, int med, uint64_t seed, float *out3) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || n < 0 || med >= (int)s->media.size()) return -1;
+    TmpFree tf;
+    float *dr, *dout;
+    if (!devTmp(tf.v, dr, (size_t)n * 6) || !devTmp(tf.v, dout, (size_t)n * 3)) return -1;
+    if (hipMemcpy(dr, rays6, (size_t)n * 6 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (launch_li(s->S, dr, n, depth, med, seed, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "li failed"; return -1; }
+    (void)hipMemcpy(out3, dout, (size_t)n * 3 * 4, hipMemcpyDeviceToHost);
+    return 0;
+}
+int dtrt_test_medium(dtrt_object *scene, int med, int kind, const float *seg8, int n, uint64_t seed, float *out) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || med < 0 || med >= (int)s->media.size() || n < 0) { g_err = "medium test: bad arguments"; return -1; }
+    TmpFree tf;
+    float *dr, *dout;
+    if (!devTmp(tf.v, dr, (size_t)n * 8) || !devTmp(tf.v, dout, n)) return -1;
+    if (hipMemcpy(dr, seg8, (size_t)n * 8 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (launch_medium(s->S, med, kind, dr, n, seed, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "medium failed"; return -1; }
+    (void)hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost);
+    return 0;
+}
+int dtrt_test_grid(dtrt_object *scene, int med, const float *pts, int n, float *out) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || med < 0 || med >= (int)s->media.size() || s->media[med]->d.type != MED_HETEROGENEOUS || n < 0) { g_err = "grid test: bad arguments"; return -1; }
+    const int nd = s->nder;
+    TmpFree tf;
+    float *dp, *dout;
+    if (!devTmp(tf.v, dp, (size_t)n * (3 + 3 * nd)) || !devTmp(tf.v, dout, (size_t)n * (1 + nd))) return -1;
+    if (hipMemcpy(dp, pts, (size_t)n * (3 + 3 * nd) * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (launch_grid(s->S, med, dp, n, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "grid failed"; return -1; }
+    (void)hipMemcpy(out, dout, (size_t)n * (1 + nd) * 4, hipMemcpyDeviceToHost);
+    return 0;
+}
+int dtrt_test_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
+    TmpFree tf;
+    uint32_t *d;
+    if (!devTmp(tf.v, d, 4)) return -1;
+    if (launch_philox(ctr4, key2, d, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) { g_err = "philox failed"; return -1; }
+    (void)hipMemcpy(out4, d, 16, hipMemcpyDeviceToHost);
+    return 0;
+}
+
+}  // extern "C"

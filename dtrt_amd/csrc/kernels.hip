@@ -1,0 +1,656 @@
+// gfx950 kernels of the forward-mode AD volumetric path tracer.
+//
+//  k_ad_paths<N>    one thread per AD path sample (pixel x spp). Runs the dual-number
+//                   estimator LiAD (volpathAD.cpp:145-620) iteratively, accumulates the
+//                   interior terms, and pushes every primal sub-path the boundary terms
+//                   need (2 per accepted edge sample, 1 per Lins sample) into an HBM queue.
+//  k_children<N>    one thread per queued sub-path: primal Li / Lins estimate, scattered
+//                   with its per-lane weights into the derivative accumulators.
+//  k_primary<N>     primary-edge samples (integratorAD.cpp:100-165): 2 camera sub-paths each.
+//  k_finalize<N>    divide by spp, drop negative values / non-finite tangents like
+//                   integratorAD.cpp:59-87, add the primary-edge image, write the output.
+#include "tracer.h"
+#include "launch.h"
+
+namespace dt {
+
+__constant__ FrameRot c_frame_rot;
+
+constexpr int BLOCK = 64;
+
+DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
+    // Blocks are dealt round-robin over the 8 XCDs; give each XCD a contiguous run
+    // of logical tiles so neighbouring pixels share that XCD's L2.
+    const uint32_t X = 8;
+    if (nb % X) return b;
+    return (b % X) * (nb / X) + b / X;
+}
+
+DI void pushChild(const RenderParams &P, char *q, unsigned *qcount, unsigned *stats, const ChildHdr &h, const float *w, int nw) {
+    unsigned idx = atomicAdd(qcount, 1u);
+    if (idx >= P.qcap) { atomicAdd(stats + STAT_QUEUE_OVERFLOW, 1u); return; }
+    char *rec = q + (size_t)idx * P.rec_stride;
+    *reinterpret_cast<ChildHdr *>(rec) = h;
+    float *wd = reinterpret_cast<float *>(rec + sizeof(ChildHdr));
+    for (int i = 0; i < nw; ++i) wd[i] = w[i];
+}
+
+// Wave-reduce the per-thread work counters and add them to work[base..base+3]
+// (box tests, triangle tests, density lookups, traced segments).
+DI void flushWork(unsigned long long *work, int base, uint32_t nbox, uint32_t ntri, uint32_t nlk, uint32_t nseg) {
+    uint32_t v[4] = {nbox, ntri, nlk, nseg};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k)
+            if (v[k]) atomicAdd(work + base + k, (unsigned long long)v[k]);
+}
+
+template <int N> DI void cameraRayAD(const DCamera &c, float x, float y, Dv<N> &o, Dv<N> &d) {   // camera.cpp:75-83
+    x /= c.width;
+    y /= c.height;
+    f3 n3{(x - 0.5f) * 2.0f, (y - 0.5f) * (-2.0f) / c.aspect, 1.0f};
+    f3 dc = normalize(f3{c.ndc2c[0] * n3.x + c.ndc2c[1] * n3.y + c.ndc2c[2] * n3.z,
+                         c.ndc2c[3] * n3.x + c.ndc2c[4] * n3.y + c.ndc2c[5] * n3.z,
+                         c.ndc2c[6] * n3.x + c.ndc2c[7] * n3.y + c.ndc2c[8] * n3.z});
+    o = Dv<N>(ld3(c.cpos));
+    d = Dv<N>(ld3(c.cs) * dc.x + ld3(c.ct) * dc.y + ld3(c.cn) * dc.z);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        o.setGrad(j, ld3(c.cpos_d[j]));
+        d.setGrad(j, ld3(c.cs_d[j]) * dc.x + ld3(c.ct_d[j]) * dc.y + ld3(c.cn_d[j]) * dc.z);
+    }
+}
+
+// shape.cpp:302-335
+DI int isSilhouette(const DScene &S, int4 e, f3 p, bool hasNormals) {
+    if (e.z < 0 || e.w < 0) return 2;
+    f3 v0 = ldg3(S.V + 3 * (size_t)e.x), v1 = ldg3(S.V + 3 * (size_t)e.y);
+    bool ff[2];
+    int fs[2] = {e.z, e.w};
+    for (int s = 0; s < 2; ++s) {
+        const int *fi = S.F + 3 * (size_t)fs[s];
+        int k = fi[0];
+        if (k == e.x || k == e.y) k = fi[1];
+        if (k == e.x || k == e.y) k = fi[2];
+        f3 v = ldg3(S.V + 3 * (size_t)k);
+        f3 n = s == 0 ? cross(v0 - v, v1 - v) : cross(v1 - v, v0 - v);
+        ff[s] = dot(n, p - v) > 0.f;   // sign test; normalisation does not change it
+    }
+    if (ff[0] != ff[1]) return 2;
+    return hasNormals ? 0 : 1;
+}
+
+// One boundary (edge) estimate at a path vertex: volpathAD.cpp:207-305 (volume) and
+// :513-616 (interface). Accepted samples push two primal sub-paths with +/- weights.
+template <int N>
+DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const Dv<N> &x, const Hit *its0,
+                 const DBsdf *b, const DPhase *ph, f3 rd_in, f3 thrv, f3 betav, int med, int depth, int pixel,
+                 uint64_t key, uint32_t &child, char *q, unsigned *qcount, unsigned *stats) {
+    const DScene &S = T.S;
+    if (!S.edges_valid) return;
+    const int ne = P.sppse;
+    for (int i = 0; i < ne; ++i) {
+        float rnd = (i + r.next()) / ne, pdfs, pdfe;
+        int sid = sampleReuse(S.shape_edge_cdf, S.nshapes, rnd, &pdfs);
+        const DShape &es = S.shapes[sid];
+        if (es.ne <= 0) continue;
+        int k = sampleReuse(S.edge_cdf + es.ebase + sid, es.ne, rnd, &pdfe);
+        float t = rnd, edgePdf = pdfs * pdfe;
+        int4 e = S.edges[es.ebase + k];
+        int sil = isSilhouette(S, e, x.val(), es.has_normals);
+        if (!(sil > 0 && S.bsdfs[es.bsdf_id].type != BSDF_NULL)) continue;
+        Dv<N> e0 = ldDv<N>(S.V + 3 * (size_t)e.x, S.Vd + (size_t)e.x * N * 3) - x;
+        Dv<N> e1 = ldDv<N>(S.V + 3 * (size_t)e.y, S.Vd + (size_t)e.y * N * 3) - x;
+        f3 c01 = cross(e0.val(), e1.val());
+        f3 nrm = normalize(c01);
+        float l0 = length(e0.val()), l1 = length(e1.val());
+        // angle between the two edge vertices as seen from x (atan2 form: well conditioned in fp32)
+        float theta = atan2f(length(c01) / (l0 * l1), dot(e0.val(), e1.val()) / (l0 * l1));
+        if (theta < EPS || theta > PI - EPS) continue;
+        float sinT = sinf(theta);
+        float w0 = sinf(theta * (1.0f - t)) / sinT / l0, w1 = sinf(theta * t) / sinT / l1;
+        float ws = w0 + w1;
+        w0 /= ws;
+        w1 /= ws;
+        // fp32 rounding: the sum test uses 1e-5 instead of the reference's 1e-7
+        if (w0 < -EPS || w1 < -EPS || fabsf(w0 + w1 - 1.0f) > 1e-5f) { atomicAdd(stats + STAT_BAD_EDGE_WEIGHT, 1u); continue; }
+        Dv<N> w = e0 * w0 + e1 * w1;
+        Df<N> dist = dnorm(w);
+        w = w / dist;
+        edgePdf /= theta;
+        f3 wv = w.val();
+        float orth = 0.f, gmax = 0.f;
+#pragma unroll
+        for (int j = 0; j < N; ++j) { orth = fmaxf(orth, fabsf(dot(w.grad(j), wv))); gmax = fmaxf(gmax, length(w.grad(j))); }
+        if (orth > 1e-4f * (1.0f + gmax)) { atomicAdd(stats + STAT_NONORTHO, 1u); continue; }
+        if (w.gradZero(EPS)) continue;
+        f3 fv;
+        int med1 = med;
+        if (volume) {
+            fv = thrv * phaseEval(*ph, -rd_in, wv);
+        } else {
+            Frame shf(its0->sn);
+            fv = thrv * bsdfEval(*b, its0->wi, shf.toLocal(wv));
+            const DShape &sh0 = shapeOf(S, its0->face);
+            if (isTransition(sh0)) med1 = targetMedium(sh0, its0->gn, wv);
+        }
+        if (allZero(fv, EPS)) continue;
+        float trans = T.evalTransmittance(x.val(), wv, !volume, med1, dist.v, r, depth - 1);
+        if (!(trans > EPS)) continue;
+        float W[N * 3];
+        f3 base = betav * fv * (1.0f / (edgePdf * ne));
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            float nd = dot(nrm, w.grad(j));
+            W[3 * j] = base.x * nd; W[3 * j + 1] = base.y * nd; W[3 * j + 2] = base.z * nd;
+        }
+        ChildHdr h;
+        f3 org = x.val();
+        h.org[0] = org.x; h.org[1] = org.y; h.org[2] = org.z;
+        h.pixel = pixel;
+        h.meta = CHILD_LI | ((depth - 1) << 4) | ((med1 + 1) << 12);
+        h.face = -1;
+        for (int s = 0; s < 2; ++s) {
+            f3 w2 = normalize(s == 0 ? wv - nrm * ANGLE_EPS : wv + nrm * ANGLE_EPS);
+            h.dir[0] = w2.x; h.dir[1] = w2.y; h.dir[2] = w2.z;
+            uint64_t ck = key + (++child);
+            h.key_lo = (uint32_t)ck; h.key_hi = (uint32_t)(ck >> 32);
+            if (s == 1)
+#pragma unroll
+                for (int k2 = 0; k2 < N * 3; ++k2) W[k2] = -W[k2];
+            pushChild(P, q, qcount, stats, h, W, N * 3);
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp, float *acc,
+                                                    char *q, unsigned *qcount, unsigned *stats, unsigned long long *work) {
+    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
+    const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    Ds<N> L;
+    const uint64_t sid = s0 + li;
+    const bool active = li < nsamp;
+    const int pixel = (int)(sid / (uint64_t)P.spp);
+    Rng r;
+    r.init(P.seed, sid);
+    if (active) {
+        int ix = P.cropped ? P.ox + pixel % P.cw : pixel % P.width;
+        int iy = P.cropped ? P.oy + pixel / P.cw : pixel / P.width;
+        float jx = 0.5f, jy = 0.5f;
+        if (P.sppe >= 0) { jx = r.next(); jy = r.next(); }
+        const uint64_t key = (sid + 1) << 20;
+        uint32_t child = 0;
+        Dv<N> ro, rd;
+        cameraRayAD<N>(S.cam, ix + jx, iy + jy, ro, rd);
+        HitAD<N> its;
+        if (T.rayIntersectAD(ro, rd, false, its)) {
+            int med = S.cam.med_id, depth = P.max_bounces, mode = P.mode;
+            Ds<N> beta(f3{1, 1, 1});
+            while (mode != 0) {
+                float r1a = r.next(), r1b = r.next();
+                float r2[4] = {r.next(), r.next(), r.next(), r.next()};
+                float r3[3] = {r.next(), r.next(), r.next()};
+                Ds<N> thr(f3{1, 1, 1});
+                bool vol = false;
+                if (med >= 0) {
+                    Dv<N> ps;
+                    vol = sampleDistanceAD<N>(S.media[med], ro, rd, its.t, r1a, r1b, r, ps, thr);
+                    if (vol) ro = ps;
+                }
+                if (vol) {
+                    if (depth == 0 || thr.isZero(EPS)) break;
+                    const DPhase &ph = S.phases[S.media[med].phase_id];
+                    f3 rdin = rd.val();
+                    if (mode & 8) {   // volume NEE with tangents
+                        f3 wo;
+                        float pdf_nee;
+                        Ds<N> value = T.neePointAD(ro, r2, r, med, depth - 1, wo, pdf_nee);
+                        if (!value.isZero(EPS)) {
+                            Df<N> pv = phaseEvalAD<N>(ph, -rd, wo);
+                            if (pv.v > EPS) L += beta * (thr * value * (pv * misPower(pdf_nee, phaseEval(ph, -rdin, wo))));
+                        }
+                    }
+                    f3 wo = phaseSample(ph, -rdin, r3[0], r3[1]);
+                    Df<N> pv = phaseSampleWeightAD<N>(ph, -rd, wo);
+                    bool cont = false;
+                    HitAD<N> its1;
+                    Ds<N> thr1;
+                    if (pv.v > EPS) {
+                        thr1 = thr * pv;
+                        float pdf_nee = 0.f;
+                        Ds<N> att = T.lookForEmitterAD(ro, Dv<N>(wo), false, r, med, depth - 1, its1, pdf_nee, nullptr, nullptr);
+                        if (!att.isZero(EPS) && (mode & 8)) L += beta * (thr1 * att * misPower(phaseEval(ph, -wo, wo), pdf_nee));
+                        cont = true;
+                    }
+                    if (P.sppse > 0 && (mode & 32))
+                        edgeTerm<N>(T, P, r, true, ro, nullptr, nullptr, &ph, rdin, thr.val(), beta.val(), med, depth, pixel, key,
+                                    child, q, qcount, stats);
+                    if (!cont) break;
+                    beta = beta * thr1;
+                    rd = Dv<N>(wo);
+                    its = its1;
+                    depth -= 1;
+                    mode &= ~1;
+                } else {
+                    if (its.face < 0 || thr.isZero(EPS)) break;
+                    const DShape &sh = shapeOf(S, its.face);
+                    if (isTransition(sh) && targetMedium(sh, its.gn.val(), -rd.val()) != med) {
+                        atomicAdd(stats + STAT_MED_INCONSISTENT, 1u);
+                        break;
+                    }
+                    if (sh.light_id >= 0 && (mode & 1)) L += beta * (thr * emitterEvalAD<N>(S.emitters[sh.light_id], its, -rd));
+                    if (depth == 0) break;
+                    const DBsdf &b = bsdfOf(S, sh);
+                    const bool isnull = b.type == BSDF_NULL;
+                    Hit its0 = its.val();
+                    // Lins boundary term (volpathAD.cpp:321-467): queued as primal sub-paths
+                    if (med >= 0 && !its.t.gradZero(EPS) && (mode & 16)) {
+                        int ns = min(max((int)(P.ddist * its.t.gradAbsMax()), 1), 100);
+                        f3 n = dot(its0.gn, rd.val()) < 0.f ? -its0.gn : its0.gn;
+                        f3 sgs = sigS(S.media[med], its0.p - n * SHADOW_EPS);
+                        f3 base = beta.val() * thr.val() * sgs * (1.0f / ns);
+                        float W[N * 3];
+#pragma unroll
+                        for (int j = 0; j < N; ++j) { W[3 * j] = base.x * its.t.d[j]; W[3 * j + 1] = base.y * its.t.d[j]; W[3 * j + 2] = base.z * its.t.d[j]; }
+                        ChildHdr h;
+                        h.org[0] = its0.p.x; h.org[1] = its0.p.y; h.org[2] = its0.p.z;
+                        f3 rv = rd.val();
+                        h.dir[0] = rv.x; h.dir[1] = rv.y; h.dir[2] = rv.z;
+                        h.pixel = pixel;
+                        h.meta = (isnull ? CHILD_LINS_NULL : CHILD_LINS_SURF) | (depth << 4) | ((med + 1) << 12);
+                        h.face = its0.face;
+                        h.gn[0] = its0.gn.x; h.gn[1] = its0.gn.y; h.gn[2] = its0.gn.z;
+                        h.sn[0] = its0.sn.x; h.sn[1] = its0.sn.y; h.sn[2] = its0.sn.z;
+                        for (int s = 0; s < ns; ++s) {
+                            uint64_t ck = key + (++child);
+                            h.key_lo = (uint32_t)ck; h.key_hi = (uint32_t)(ck >> 32);
+                            pushChild(P, q, qcount, stats, h, W, N * 3);
+                        }
+                    }
+                    FrameAD<N> shf(its.sn);
+                    if (!isnull && (mode & 2)) {   // surface NEE with tangents
+                        Dv<N> woL;
+                        float pdf_nee;
+                        Ds<N> value = T.neeSurfaceAD(its, shf, r2, r, med, depth - 1, woL, pdf_nee);
+                        if (!value.isZero(EPS)) {
+                            Ds<N> bv = bsdfEvalAD<N>(b, its.wi, woL);
+                            L += beta * (thr * value * bv * misPower(pdf_nee, bsdfPdf(b, its0.wi, woL.val())));
+                        }
+                    }
+                    // BSDF sampling (bsdf.cpp:11-21; null.cpp:18-23)
+                    bool cont = false;
+                    Dv<N> nd;
+                    Ds<N> bw(f3{1, 1, 1});
+                    float bpdf = 1.f;
+                    if (isnull) {
+                        nd = rd;   // toWorld(-wi) == incoming direction, tangents included
+                        cont = true;
+                    } else {
+                        f3 wo0;
+                        f3 w0 = bsdfSample(b, its0.wi, r3[0], r3[1], r3[2], wo0, bpdf);
+                        if (!allZero(w0, EPS) && bpdf > EPS) {
+                            Frame shv = shf.val();
+                            f3 ww = shv.toWorld(wo0);
+                            Dv<N> woL = shf.toLocal(Dv<N>(ww));
+                            bw = bsdfEvalAD<N>(b, its.wi, woL) * (1.0f / bpdf);
+                            if (!bw.isZero(EPS)) { nd = Dv<N>(ww); cont = true; }
+                        }
+                    }
+                    HitAD<N> its1;
+                    Ds<N> thr1 = thr;
+                    int med1 = med, mode1 = mode;
+                    Dv<N> no = its.p;
+                    if (cont) {
+                        med1 = isTransition(sh) ? targetMedium(sh, its0.gn, nd.val()) : med;
+                        if (isnull) {
+                            T.rayIntersectAD(no, nd, true, its1);
+                        } else {
+                            float pdf_nee = 0.f;
+                            Ds<N> att = T.lookForEmitterAD(no, nd, true, r, med1, depth - 1, its1, pdf_nee, nullptr, nullptr);
+                            thr1 = thr * bw;
+                            if (!att.isZero(EPS) && (mode & 2)) L += beta * (thr1 * att * misPower(bpdf, pdf_nee));
+                            mode1 = mode & ~1;
+                        }
+                    }
+                    if (!isnull && P.sppse > 0 && (mode & 4))
+                        edgeTerm<N>(T, P, r, false, its.p, &its0, &b, nullptr, f3{0, 0, 0}, thr.val(), beta.val(), med, depth,
+                                    pixel, key, child, q, qcount, stats);
+                    if (!cont) break;
+                    beta = beta * thr1;
+                    ro = no;
+                    rd = nd;
+                    its = its1;
+                    med = med1;
+                    mode = mode1;
+                    depth -= 1;
+                }
+            }
+        }
+    }
+    // accumulate: one atomic per wave when the wave covers one pixel, else per lane
+    const int nd = P.npix;
+    if (P.spp % BLOCK == 0) {
+        float v[3 * (N + 1)];
+        for (int c = 0; c < 3; ++c) {
+            v[c] = L.c[c].v;
+#pragma unroll
+            for (int j = 0; j < N; ++j) v[3 + 3 * j + c] = L.c[c].d[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 3 * (N + 1); ++k)
+            for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+        if (threadIdx.x == 0 && active)
+            for (int k = 0; k < 3 * (N + 1); ++k)
+                if (v[k] != 0.f) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + pixel) * 3 + (k % 3), v[k]);
+    } else if (active) {
+        for (int c = 0; c < 3; ++c) {
+            if (L.c[c].v != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + c, L.c[c].v);
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (L.c[c].d[j] != 0.f) unsafeAtomicAdd(acc + ((size_t)(j + 1) * nd + pixel) * 3 + c, L.c[c].d[j]);
+        }
+    }
+    flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
+                                                    float *acc, float *acc_pe, unsigned long long *work) {
+    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
+    Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    const unsigned n = min(*qcount, P.qcap);
+    const int nd = P.npix;
+    uint32_t nlk = 0;
+    for (unsigned i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const char *rec = q + (size_t)i * P.rec_stride;
+        const ChildHdr h = *reinterpret_cast<const ChildHdr *>(rec);
+        const float *W = reinterpret_cast<const float *>(rec + sizeof(ChildHdr));
+        Rng r;
+        r.init(P.seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
+        int kind = h.meta & 3, depth = (h.meta >> 4) & 255, med = ((h.meta >> 12) & 255) - 1;
+        bool prim = (h.meta & 8) != 0;
+        f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
+        f3 Lv;
+        if (kind == CHILD_LI) {
+            Lv = T.Li(r, org, dir, depth, med, true);
+        } else {
+            Hit its0;
+            its0.face = h.face;
+            its0.p = org;
+            its0.gn = f3{h.gn[0], h.gn[1], h.gn[2]};
+            its0.sn = f3{h.sn[0], h.sn[1], h.sn[2]};
+            its0.wi = Frame(its0.sn).toLocal(-dir);
+            its0.t = 0.f;
+            Lv = T.LinsOnce(r, its0, dir, med, depth, kind == CHILD_LINS_NULL);
+        }
+        float *dst = prim ? acc_pe : acc + (size_t)nd * 3;
+        for (int c = 0; c < 3; ++c) {
+            float lc = Lv[c];
+            if (lc == 0.f) continue;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                float w = W[3 * j + c] * lc;
+                if (w != 0.f) unsafeAtomicAdd(dst + ((size_t)j * nd + h.pixel) * 3 + c, w);
+            }
+        }
+        nlk += r.nlk;
+    }
+    flushWork(work, 4, T.nbox, T.ntri, nlk, T.nseg);
+}
+
+// edge_manager.cpp:112-134 + integratorAD.cpp:126-144 for one primary-edge sample
+template <int N>
+DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t sid, uint64_t total, char *q,
+                          unsigned *qcount, unsigned *stats) {
+    const DScene &S = T.S;
+    const DCamera &c = S.cam;
+    float u = r.next();
+    int idx = sampleReuse(S.pe_cdf, S.npe, u, nullptr);   // edge_manager.cpp:112-134
+    const int stride = 16 + 4 * N;
+    const float *pe = S.pe + (size_t)idx * stride;
+    double px_ = pe[0] + (double)u * (pe[2] - pe[0]), py_ = pe[1] + (double)u * (pe[3] - pe[1]);
+    float pd[N][2];
+    bool nz = false;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const float *g = pe + 16 + 4 * j;
+        pd[j][0] = g[0] + u * (g[2] - g[0]);
+        pd[j][1] = g[1] + u * (g[3] - g[1]);
+        nz = nz || fabsf(pd[j][0]) > EPS || fabsf(pd[j][1]) > EPS;
+    }
+    if (!nz) return;
+    int ipx = (int)floor(px_), ipy = (int)floor(py_);
+    double nx = pe[3] - pe[1], ny = pe[0] - pe[2], nl = sqrt(nx * nx + ny * ny);
+    nx /= nl;
+    ny /= nl;
+    // maxD: distance to the edge point in camera space (edge_manager.cpp:119-133)
+    double fov = c.fov, asp = c.aspect;
+    double tx = (px_ / c.width - 0.5) * 2.0 / fov, ty = (py_ / c.height - 0.5) * (-2.0) / (asp * fov), tz = 1.0;
+    double v0p[3] = {pe[4], pe[5], pe[6]}, v1p[3] = {pe[7], pe[8], pe[9]}, v0c[3] = {pe[10], pe[11], pe[12]}, v1c[3] = {pe[13], pe[14], pe[15]};
+    double d0 = sqrt((tx - v0p[0]) * (tx - v0p[0]) + (ty - v0p[1]) * (ty - v0p[1]) + (tz - v0p[2]) * (tz - v0p[2]));
+    double d01 = sqrt((v1p[0] - v0p[0]) * (v1p[0] - v0p[0]) + (v1p[1] - v0p[1]) * (v1p[1] - v0p[1]) + (v1p[2] - v0p[2]) * (v1p[2] - v0p[2]));
+    double t0 = d0 / d01;
+    double cb = sqrt(v0p[0] * v0p[0] + v0p[1] * v0p[1] + v0p[2] * v0p[2]) / sqrt(v0c[0] * v0c[0] + v0c[1] * v0c[1] + v0c[2] * v0c[2]);
+    double cc = sqrt(v1p[0] * v1p[0] + v1p[1] * v1p[1] + v1p[2] * v1p[2]) / sqrt(v1c[0] * v1c[0] + v1c[1] * v1c[1] + v1c[2] * v1c[2]);
+    double bx = (1 - t0) * cb, by = t0 * cc, t1 = by / (bx + by);   // utils.cpp:256-266
+    double mx = v0c[0] + t1 * (v1c[0] - v0c[0]), my = v0c[1] + t1 * (v1c[1] - v0c[1]), mz = v0c[2] + t1 * (v1c[2] - v0c[2]);
+    float maxD = (float)sqrt(mx * mx + my * my + mz * mz);
+    // camera rays in double: the two sides of the edge are PRIMARY_DELTA pixels apart
+    auto camdir = [&](double x, double y) {
+        x /= c.width;
+        y /= c.height;
+        double n0 = (x - 0.5) * 2.0, n1 = (y - 0.5) * (-2.0) / asp;
+        double dx = c.ndc2c[0] * n0 + c.ndc2c[1] * n1 + c.ndc2c[2], dy = c.ndc2c[3] * n0 + c.ndc2c[4] * n1 + c.ndc2c[5],
+               dz = c.ndc2c[6] * n0 + c.ndc2c[7] * n1 + c.ndc2c[8];
+        double l = sqrt(dx * dx + dy * dy + dz * dz);
+        dx /= l; dy /= l; dz /= l;
+        return f3{(float)(c.c2w[0] * dx + c.c2w[1] * dy + c.c2w[2] * dz), (float)(c.c2w[4] * dx + c.c2w[5] * dy + c.c2w[6] * dz),
+                  (float)(c.c2w[8] * dx + c.c2w[9] * dy + c.c2w[10] * dz)};
+    };
+    f3 org{c.c2w[3], c.c2w[7], c.c2w[11]};
+    float trans = T.evalTransmittance(org, camdir(px_, py_), false, c.med_id, maxD, r, P.max_bounces - 1);
+    if (!(trans > EPS)) return;
+    bool inside = P.cropped ? (ipx >= P.ox && ipx < P.ox + P.cw && ipy >= P.oy && ipy < P.oy + P.ch)
+                            : (ipx >= 0 && ipx < P.width && ipy >= 0 && ipy < P.height);
+    if (!inside) return;
+    int pix = P.cropped ? (ipx - P.ox) + P.cw * (ipy - P.oy) : ipx + P.width * ipy;
+    float W[N * 3];
+    float scale = (float)(S.pe_sum / (double)total);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        float w = (float)(nx * pd[j][0] + ny * pd[j][1]) * scale;
+        W[3 * j] = w; W[3 * j + 1] = w; W[3 * j + 2] = w;
+    }
+    ChildHdr h;
+    h.org[0] = org.x; h.org[1] = org.y; h.org[2] = org.z;
+    h.pixel = pix;
+    h.meta = CHILD_LI | 8 | (P.max_bounces << 4) | ((c.med_id + 1) << 12);
+    h.face = -1;
+    for (int s = 0; s < 2; ++s) {
+        double sg = s == 0 ? -1.0 : 1.0;
+        f3 d = camdir(px_ + sg * nx * PRIMARY_DELTA, py_ + sg * ny * PRIMARY_DELTA);
+        h.dir[0] = d.x; h.dir[1] = d.y; h.dir[2] = d.z;
+        uint64_t ck = ((1ull << 62) | sid) * 4 + s;
+        h.key_lo = (uint32_t)ck; h.key_hi = (uint32_t)(ck >> 32);
+        if (s == 1)
+#pragma unroll
+            for (int k = 0; k < N * 3; ++k) W[k] = -W[k];
+        pushChild(P, q, qcount, stats, h, W, N * 3);
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp, uint64_t total,
+                                                   char *q, unsigned *qcount, unsigned *stats, unsigned long long *work) {
+    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
+    Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    const uint64_t li = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t sid = s0 + li;
+    Rng r;
+    r.init(P.seed, (1ull << 62) | sid);
+    if (li < nsamp) primaryEdgeSample<N>(T, P, r, sid, total, q, qcount, stats);
+    flushWork(work, 8, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
+template <int N>
+__global__ void k_finalize(RenderParams P, const float *acc, const float *acc_pe, float *out, unsigned *stats) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.npix) return;
+    const size_t nd = P.npix;
+    if (P.spp > 0) {
+        float inv = 1.0f / P.spp;
+        float v0 = acc[p * 3] * inv, v1 = acc[p * 3 + 1] * inv, v2 = acc[p * 3 + 2] * inv;
+        if (fminf(v0, fminf(v1, v2)) < 0.f) atomicAdd(stats + STAT_NEG_PIXEL, 1u);
+        else { out[p * 3] = v0; out[p * 3 + 1] = v1; out[p * 3 + 2] = v2; }
+        for (int j = 0; j < N; ++j) {
+            const float *a = acc + ((j + 1) * nd + p) * 3;
+            float g0 = a[0] * inv, g1 = a[1] * inv, g2 = a[2] * inv;
+            if (!isfinite(g0) || !isfinite(g1) || !isfinite(g2)) { atomicAdd(stats + STAT_NONFINITE, 1u); continue; }
+            float *o = out + ((j + 1) * nd + p) * 3;
+            o[0] = g0; o[1] = g1; o[2] = g2;
+        }
+    }
+    if (P.has_pe)
+        for (int j = 0; j < N; ++j) {
+            const float *a = acc_pe + (j * nd + p) * 3;
+            float *o = out + ((j + 1) * nd + p) * 3;
+            o[0] += a[0]; o[1] += a[1]; o[2] += a[2];
+        }
+}
+
+// ---------------------------------------------------------------- test hooks
+__global__ void k_trace(DScene S, const float *rays, int n, int *out_face, float *out_t) {
+    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
+    Tracer<1> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float *r = rays + (size_t)i * 8;
+    f3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+    float t;
+    int f = T.trace(o, d, r[6], r[7], t);
+    out_face[i] = f;
+    if (f >= 0) { Hit h; refineHit(S, f, o, d, h); out_t[i] = h.t; }
+    else out_t[i] = FINF;
+}
+__global__ void k_li(DScene S, const float *rays, int n, int depth, int med, uint64_t seed, float *out) {
+    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
+    Tracer<1> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float *rr = rays + (size_t)i * 6;
+    Rng r;
+    r.init(seed, (uint64_t)i);
+    f3 L = T.Li(r, f3{rr[0], rr[1], rr[2]}, f3{rr[3], rr[4], rr[5]}, depth, med, true);
+    out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+}
+__global__ void k_medium(DScene S, int med, int kind, const float *seg, int n, uint64_t seed, float *out) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float *s = seg + (size_t)i * 8;
+    Rng r;
+    r.init(seed, (uint64_t)i);
+    f3 o{s[0], s[1], s[2]}, d{s[3], s[4], s[5]};
+    const DMedium &m = S.media[med];
+    if (kind == 0) out[i] = mediumTransmittance(m, o, d, s[6], s[7], r);
+    else {
+        float a = r.next(), b = r.next();
+        f3 ps, thr{1, 1, 1};
+        out[i] = sampleDistance(m, o, d, s[7], a, b, r, ps, thr) ? 1.f : 0.f;
+    }
+}
+template <int N> __global__ void k_grid(DScene S, int med, const float *pts, int n, float *out) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float *p = pts + (size_t)i * (3 + 3 * N);
+    Dv<N> x(f3{p[0], p[1], p[2]});
+    for (int j = 0; j < N; ++j) x.setGrad(j, f3{p[3 + 3 * j], p[4 + 3 * j], p[5 + 3 * j]});
+    Df<N> v = lookupDensityAD<N>(S.media[med], x);
+    out[(size_t)i * (1 + N)] = v.v;
+    for (int j = 0; j < N; ++j) out[(size_t)i * (1 + N) + 1 + j] = v.d[j];
+}
+__global__ void k_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t *out) {
+    u4 o = philox4x32_10(u4{c0, c1, c2, c3}, k0, k1);
+    out[0] = o.a; out[1] = o.b; out[2] = o.c; out[3] = o.d;
+}
+
+// ------------------------------------------------------------------ launchers
+#define DT_FOR_N(X) X(1) X(2) X(3) X(4) X(8) X(16)
+
+hipError_t setFrameRotation(const float *m9) { return hipMemcpyToSymbol(HIP_SYMBOL(c_frame_rot), m9, sizeof(FrameRot)); }
+
+template <int N> static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc, char *q,
+                                      unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_ad_paths<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, q, qc, st, wk);
+}
+template <int N> static void launchChildren(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc,
+                                            float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
+    hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, acc, acc_pe, wk);
+}
+template <int N> static void launchPrimary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total,
+                                           char *q, unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, total, q, qc, st, wk);
+}
+template <int N> static void launchFinalize(const RenderParams &P, const float *acc, const float *acc_pe, float *out,
+                                            unsigned *st, hipStream_t sm) {
+    hipLaunchKernelGGL(k_finalize<N>, dim3((P.npix + 255) / 256), dim3(256), 0, sm, P, acc, acc_pe, out, st);
+}
+template <int N> static void launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
+    hipLaunchKernelGGL(k_grid<N>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, med, pts, n, out);
+}
+
+#define DT_DISPATCH(FN, ...)                          \
+    switch (S.nder) {                                 \
+        case 1: FN<1>(__VA_ARGS__); break;            \
+        case 2: FN<2>(__VA_ARGS__); break;            \
+        case 3: FN<3>(__VA_ARGS__); break;            \
+        case 4: FN<4>(__VA_ARGS__); break;            \
+        case 8: FN<8>(__VA_ARGS__); break;            \
+        case 16: FN<16>(__VA_ARGS__); break;          \
+        default: return hipErrorInvalidValue;         \
+    }                                                 \
+    return hipGetLastError();
+
+hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc, char *q, unsigned *qc,
+                           unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    DT_DISPATCH(launchAd, S, P, s0, n, acc, q, qc, st, wk, sm)
+}
+hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc, float *acc, float *acc_pe,
+                           unsigned long long *wk, unsigned grid, hipStream_t sm) {
+    DT_DISPATCH(launchChildren, S, P, q, qc, acc, acc_pe, wk, grid, sm)
+}
+hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,
+                          unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    DT_DISPATCH(launchPrimary, S, P, s0, n, total, q, qc, st, wk, sm)
+}
+hipError_t launch_finalize(const DScene &S, const RenderParams &P, const float *acc, const float *acc_pe, float *out, unsigned *st,
+                           hipStream_t sm) {
+    DT_DISPATCH(launchFinalize, P, acc, acc_pe, out, st, sm)
+}
+hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
+    DT_DISPATCH(launchGrid, S, med, pts, n, out, sm)
+}
+hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm) {
+    hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, face, t);
+    return hipGetLastError();
+}
+hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm) {
+    hipLaunchKernelGGL(k_li, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, depth, med, seed, out);
+    return hipGetLastError();
+}
+hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out, hipStream_t sm) {
+    hipLaunchKernelGGL(k_medium, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, med, kind, seg, n, seed, out);
+    return hipGetLastError();
+}
+hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm) {
+    hipLaunchKernelGGL(k_philox, dim3(1), dim3(1), 0, sm, ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
+    return hipGetLastError();
+}
+
+}  // namespace dt

@@ -1,0 +1,47 @@
+// Host-visible launch interface of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "scene_types.h"
+
+namespace dt {
+
+// Per-render parameters (RenderOptions, integrator.h:10-25, plus the image layout).
+struct RenderParams {
+    uint64_t seed;
+    int spp, max_bounces, sppe, sppse, mode;
+    float ddist;
+    int npix, width, height, ox, oy, cw, ch, cropped, has_pe;
+    int rec_stride;          // bytes per child record (ChildHdr + 12*nder, 16-byte aligned)
+    unsigned qcap;           // child records the queue can hold
+};
+
+// Counters written by the kernels (warnings the reference prints, as counts).
+enum Stat {
+    STAT_NEG_PIXEL = 0, STAT_NONFINITE, STAT_DT_CAP, STAT_BAD_EDGE_WEIGHT, STAT_NONORTHO, STAT_MED_INCONSISTENT,
+    STAT_QUEUE_OVERFLOW, STAT_COUNT
+};
+
+// Screen-space half-gap between the two primary-edge sub-paths, in pixels. The
+// reference uses 1e-4 (integratorAD.cpp:107) at fp64; at fp32 the two rays must be
+// separated by more than direction rounding, so 1e-3 is used.
+constexpr double PRIMARY_DELTA = 1e-3;
+
+hipError_t setFrameRotation(const float *m9);
+// work[0..3] AD paths, [4..7] child sub-paths, [8..11] primary-edge samples:
+// box tests, triangle tests, density lookups, traced segments.
+hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc, char *q, unsigned *qc,
+                           unsigned *st, unsigned long long *work, hipStream_t sm);
+hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc, float *acc, float *acc_pe,
+                           unsigned long long *work, unsigned grid, hipStream_t sm);
+hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,
+                          unsigned *st, unsigned long long *work, hipStream_t sm);
+hipError_t launch_finalize(const DScene &S, const RenderParams &P, const float *acc, const float *acc_pe, float *out, unsigned *st,
+                           hipStream_t sm);
+hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);
+hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm);
+hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out, hipStream_t sm);
+hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);
+hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm);
+
+}  // namespace dt

@@ -1,0 +1,387 @@
+// Ray-level scene services and the two estimators of the gfx950 path tracer:
+//  * Tracer::Li        primal volumetric path tracer      (volpathAD.cpp:50-142)
+//  * Tracer::LinsOnce  one sample of the Lins boundary term (volpathAD.cpp:338-441)
+// plus transmittance through null interfaces (scene.cpp:177-237), emitter lookup
+// (scene.cpp:617-793) and next-event estimation (scene.cpp:399-524).
+#pragma once
+#include "dscene.h"
+
+namespace dt {
+
+DI float misPower(float a, float b) { a *= a; b *= b; return a / (a + b); }   // volpathAD.cpp:43-47
+
+template <int N> struct Tracer {
+    const DScene &S;
+    LStack stk;
+    uint32_t nbox = 0, ntri = 0, nseg = 0;   // work counters: box tests, triangle tests, traced segments
+    DI Tracer(const DScene &s, LStack st) : S(s), stk(st) {}
+
+    DI int trace(f3 o, f3 d, float tnear, float tfar, float &t) {
+        nseg++;
+        return traceClosest(S, o, d, tnear, tfar, t, stk, nbox, ntri);
+    }
+
+    // scene.cpp:538-575
+    DI bool rayIntersect(f3 o, f3 d, bool onSurface, Hit &h) {
+        float t;
+        int f = trace(o, d, onSurface ? SHADOW_EPS : 0.f, FINF, t);
+        if (f < 0) { h.face = -1; h.t = FINF; return false; }
+        refineHit(S, f, o, d, h);
+        return true;
+    }
+    // scene.cpp:577-615
+    DI bool rayIntersectAD(const Dv<N> &o, const Dv<N> &d, bool onSurface, HitAD<N> &h) {
+        float t;
+        int f = trace(o.val(), d.val(), onSurface ? SHADOW_EPS : 0.f, FINF, t);
+        if (f < 0) { h.face = -1; h.t = Df<N>(FINF); return false; }
+        refineHitAD<N>(S, f, o, d, h);
+        return true;
+    }
+
+    // scene.cpp:177-237 — shadow-ray transmittance through null interfaces
+    DI float evalTransmittance(f3 o, f3 d, bool onSurface, int med, float remaining, Rng &r, int max_inter) {
+        float tmin = onSurface ? SHADOW_EPS : 0.f, T = 1.f, tnear = tmin, tfar = (1.0f - SHADOW_EPS) * remaining;
+        int inter = 0;
+        while (remaining > 0.f) {
+            float tmax = remaining, th;
+            int f = trace(o, d, tnear, tfar, th);
+            Hit nh;
+            if (f >= 0) {
+                const DShape &sh = shapeOf(S, f);
+                if (inter == max_inter || bsdfOf(S, sh).type != BSDF_NULL) return 0.f;
+                refineHit(S, f, o, d, nh);
+                if (nh.t < tmax) tmax = nh.t;
+            }
+            if (med >= 0) T *= mediumTransmittance(S.media[med], o, d, tmin, tmax, r);
+            if (f < 0) break;
+            const DShape &sh = shapeOf(S, f);
+            if (isTransition(sh)) med = targetMedium(sh, nh.gn, d);
+            o = o + d * tmax;
+            tnear = tmin = SHADOW_EPS;
+            remaining -= tmax;
+            tfar = (1.0f - SHADOW_EPS) * remaining;
+            inter++;
+        }
+        return T;
+    }
+
+    // scene.cpp:617-701
+    DI f3 lookForEmitter(f3 o, f3 d, bool onSurface, Rng &r, int med, int max_inter, Hit &its, float &pdf_nee) {
+        float tmin = onSurface ? SHADOW_EPS : 0.f, th;
+        f3 scat = o;
+        int f = trace(o, d, tmin, FINF, th);
+        if (f < 0) { its.face = -1; its.t = FINF; return f3{0, 0, 0}; }
+        refineHit(S, f, o, d, its);
+        Hit nh = its;
+        float T = 1.f;
+        int inter = 0;
+        while (true) {
+            const DShape &sh = shapeOf(S, f);
+            float tmax = nh.t;
+            if (med >= 0) T *= mediumTransmittance(S.media[med], o, d, tmin, tmax, r);
+            if (sh.light_id >= 0) {
+                f3 dv = nh.p - scat;
+                float G = nh.wi.z / dot(dv, dv);
+                pdf_nee = S.light_pmf[sh.light_id] / (S.light_areas[sh.light_id] * G);
+                return emitterEval(S.emitters[sh.light_id], nh.gn, nh.sn, -d) * T;
+            }
+            if (inter == max_inter || bsdfOf(S, sh).type != BSDF_NULL) return f3{0, 0, 0};
+            if (isTransition(sh)) med = targetMedium(sh, nh.gn, d);
+            o = o + d * tmax;
+            tmin = SHADOW_EPS;
+            inter++;
+            f = trace(o, d, SHADOW_EPS, FINF, th);
+            if (f < 0) return f3{0, 0, 0};
+            refineHit(S, f, o, d, nh);
+        }
+    }
+
+    // scene.cpp:703-793 — with tangents; reports the far hit's face and AD distance
+    DI Ds<N> lookForEmitterAD(Dv<N> o, const Dv<N> &d, bool onSurface, Rng &r, int med, int max_inter, HitAD<N> &its,
+                              float &pdf_nee, int *far_face, float *far_t) {
+        float tmin = onSurface ? SHADOW_EPS : 0.f, th;
+        f3 scat = o.val(), dv = d.val();
+        int f = trace(scat, dv, tmin, FINF, th);
+        if (f < 0) { its.face = -1; its.t = Df<N>(FINF); return Ds<N>(); }
+        refineHitAD<N>(S, f, o, d, its);
+        HitAD<N> nh = its;
+        Df<N> T(1.f), dist(0.f);
+        int inter = 0;
+        while (true) {
+            const DShape &sh = shapeOf(S, f);
+            Df<N> tmax = nh.t;
+            dist += nh.t;
+            if (med >= 0) T *= mediumTransmittanceAD<N>(S.media[med], o, d, tmin, tmax, r);
+            if (sh.light_id >= 0) {
+                f3 dd = nh.p.val() - scat;
+                float G = nh.wi.z.v / dot(dd, dd);
+                pdf_nee = S.light_pmf[sh.light_id] / (S.light_areas[sh.light_id] * G);
+                if (far_face) { *far_face = f; *far_t = dist.v; }
+                return emitterEvalAD<N>(S.emitters[sh.light_id], nh, -d) * T;
+            }
+            if (inter == max_inter || bsdfOf(S, sh).type != BSDF_NULL) return Ds<N>();
+            if (isTransition(sh)) med = targetMedium(sh, nh.gn.val(), dv);
+            o = o + d * tmax;
+            tmin = SHADOW_EPS;
+            inter++;
+            f = trace(o.val(), dv, SHADOW_EPS, FINF, th);
+            if (f < 0) return Ds<N>();
+            refineHitAD<N>(S, f, o, d, nh);
+        }
+    }
+
+    // scene.cpp:401-410 — light by power, triangle by area, point by (1-sqrt u, sqrt u * v)
+    DI int sampleLight(const float *r4, f3 &pos, f3 &nrm) {
+        int l = upperBound(S.light_cdf, S.nlights, r4[0]);
+        const DEmitter &e = S.emitters[l];
+        const DShape &sh = S.shapes[e.shape_id];
+        int k = upperBound(S.area_cdf_pool + S.area_cdf_begin[l], sh.nf, r4[1]);
+        const int *fi = S.F + 3 * (size_t)(sh.fbase + k);
+        f3 v0 = ldg3(S.V + 3 * (size_t)fi[0]), v1 = ldg3(S.V + 3 * (size_t)fi[1]), v2 = ldg3(S.V + 3 * (size_t)fi[2]);
+        float a = sqrtf(r4[2]), b1 = 1.0f - a, b2 = a * r4[3];
+        f3 e1 = v1 - v0, e2 = v2 - v0;
+        nrm = normalize(cross(e1, e2));
+        pos = v0 + e1 * b1 + e2 * b2;
+        return l;
+    }
+
+    // scene.cpp:399-426 (point in a medium; wo in world space)
+    DI f3 neePoint(f3 ps, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf) {
+        if (S.nlights == 0) return f3{0, 0, 0};
+        f3 lp, ln;
+        int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - ps;
+        pdf = S.light_pmf[l] / S.light_areas[l];
+        if (dot(dir, ln) < 0.f && pdf != 0.f) {
+            float dist = length(dir);
+            dir = dir / dist;
+            float T = evalTransmittance(ps, dir, false, med, dist, r, max_inter);
+            if (T != 0.f) {
+                pdf *= dist * dist / dot(ln, -dir);
+                wo = dir;
+                return emitterEvalNorm(S.emitters[l], ln, -dir) * (T / pdf);
+            }
+        }
+        return f3{0, 0, 0};
+    }
+    // scene.cpp:456-491 (surface point; wo in the shading frame)
+    DI f3 neeSurface(const Hit &its, const Frame &shf, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf,
+                     bool flag) {
+        if (S.nlights == 0) return f3{0, 0, 0};
+        f3 lp, ln;
+        int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - its.p;
+        pdf = S.light_pmf[l] / S.light_areas[l];
+        const DShape &lsh = S.shapes[S.emitters[l].shape_id];
+        int lbt = bsdfOf(S, lsh).type;
+        bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
+        if ((transm || dot(dir, its.sn) > 0.f) && dot(dir, ln) < 0.f && pdf != 0.f) {
+            const DShape &sh = shapeOf(S, its.face);
+            if (isTransition(sh)) med = targetMedium(sh, its.gn, dir);
+            float dist = length(dir);
+            dir = dir / dist;
+            wo = shf.toLocal(dir);
+            if (flag && signum(its.wi.z) != signum(wo.z)) --max_inter;
+            if (max_inter >= 0) {
+                float T = evalTransmittance(its.p, dir, true, med, dist, r, max_inter);
+                if (T != 0.f) {
+                    pdf *= dist * dist / dot(ln, -dir);
+                    return emitterEvalNorm(S.emitters[l], ln, -dir) * (T / pdf);
+                }
+            }
+        }
+        return f3{0, 0, 0};
+    }
+    // scene.cpp:428-454
+    DI Ds<N> neePointAD(const Dv<N> &ps, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf) {
+        if (S.nlights == 0) return Ds<N>();
+        f3 lp, ln;
+        int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - ps.val();
+        if (dot(dir, ln) < 0.f) {
+            float dist = length(dir);
+            dir = dir / dist;
+            HitAD<N> inear;
+            int ff = -1;
+            float ft = 0.f;
+            Ds<N> ret = lookForEmitterAD(ps, Dv<N>(dir), false, r, med, max_inter, inear, pdf, &ff, &ft);
+            if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
+                fabsf(ft - dist) < SHADOW_EPS) {
+                wo = dir;
+                return ret * (1.0f / pdf);
+            }
+        }
+        return Ds<N>();
+    }
+    // scene.cpp:493-524
+    DI Ds<N> neeSurfaceAD(const HitAD<N> &its, const FrameAD<N> &shf, const float *r4, Rng &r, int med, int max_inter,
+                          Dv<N> &wo, float &pdf) {
+        if (S.nlights == 0) return Ds<N>();
+        f3 lp, ln;
+        int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - its.p.val();
+        const DShape &lsh = S.shapes[S.emitters[l].shape_id];
+        int lbt = bsdfOf(S, lsh).type;
+        bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
+        if ((transm || dot(dir, its.sn.val()) > 0.f) && dot(dir, ln) < 0.f) {
+            const DShape &sh = shapeOf(S, its.face);
+            if (isTransition(sh)) med = targetMedium(sh, its.gn.val(), dir);
+            float dist = length(dir);
+            dir = dir / dist;
+            HitAD<N> inear;
+            int ff = -1;
+            float ft = 0.f;
+            Ds<N> ret = lookForEmitterAD(its.p, Dv<N>(dir), true, r, med, max_inter, inear, pdf, &ff, &ft);
+            if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
+                fabsf(ft - dist) < SHADOW_EPS) {
+                wo = shf.toLocal(Dv<N>(dir));
+                return ret * (1.0f / pdf);
+            }
+        }
+        return Ds<N>();
+    }
+
+    // volpathAD.cpp:50-142 — primal path tracer used by every edge / boundary sub-path
+    DI f3 Li(Rng &r, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {
+        Hit its;
+        f3 ret{0, 0, 0}, thr{1, 1, 1};
+        rayIntersect(ro, rd, true, its);
+        for (int depth = 0; depth <= max_depth; ++depth) {
+            int max_inter = max_depth - depth - 1;
+            float r1a = r.next(), r1b = r.next();
+            float r2[4] = {r.next(), r.next(), r.next(), r.next()};
+            float r3[3] = {r.next(), r.next(), r.next()};
+            bool inside = false;
+            if (med >= 0) {
+                f3 ps;
+                inside = sampleDistance(S.media[med], ro, rd, its.t, r1a, r1b, r, ps, thr);
+                if (inside) ro = ps;
+            }
+            if (inside) {
+                if (depth >= max_depth) break;
+                if (allZero(thr, EPS)) break;
+                const DPhase &ph = S.phases[S.media[med].phase_id];
+                f3 wo;
+                float pdf_nee;
+                f3 value = neePoint(ro, r2, r, med, max_inter, wo, pdf_nee);
+                if (!allZero(value, EPS)) {
+                    float pv = phaseEval(ph, -rd, wo);
+                    if (pv > EPS) ret = ret + thr * value * (misPower(pdf_nee, pv) * pv);
+                }
+                wo = phaseSample(ph, -rd, r3[0], r3[1]);
+                rd = wo;
+                f3 att = lookForEmitter(ro, rd, false, r, med, max_inter, its, pdf_nee);
+                if (!allZero(att, EPS)) ret = ret + thr * att * misPower(phaseEval(ph, -rd, wo), pdf_nee);
+                incEmission = false;
+            } else {
+                if (its.face < 0) break;
+                const DShape &sh = shapeOf(S, its.face);
+                if (sh.light_id >= 0 && incEmission) ret = ret + thr * emitterEval(S.emitters[sh.light_id], its.gn, its.sn, -rd);
+                if (depth >= max_depth) break;
+                const DBsdf &b = bsdfOf(S, sh);
+                Frame shf(its.sn);
+                f3 wo;
+                float pdf_nee;
+                if (b.type != BSDF_NULL) {
+                    f3 value = neeSurface(its, shf, r2, r, med, max_inter, wo, pdf_nee, false);
+                    if (!allZero(value, EPS))
+                        ret = ret + thr * value * bsdfEval(b, its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, its.wi, wo));
+                }
+                float bpdf = 0.f;
+                f3 bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+                if (allZero(bw, EPS)) break;
+                wo = shf.toWorld(wo);
+                ro = its.p;
+                rd = wo;
+                if (isTransition(sh)) med = targetMedium(sh, its.gn, wo);
+                if (b.type == BSDF_NULL) {
+                    rayIntersect(ro, rd, true, its);
+                    continue;
+                }
+                f3 att = lookForEmitter(ro, rd, true, r, med, max_inter, its, pdf_nee);
+                thr = thr * bw;
+                if (!allZero(att, 1e-12f)) ret = ret + thr * att * misPower(bpdf, pdf_nee);
+                incEmission = false;
+            }
+        }
+        return ret;
+    }
+
+    // volpathAD.cpp:338-441 — one sample of Lins at a boundary hit (ALTERNATIVE_LINS_BOUNDARY_SAMPLING)
+    DI f3 LinsOnce(Rng &r, const Hit &its0, f3 rdv, int med, int max_depth, bool null_bsdf) {
+        const DShape &shp = shapeOf(S, its0.face);
+        const DBsdf &b = bsdfOf(S, shp);
+        const DPhase &ph = S.phases[S.media[med].phase_id];
+        Frame shf(its0.sn);
+        f3 n = dot(its0.gn, rdv) < 0.f ? -its0.gn : its0.gn;
+        f3 Lins{0, 0, 0}, wo;
+        float pdf_nee, pv = 0.f;
+        float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+        if (null_bsdf) {
+            f3 att = neeSurface(its0, shf, r4, r, med, max_depth - 1, wo, pdf_nee, true);
+            if (!allZero(att, EPS)) {
+                wo = shf.toWorld(wo);
+                pv = phaseEval(ph, -rdv, wo);
+                if (pv > EPS) Lins = Lins + att * (pv * misPower(pdf_nee, pv));
+            }
+            float u1 = r.next(), u2 = r.next();
+            wo = phaseSample(ph, -rdv, u1, u2);
+            pv = 1.f;
+            int md1 = max_depth - 1;
+            if (dot(wo, n) > EPS) {
+                if (shp.light_id >= 0) Lins = Lins + emitterEval(S.emitters[shp.light_id], its0.gn, its0.sn, -wo) * pv;
+                --md1;
+            }
+            if (md1 > 0) {
+                int med1 = isTransition(shp) ? targetMedium(shp, its0.gn, wo) : med;
+                Hit h1;
+                f3 att = lookForEmitter(its0.p, wo, true, r, med1, md1, h1, pdf_nee);
+                if (!allZero(att, EPS)) Lins = Lins + att * (pv * misPower(phaseEval(ph, -rdv, wo), pdf_nee));
+                Lins = Lins + Li(r, its0.p, wo, md1, med1, false) * pv;
+            }
+        } else {
+            f3 att = neeSurface(its0, shf, r4, r, med, max_depth - 1, wo, pdf_nee, false);
+            if (!allZero(att, EPS)) {
+                wo = shf.toWorld(wo);
+                if (dot(wo, n) < -EPS) {
+                    pv = phaseEval(ph, -rdv, wo);
+                    if (pv > EPS) Lins = Lins + att * (pv * misPower(pdf_nee, pv));
+                }
+            }
+            float u1 = r.next(), u2 = r.next();
+            wo = phaseSample(ph, -rdv, u1, u2);
+            pv = 1.f;
+            float wdn = dot(wo, n);
+            if (wdn < -EPS) {
+                Hit h1;
+                f3 att2 = lookForEmitter(its0.p, wo, true, r, med, max_depth - 1, h1, pdf_nee);
+                if (!allZero(att2, EPS)) Lins = Lins + att2 * (pv * misPower(phaseEval(ph, -rdv, wo), pdf_nee));
+                Lins = Lins + Li(r, its0.p, wo, max_depth - 1, med, false) * pv;
+            } else if (wdn > EPS) {
+                if (shp.light_id >= 0) Lins = Lins + emitterEval(S.emitters[shp.light_id], its0.gn, its0.sn, -wo) * pv;
+                if (max_depth > 1) {
+                    Hit itsb = its0;
+                    itsb.wi = shf.toLocal(-wo);
+                    float r4b[4] = {r.next(), r.next(), r.next(), r.next()};
+                    f3 wl;
+                    f3 att3 = neeSurface(itsb, shf, r4b, r, med, max_depth - 2, wl, pdf_nee, false);
+                    if (!allZero(att3, EPS)) Lins = Lins + bsdfEval(b, itsb.wi, wl) * att3 * (pv * misPower(pdf_nee, bsdfPdf(b, itsb.wi, wl)));
+                    float q0 = r.next(), q1 = r.next(), q2 = r.next(), bpdf = 0.f;
+                    f3 bv = bsdfSample(b, itsb.wi, q0, q1, q2, wl, bpdf);
+                    if (!allZero(bv, EPS)) {
+                        f3 thr1 = bv * pv;
+                        f3 wow = shf.toWorld(wl);
+                        int med1 = isTransition(shp) ? targetMedium(shp, itsb.gn, wow) : med;
+                        Hit itsc;
+                        f3 att4 = lookForEmitter(itsb.p, wow, true, r, med1, max_depth - 2, itsc, pdf_nee);
+                        if (!allZero(att4, 1e-12f)) Lins = Lins + thr1 * att4 * misPower(bpdf, pdf_nee);
+                        Lins = Lins + thr1 * Li(r, itsb.p, wow, max_depth - 2, med1, false);
+                    }
+                }
+            }
+        }
+        return Lins;
+    }
+};
+
+}  // namespace dt

@@ -1,0 +1,162 @@
+"""Drop-in ``dtrt`` module backed by libdtrt_mi355x.so (gfx950 HIP).
+
+``import dtrt_amd.dtrt as dtrt`` (or ``sys.modules['dtrt'] = dtrt_amd.dtrt``)
+gives the names the reference's pybind module exports (src/dtrt.cpp:24-155).
+There is no CPU fallback: if the HIP library is missing this import raises.
+"""
+import ctypes as C
+import os
+
+from .abi import bind, env_nder
+
+_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libdtrt_mi355x.so")
+if not os.path.exists(_LIB):
+    raise ImportError("libdtrt_mi355x.so is not built (run `python -m dtrt_amd.build`): " + _LIB)
+
+_m = bind(_LIB, "dtrt_", name="dtrt")
+_lib = _m.lib
+
+float_ptr = _m.float_ptr
+int_ptr = _m.int_ptr
+Spectrum3f = _m.Spectrum3f
+Camera = _m.Camera
+Shape = _m.Shape
+BSDF = _m.BSDF
+BSDF_diffuse = _m.BSDF_diffuse
+BSDF_null = _m.BSDF_null
+BSDF_Phong = _m.BSDF_Phong
+BSDF_roughdielectric = _m.BSDF_roughdielectric
+Phase = _m.Phase
+HG = _m.HG
+Isotropic = _m.Isotropic
+Emitter = _m.Emitter
+AreaLight = _m.AreaLight
+AreaLightEx = _m.AreaLightEx
+Medium = _m.Medium
+Homogeneous = _m.Homogeneous
+Heterogeneous = _m.Heterogeneous
+HeterogeneousGrid = _m.HeterogeneousGrid
+Scene = _m.Scene
+RenderOptions = _m.RenderOptions
+Integrator = _m.Integrator
+VolPathTracerAD = _m.VolPathTracerAD
+angleEps = _m.angleEps
+abi_version = _m.abi_version
+set_nder = _m.set_nder
+fns = _m.fns
+lib = _lib
+prefix = "dtrt_"
+
+
+def __getattr__(name):
+    if name == "nder":   # process-wide lane count (the reference's compile-time nder)
+        return _m.nder
+    raise AttributeError(name)
+
+
+if "DTRT_NDER" in os.environ:
+    set_nder(env_nder())
+
+# ------------------------------------------------------------ test / bench hooks
+_P = C.c_void_p
+_lib.dtrt_test_kernel_times.argtypes = [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
+_lib.dtrt_test_scene_info.argtypes = [_P, C.POINTER(C.c_int64)]
+_lib.dtrt_test_shape_edges.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]
+_lib.dtrt_test_trace.argtypes = [_P, _P, C.c_int, _P, _P]
+_lib.dtrt_test_li.argtypes = [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, _P]
+_lib.dtrt_test_medium.argtypes = [_P, C.c_int, C.c_int, _P, C.c_int, C.c_uint64, _P]
+_lib.dtrt_test_grid.argtypes = [_P, C.c_int, _P, C.c_int, _P]
+_lib.dtrt_test_philox.argtypes = [_P, _P, _P]
+_lib.dtrt_render_stats.argtypes = [_P, C.POINTER(C.c_int64)]
+_lib.dtrt_render_device.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_), _P]
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s: %s" % (what, _lib.dtrt_last_error().decode()))
+
+
+def kernel_times(scene):
+    import numpy as np
+    ms = (C.c_double * 4)()
+    ln = (C.c_int64 * 4)()
+    wk = (C.c_uint64 * 12)()
+    _chk(_lib.dtrt_test_kernel_times(scene._h, ms, ln, wk), "kernel_times")
+    return np.array(ms[:]), np.array(ln[:]), np.array(wk[:], dtype=np.uint64)
+
+
+def render_stats(scene):
+    a = (C.c_int64 * 10)()
+    _chk(_lib.dtrt_render_stats(scene._h, a), "render_stats")
+    keys = ["neg_pixels", "nonfinite_der", "dt_cap", "bad_edge_weight", "nonortho", "med_inconsistent",
+            "queue_overflow", "ad_samples", "children", "pe_samples"]
+    return dict(zip(keys, a[:]))
+
+
+def scene_info(scene):
+    import numpy as np
+    a = (C.c_int64 * 8)()
+    _chk(_lib.dtrt_test_scene_info(scene._h, a), "scene_info")
+    pe_sum = np.array([a[4]], dtype=np.int64).view(np.float64)[0]
+    return dict(nodes=a[0], tris=a[1], edges=a[2], primary_edges=a[3], primary_edge_sum=float(pe_sum),
+                edges_valid=a[5], vertices=a[6], faces=a[7])
+
+
+def shape_edges(shape, cap=1 << 22):
+    import numpy as np
+    e = np.zeros((cap, 4), np.int32)
+    l = np.zeros(cap, np.float64)
+    n = _lib.dtrt_test_shape_edges(shape._h, e.ctypes.data_as(C.POINTER(C.c_int)), l.ctypes.data_as(C.POINTER(C.c_double)), cap)
+    if n < 0:
+        raise RuntimeError(_lib.dtrt_last_error().decode())
+    return e[:n], l[:n]
+
+
+def trace(scene, rays):
+    import numpy as np
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = len(rays)
+    face = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    _chk(_lib.dtrt_test_trace(scene._h, rays.ctypes.data, n, face.ctypes.data, t.ctypes.data), "trace")
+    return face, t
+
+
+def li(scene, rays, max_depth, medium=-1, seed=0):
+    import numpy as np
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    out = np.zeros((len(rays), 3), np.float32)
+    _chk(_lib.dtrt_test_li(scene._h, rays.ctypes.data, len(rays), int(max_depth), int(medium), int(seed), out.ctypes.data), "li")
+    return out
+
+
+def medium_estimate(scene, medium, kind, segs, seed=0):
+    import numpy as np
+    segs = np.ascontiguousarray(segs, dtype=np.float32)
+    out = np.zeros(len(segs), np.float32)
+    _chk(_lib.dtrt_test_medium(scene._h, int(medium), int(kind), segs.ctypes.data, len(segs), int(seed), out.ctypes.data),
+         "medium_estimate")
+    return out
+
+
+def grid_lookup(scene, medium, pts, nder):
+    import numpy as np
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    out = np.zeros((len(pts), 1 + nder), np.float32)
+    _chk(_lib.dtrt_test_grid(scene._h, int(medium), pts.ctypes.data, len(pts), out.ctypes.data), "grid_lookup")
+    return out
+
+
+def philox(ctr, key):
+    import numpy as np
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, np.uint32)
+    _chk(_lib.dtrt_test_philox(c.ctypes.data, k.ctypes.data, o.ctypes.data), "philox")
+    return o
+
+
+def render_device(integrator, scene, options, out_ptr):
+    """``dtrt_render_device``: ``out_ptr`` is a device address (e.g. ``tensor.data_ptr()`` of a HIP tensor)."""
+    st = options._struct()
+    _chk(_lib.dtrt_render_device(integrator, scene._h, C.byref(st), out_ptr), "render_device")

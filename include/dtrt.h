@@ -1,0 +1,139 @@
+/*
+ * dtrt.h — C-ABI of the MI355X (gfx950) differentiable volumetric path tracer.
+ *
+ * Drop-in for the reference's pybind11 module `dtrt` (src/dtrt.cpp:24-155): the
+ * object constructors take the same arguments, with the same meaning and tensor
+ * layouts, as the pybind constructors they replace; the render call fills the same
+ * caller-owned float32[(nder+1)][H][W][3] image (integratorAD.cpp:66-85).
+ *
+ * Conventions
+ *  - Pointers are plain host pointers to float32 / int32 data; NULL where the
+ *    reference accepts a null `float_ptr(0)` (src/dtrt.cpp:27-30, include/ptr.h).
+ *  - Constructors copy their input arrays (shape.cpp:14-32, camera.cpp:13-27);
+ *    a scene references its camera/shapes/materials: keep them alive until the
+ *    scene is destroyed (scene.h:42-46 keeps raw pointers as well).
+ *  - Tangent ("velocity") layouts, with nder = dtrt_nder():
+ *      camera      [2][nder][3]  translations, then rotation axes (camera.cpp:29-37)
+ *      shape       [nder][nv][3] vertex tangents; if normals are given, the normal
+ *                  tangents are read from offset nder*nv*3 for every lane, exactly
+ *                  as shape.cpp:45 does
+ *      spectra     [nder][3]; scalars [nder]
+ *      medium      trans/rot [nder][3] (heterogeneous.h:187-201)
+ *  - Functions returning int give 0 on success and a negative code on failure;
+ *    constructors return NULL on failure. dtrt_last_error() describes the last
+ *    failure of the calling thread. Nothing aborts the host process.
+ *  - One render per scene at a time; different scenes may render concurrently.
+ */
+#ifndef DTRT_MI355X_H
+#define DTRT_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dtrt_object dtrt_object;   /* every handle below is a dtrt_object */
+
+/* RenderOptions (include/integrator.h:10-25; bound at src/dtrt.cpp:104-115). */
+typedef struct dtrt_render_options {
+    uint64_t seed;
+    int32_t spp;           /* num_samples */
+    int32_t max_bounces;
+    int32_t sppe;          /* num_samples_primary_edge (<0: pixel centres, no jitter) */
+    int32_t sppse;         /* num_samples_secondary_edge */
+    int32_t quiet;
+    int32_t mode;          /* EMode bitmask (integratorAD.h:12-24); -1 = all terms */
+    float ddist_coeff;     /* ddistCoeff */
+} dtrt_render_options;
+
+/* Module attributes: `dtrt.nder`, `dtrt.angleEps` (src/dtrt.cpp:152-153). The
+ * reference fixes nder at compile time (include/config.h:5); here it is a process
+ * setting (default: env DTRT_NDER, else 4) that must be chosen before objects are
+ * created. Supported: 1, 2, 3, 4, 8, 16. */
+int dtrt_abi_version(void);
+int dtrt_set_nder(int nder);
+int dtrt_nder(void);
+double dtrt_angle_eps(void);
+const char *dtrt_last_error(void);
+void dtrt_destroy(dtrt_object *obj);
+
+/* dtrt.Camera(w, h, cam_to_world[16] row-major, cam_to_ndc[9], clip_near, med_id, velocities)
+ * — src/dtrt.cpp:35-37, camera.cpp:8-38. */
+dtrt_object *dtrt_camera_create(int width, int height, const float *cam_to_world, const float *cam_to_ndc, float clip_near,
+                                int med_id, const float *velocities);
+/* Camera.set_rect(ox, oy, w, h) — src/dtrt.cpp:37, camera.h:27. */
+int dtrt_camera_set_rect(dtrt_object *camera, int offset_x, int offset_y, int crop_width, int crop_height);
+
+/* dtrt.Shape(vertices[nv][3], indices[nf][3], uvs[nv][2], normals[nv][3], nv, nf, light_id, bsdf_id,
+ *            med_int_id, med_ext_id, velocities) — src/dtrt.cpp:39-50, shape.cpp:10-52. */
+dtrt_object *dtrt_shape_create(const float *vertices, const int *indices, const float *uvs, const float *normals,
+                               int num_vertices, int num_triangles, int light_id, int bsdf_id, int med_int_id,
+                               int med_ext_id, const float *velocities);
+
+/* dtrt.BSDF_diffuse(Spectrum3f[, d_reflectance]) — src/dtrt.cpp:56-58, bsdf/diffuse.h:9-14. */
+dtrt_object *dtrt_bsdf_diffuse_create(const float *reflectance3, const float *d_reflectance);
+/* dtrt.BSDF_null() — src/dtrt.cpp:59-60. */
+dtrt_object *dtrt_bsdf_null_create(void);
+/* dtrt.BSDF_Phong(kd, ks, exponent[, dkd, dks, dexp]) — src/dtrt.cpp:61-63, bsdf/phong.h:9-39. */
+dtrt_object *dtrt_bsdf_phong_create(const float *kd3, const float *ks3, float exponent, const float *d_kd,
+                                    const float *d_ks, const float *d_exponent);
+/* dtrt.BSDF_roughdielectric(alpha, intIOR, extIOR[, d_alpha[, d_eta]]) — src/dtrt.cpp:64-67. */
+dtrt_object *dtrt_bsdf_roughdielectric_create(float alpha, float int_ior, float ext_ior, const float *d_alpha,
+                                              const float *d_eta);
+/* dtrt.HG(g[, d_g]) and dtrt.Isotropic() — src/dtrt.cpp:70-74, include/phase.h:24-79. */
+dtrt_object *dtrt_phase_hg_create(float g, const float *d_g);
+dtrt_object *dtrt_phase_isotropic_create(void);
+/* dtrt.AreaLight(shape_id, intensity_ptr, two_sided) — src/dtrt.cpp:77-78, emitter/area.h:10-15. */
+dtrt_object *dtrt_area_light_create(int shape_id, const float *intensity3, int two_sided);
+/* dtrt.AreaLightEx(shape_id, intensity, kappa[, d_intensity, d_kappa]) — src/dtrt.cpp:79-81. */
+dtrt_object *dtrt_area_light_ex_create(int shape_id, const float *intensity3, float kappa, const float *d_intensity,
+                                       const float *d_kappa);
+/* dtrt.Homogeneous(sigma_t, albedo, phase_id[, d_sigma_t, d_albedo]) — src/dtrt.cpp:84-86. */
+dtrt_object *dtrt_homogeneous_create(float sigma_t, const float *albedo3, int phase_id, const float *d_sigma_t,
+                                     const float *d_albedo);
+/* dtrt.Heterogeneous(density_file, toWorld[16], scale, albedo | albedo_file, phase_id
+ *                    [, trans_vel, rot_vel, d_density[, d_albedo]]) — src/dtrt.cpp:87-93,
+ * medium/heterogeneous.h:13-106. Exactly one of albedo3 / albedo_file is used
+ * (albedo_file != NULL selects the VOL albedo grid). */
+dtrt_object *dtrt_heterogeneous_create(const char *density_file, const float *to_world, float density_scale,
+                                       const float *albedo3, const char *albedo_file, int phase_id,
+                                       const float *trans_velocities, const float *rot_velocities,
+                                       const float *d_density, const float *d_albedo);
+/* In-memory variant of the above (no VOL file): density float32[rz][ry][rx] on the
+ * volume-space box [bmin, bmax]. No counterpart in the reference, which only reads
+ * VOL files (gridvolume.h:88-132); it is for callers that generate the grid. */
+dtrt_object *dtrt_heterogeneous_create_grid(const float *density, int res_x, int res_y, int res_z, const float *bmin3,
+                                            const float *bmax3, const float *to_world, float density_scale,
+                                            const float *albedo3, int phase_id, const float *trans_velocities,
+                                            const float *rot_velocities, const float *d_density,
+                                            const float *d_albedo);
+
+/* dtrt.Scene(camera, shapes, bsdfs, area_lights, phases, media) — src/dtrt.cpp:95-101,
+ * scene.cpp:9-136 (builds the BVH, light and edge distributions, uploads to HBM). */
+dtrt_object *dtrt_scene_create(dtrt_object *camera, dtrt_object *const *shapes, int num_shapes, dtrt_object *const *bsdfs,
+                               int num_bsdfs, dtrt_object *const *area_lights, int num_lights,
+                               dtrt_object *const *phases, int num_phases, dtrt_object *const *media, int num_media);
+/* Scene.initEdges(weights[num_shapes]) — src/dtrt.cpp:102, scene.h:29-32. */
+int dtrt_scene_init_edges(dtrt_object *scene, const float *weights);
+
+/* Integrator.render(scene, options, float_ptr image) — src/dtrt.cpp:148-150,
+ * integratorAD.cpp:19-166. integrator 0 = VolPathTracerAD. `image` is host memory:
+ * float32[(nder+1)][H][W][3] (crop size if cropped), zeroed by the caller; values
+ * and tangents the reference would drop (negative pixel, non-finite tangent) are
+ * left untouched, primary-edge contributions are added. Synchronous. */
+int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image);
+/* Same, with `image` in device memory of the current HIP device (no host copies). */
+int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);
+
+/* Counters of the last render on `scene` (the warnings the reference prints):
+ * out[0] negative pixels, [1] non-finite tangents, [2] delta-tracking caps,
+ * [3] invalid edge weights, [4] non-orthogonal edge tangents, [5] medium
+ * inconsistencies, [6] child-queue overflows, [7] AD path samples,
+ * [8] child sub-paths, [9] primary-edge samples. */
+int dtrt_render_stats(dtrt_object *scene, int64_t *out10);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DTRT_MI355X_H */

@@ -1,0 +1,49 @@
+/*
+ * dtrt_testing.h — measurement and unit-test hooks of libdtrt_mi355x.
+ *
+ * These entry points have no counterpart in the reference. They exist so that the
+ * parity tests can drive individual device stages (BVH closest hit, primal Li,
+ * medium estimators, grid lookups with tangents, the RNG) on explicit inputs, and
+ * so that the benchmark can read per-kernel HIP-event times and work counters of
+ * the last render.
+ */
+#ifndef DTRT_MI355X_TESTING_H
+#define DTRT_MI355X_TESTING_H
+
+#include <stdint.h>
+#include "dtrt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kernel times of the last render (HIP events on the scene's stream), in ms, summed
+ * per kernel: [0] AD paths, [1] child sub-paths, [2] primary edges, [3] finalize;
+ * launches4 = launch counts. work12: [0..3] AD-path kernel, [4..7] child kernel,
+ * [8..11] primary-edge kernel; each = box tests, triangle tests, grid lookups,
+ * traced segments. */
+int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, uint64_t *work12);
+/* [0] BVH nodes, [1] triangles, [2] edges, [3] primary edges, [4] primary-edge
+ * length sum (double bits), [5] secondary edges usable, [6] vertices, [7] faces. */
+int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8);
+/* Edge list of a shape (v0, v1, f0, f1 local ids; f1 = -1 on boundaries) and lengths;
+ * returns the number of edges. */
+int dtrt_test_shape_edges(dtrt_object *shape, int *out_v0v1f0f1, double *out_len, int cap);
+/* Closest hit of n rays {ox,oy,oz,dx,dy,dz,tnear,tfar}: global face id (-1 = miss)
+ * and the refined distance. */
+int dtrt_test_trace(dtrt_object *scene, const float *rays8, int n, int *out_face, float *out_t);
+/* Primal radiance Li of n rays {o, d} (one RNG stream per ray). */
+int dtrt_test_li(dtrt_object *scene, const float *rays6, int n, int max_depth, int medium, uint64_t seed, float *out3);
+/* Medium estimators along segments {o, d, tmin, tmax}: kind 0 ratio-tracking
+ * transmittance, kind 1 delta-tracking scatter indicator. */
+int dtrt_test_medium(dtrt_object *scene, int medium, int kind, const float *seg8, int n, uint64_t seed, float *out);
+/* Density lookups with tangents: points {x, y, z, nder tangent vectors} ->
+ * {value, nder derivatives}. */
+int dtrt_test_grid(dtrt_object *scene, int medium, const float *pts, int n, float *out);
+/* One Philox4x32-10 block. */
+int dtrt_test_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
