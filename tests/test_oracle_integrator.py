@@ -1,0 +1,61 @@
+"""Integrator-level pins of the oracle (volpathAD.cpp, integratorAD.cpp) against
+closed forms and common-random-number finite differences."""
+import math
+
+import numpy as np
+import pytest
+
+from dtrt_amd import scenes
+from tests import scene_fixtures as F
+
+
+@pytest.mark.parametrize("hetero", [False, True])
+def test_furnace(oracle, hetero):
+    d = F.furnace(nder=1, heterogeneous=hetero, res=12, spp=32)
+    img = scenes.render(oracle, d, seed=3)
+    v = img[0, ..., 0]
+    assert abs(v.mean() - 1.0) < 4 * v.std() / math.sqrt(v.size) + 1e-3
+    g = img[1, ..., 0]                   # d/d(density scale) vanishes in a furnace
+    assert abs(g.mean()) < 4 * g.std() / math.sqrt(g.size) + 1e-3
+
+
+def test_direct_lighting_matches_lambert_polygon(oracle):
+    d, lv, rho, Le = F.direct_lighting(res=16, spp=256)
+    img = scenes.render(oracle, d, seed=1)
+    ref = F.direct_expected(d, lv, rho, Le)
+    rel = (img[0, ..., 0].mean() - ref.mean()) / ref.mean()
+    assert abs(rel) < 5e-3
+    assert np.abs(img[0, ..., 0] - ref).max() < 0.25 * ref.max()
+
+
+def test_absorbing_slab_homogeneous_exact(oracle):
+    d, L, dL = F.absorbing_slab(heterogeneous=False)
+    img = scenes.render(oracle, d, seed=0)
+    assert np.allclose(img[0, ..., 0], L, rtol=1e-5, atol=1e-6)
+    assert np.allclose(img[1, ..., 0], dL, rtol=1e-4, atol=1e-5)
+
+
+def test_absorbing_slab_heterogeneous_statistical(oracle):
+    d, L, dL = F.absorbing_slab(heterogeneous=True, spp=512)
+    img = scenes.render(oracle, d, seed=0)
+    assert abs(img[0, ..., 0].mean() / L.mean() - 1) < 5e-3
+    assert abs(img[1, ..., 0].mean() / dL.mean() - 1) < 2e-2
+
+
+def _fd_crn(mod, make, h, seed=5):
+    base = scenes.render(mod, make(0.0), seed=seed)
+    up = scenes.render(mod, make(+h), seed=seed)
+    dn = scenes.render(mod, make(-h), seed=seed)
+    return base[1], (up[0] - dn[0]) / (2 * h)
+
+
+def test_reflectance_tangent_equals_crn_finite_difference(oracle):
+    ad, fd = _fd_crn(oracle, lambda e: F.cfg0_reflectance_lane(res=16, spp=4, rho=0.8 + e), 1e-3)
+    assert np.abs(fd).max() > 1e-2
+    assert np.allclose(ad, fd, rtol=2e-3, atol=2e-4)
+
+
+def test_medium_albedo_tangent_equals_crn_finite_difference(oracle):
+    ad, fd = _fd_crn(oracle, lambda e: F.cfg1_albedo_lane(res=12, spp=4, albedo=0.8 + e, grid_res=12), 1e-3)
+    assert np.abs(fd).max() > 1e-2
+    assert np.allclose(ad, fd, rtol=2e-3, atol=2e-3)
