@@ -1,0 +1,163 @@
+"""Benchmark: forward-mode AD path samples per second on one MI355X.
+
+Workload (one "step"): a full render of the heterogeneous-medium configuration —
+a null cube holding a 64^3 seeded fBm density grid (HG g = 0.3, albedo 0.8), a
+quad light and an enclosing emissive sphere, 512x512 pixels, 128 spp, depth 10,
+3 tangent lanes (medium + boundary translation in x, y, z), 10 secondary edge
+samples per vertex and the primary-edge pass at 4096 samples per unit length.
+Inputs (scene, BVH, grid) are resident in HBM before timing starts; the image is
+written to device memory.
+
+Prints one JSON line (see the keys below). `roofline` is computed for the kernel
+with the largest share of GPU time from its work counters (box tests, triangle
+tests, density lookups, traced segments) and its HIP-event time on the scene's
+stream; `cpu_baseline` times the CPU oracle (a restatement of the reference
+integrator, OpenMP over all host threads) on a centred crop of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["ad_paths", "children", "primary_edges", "finalize"]
+
+
+def algorithmic_bytes(work, nder, spp, npix):
+    """Algorithmic HBM bytes of each kernel from its work counters.
+
+    Per event: 32 B per density lookup (8 fp32 voxels), 36 B per triangle test
+    (3 x float3), 32 B per box test (one fp32 AABB + link), and per traced segment the
+    carried path state (org, dir, throughput, accumulator: 12 scalars) read + written:
+    96 * P bytes, with P = 1 + nder lanes on AD paths and P = 1 on primal sub-paths.
+    The AD kernel also writes its pixel: 12 * P bytes per pixel.
+    """
+    P = 1 + nder
+    out = {}
+    for k, base, p in (("ad_paths", 0, P), ("children", 4, 1), ("primary_edges", 8, 1)):
+        box, tri, lk, seg = [int(x) for x in work[base:base + 4]]
+        out[k] = 32 * lk + 36 * tri + 32 * box + 96 * p * seg
+    out["ad_paths"] += 12 * P * npix
+    out["finalize"] = 4 * 3 * npix * (2 * P + nder)
+    return out
+
+
+def cpu_baseline(desc, target_s=15.0):
+    """Time the CPU oracle on a centred crop sized for roughly `target_s` seconds."""
+    from oracle import pyoracle
+    from dtrt_amd import scenes
+    o = pyoracle.load()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o.set_threads(threads)
+
+    def run(side):
+        d = scenes.config1(nder=desc.nder, res=desc.width, spp=desc.spp,
+                           crop=((desc.width - side) // 2, (desc.height - side) // 2, side, side))
+        sc = scenes.instantiate(o, d)
+        t = time.perf_counter()
+        scenes.render(o, d, seed=0, scene=sc)
+        return time.perf_counter() - t, side * side * d.spp
+
+    side = 8
+    t, n = run(side)
+    side = int(min(96, max(8, side * (target_s / max(t, 1e-3)) ** 0.5)))
+    t, n = run(side)
+    return {"value": n / t, "unit": "AD path samples/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (fp64 restatement of volpathAD/integratorAD, OpenMP) on a centred %dx%d crop of the "
+                      "same 512x512 scene at 128 spp, depth 10, nder 3, sppse 10, sppe 4096: %d samples in %.1f s"
+                      % (side, side, n, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", type=int, default=512)
+    ap.add_argument("--spp", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the renderer runs on a single device (--gpus must be 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+
+    desc = scenes.config1(nder=3, res=args.res, spp=args.spp)
+    scene = scenes.instantiate(G, desc)
+    out = torch.zeros(desc.out_shape(), dtype=torch.float32, device="cuda")
+    opts = G.RenderOptions(0, desc.spp, desc.max_bounces, desc.sppe, desc.sppse, True, -1, desc.ddist)
+
+    def step(seed):
+        out.zero_()
+        opts.seed = seed
+        G.render_device(0, scene, opts, out.data_ptr())
+
+    for w in range(args.warmup):
+        step(1000 + w)
+    torch.cuda.synchronize()
+    ms = np.zeros(4)
+    launches = np.zeros(4, np.int64)
+    work = np.zeros(12, np.uint64)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(s)
+        m, l, w = G.kernel_times(scene)
+        ms += m
+        launches += l
+        work += w
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = G.render_stats(scene)
+    npix = desc.width * desc.height
+    samples = npix * desc.spp * args.steps
+    value = samples / elapsed
+    alg = algorithmic_bytes(work, desc.nder, desc.spp, npix * args.steps)
+    dom = int(np.argmax(ms))
+    name = KERNELS[dom]
+    achieved = alg[name] / (ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(name)
+    rec = {
+        "metric": "AD path samples/s at 1 GPU",
+        "value": value,
+        "unit": "AD path samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm density grid, generated meshes)",
+        "config": {"workload": "heterogeneous-medium AD render: null cube + %d^3 fBm grid, HG 0.3, quad light + "
+                               "emissive env sphere" % 64, "width": desc.width, "height": desc.height,
+                   "spp": desc.spp, "max_bounces": desc.max_bounces, "nder": desc.nder, "sppse": desc.sppse,
+                   "sppe": desc.sppe, "primary_edge_samples_per_step": int(st["pe_samples"])},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_ms": ms[dom] / max(1, launches[dom]),
+                     "algorithmic_bytes_per_launch": alg[name] / max(1, launches[dom])},
+        "kernel_ms_per_step": {k: ms[i] / args.steps for i, k in enumerate(KERNELS)},
+        "work_per_step": {k: [int(x) // args.steps for x in work[b:b + 4]]
+                          for k, b in (("ad_paths", 0), ("children", 4), ("primary_edges", 8))},
+        "render_stats": {k: int(v) for k, v in st.items()},
+    }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(desc)
+        rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

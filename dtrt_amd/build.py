@@ -17,7 +17,8 @@ LIB = os.path.join(LIBDIR, "libdtrt_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DTRT_ARCH", "gfx950")
 
-# -packed-fp32-ops: keep hipcc from forming packed fp32 instructions
+# -packed-fp32-ops: keep hipcc from forming packed fp32 (v_pk_*_f32) instructions,
+# some forms of which have returned wrong results on gfx950 without any error.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include")]
 

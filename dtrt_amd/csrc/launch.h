@@ -22,10 +22,13 @@ enum Stat {
     STAT_QUEUE_OVERFLOW, STAT_COUNT
 };
 
-// Screen-space half-gap between the two primary-edge sub-paths, in pixels. The
-// reference uses 1e-4 (integratorAD.cpp:107) at fp64; at fp32 the two rays must be
-// separated by more than direction rounding, so 1e-3 is used.
-constexpr double PRIMARY_DELTA = 1e-3;
+// Screen-space half-gap between the two primary-edge sub-paths, in pixels
+// (integratorAD.cpp:107). The estimate is not insensitive to it: faces seen nearly
+// edge-on near a silhouette are narrower than 1e-3 px, and a wider gap steps over
+// them (a 24-px view of a subdivided sphere moves by ~7% between 1e-3 and 1e-4).
+// The side rays are built in double and rounded once; 1e-4 px stays several ulp of
+// an fp32 direction apart up to ~1000-px-wide images.
+constexpr double PRIMARY_DELTA = 1e-4;
 
 hipError_t setFrameRotation(const float *m9);
 // work[0..3] AD paths, [4..7] child sub-paths, [8..11] primary-edge samples:

@@ -357,6 +357,7 @@ def instantiate(mod, d: SceneDesc):
                                                m.albedo, m.phase_id, keep(m.trans_vel), keep(m.rot_vel),
                                                keep(m.d_scale), keep(m.d_albedo)))
     scene = mod.Scene(cam, shapes, bsdfs, lights, phases, media)
+    scene.media_handles = media
     if d.edge_weights is not None:
         scene.initEdges(keep(d.edge_weights))
     scene._keep = keep

@@ -1,0 +1,127 @@
+"""Render-level parity of the HIP path: closed forms on the GPU and per-term 3-sigma
+agreement with the oracle on small versions of the benchmark scenes."""
+import math
+
+import numpy as np
+import pytest
+
+from dtrt_amd import scenes
+from tests import scene_fixtures as F
+from tests.helpers import assert_3sigma, block_z, renders
+
+pytestmark = pytest.mark.gpu
+
+# EMode bits (integratorAD.h:12-24)
+EMISSION, SURF_MAIN, SURF_BOUNDARY, VOL_MAIN, VOL_BOUNDARY1, VOL_BOUNDARY2 = 1, 2, 4, 8, 16, 32
+
+
+@pytest.mark.parametrize("hetero", [False, True])
+def test_furnace(gpu, hetero):
+    d = F.furnace(nder=1, heterogeneous=hetero, res=32, spp=64)
+    img = scenes.render(gpu, d, seed=3).astype(np.float64)
+    v, g = img[0, ..., 0], img[1, ..., 0]
+    assert abs(v.mean() - 1.0) < 4 * v.std() / math.sqrt(v.size) + 1e-3
+    assert abs(g.mean()) < 4 * g.std() / math.sqrt(g.size) + 1e-3
+
+
+def test_direct_lighting_matches_lambert_polygon(gpu):
+    d, lv, rho, Le = F.direct_lighting(res=32, spp=1024)
+    img = scenes.render(gpu, d, seed=1)
+    ref = F.direct_expected(d, lv, rho, Le)
+    assert abs(img[0, ..., 0].mean() / ref.mean() - 1) < 3e-3
+    assert np.abs(img[0, ..., 0] - ref).max() < 0.1 * ref.max()
+
+
+def test_absorbing_slab(gpu):
+    d, L, dL = F.absorbing_slab(heterogeneous=False)
+    img = scenes.render(gpu, d, seed=0)
+    assert np.allclose(img[0, ..., 0], L, rtol=1e-4, atol=1e-5)
+    assert np.allclose(img[1, ..., 0], dL, rtol=1e-3, atol=1e-4)
+    d, L, dL = F.absorbing_slab(heterogeneous=True, spp=2048)
+    img = scenes.render(gpu, d, seed=0)
+    assert abs(img[0, ..., 0].mean() / L.mean() - 1) < 3e-3
+    assert abs(img[1, ..., 0].mean() / dL.mean() - 1) < 1e-2
+
+
+def test_primary_edges_translating_emitter(gpu):
+    d, half = F.translating_emitter(res=64)
+    g = scenes.render(gpu, d, seed=0)[1, ..., 0].astype(np.float64)
+    w = d.width // 2
+    assert abs(g[:, :w].sum() / -half - 1) < 5e-3
+    assert abs(g[:, w:].sum() / half - 1) < 5e-3
+
+
+def test_interface_edges_match_lambert_derivative(gpu):
+    d, dE = F.translating_light(res=16, spp=4096)
+    g = scenes.render(gpu, d, seed=0)[1, ..., 0]
+    assert np.corrcoef(g.ravel(), dE.ravel())[0, 1] > 0.999
+    assert np.abs(g - dE).max() < 0.03 * np.abs(dE).max()
+
+
+def _crn_fd(gpu, make, h, seed=5):
+    base = scenes.render(gpu, make(0.0), seed=seed).astype(np.float64)
+    up = scenes.render(gpu, make(+h), seed=seed).astype(np.float64)
+    dn = scenes.render(gpu, make(-h), seed=seed).astype(np.float64)
+    return base[1], (up[0] - dn[0]) / (2 * h)
+
+
+def test_reflectance_tangent_equals_crn_fd(gpu):
+    ad, fd = _crn_fd(gpu, lambda e: F.cfg0_reflectance_lane(res=32, spp=16, rho=0.8 + e), 1e-2)
+    assert np.abs(fd).max() > 1e-2
+    assert np.allclose(ad, fd, rtol=2e-3, atol=2e-3)
+
+
+def test_albedo_tangent_equals_crn_fd(gpu):
+    ad, fd = _crn_fd(gpu, lambda e: F.cfg1_albedo_lane(res=24, spp=16, albedo=0.8 + e), 1e-2)
+    assert np.abs(fd).max() > 1e-2
+    assert np.allclose(ad, fd, rtol=5e-3, atol=5e-3)
+
+
+def _parity(gpu, oracle, d, mode=-1, seeds=16, block=4, **kw):
+    a = renders(gpu, d, range(seeds), mode=mode, **kw)
+    b = renders(oracle, d, range(1000, 1000 + seeds), mode=mode, **kw)
+    return block_z(a, b, block)
+
+
+@pytest.mark.parametrize("mode", [-1, EMISSION | SURF_MAIN, SURF_BOUNDARY])
+def test_cfg0_parity(gpu, oracle, mode):
+    d = scenes.config0(nder=1, res=24, spp=32)
+    z = _parity(gpu, oracle, d, mode=mode, sppe=1024)
+    assert_3sigma(z, what="cfg0 mode %d" % mode)
+
+
+@pytest.mark.parametrize("mode", [-1, EMISSION | VOL_MAIN, VOL_BOUNDARY1, VOL_BOUNDARY2])
+def test_cfg1_parity(gpu, oracle, mode):
+    d = scenes.config1(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
+    z = _parity(gpu, oracle, d, mode=mode, sppe=256)
+    assert_3sigma(z, what="cfg1 mode %d" % mode)
+
+
+@pytest.mark.parametrize("move", ["light", "box"])
+def test_medium_box_parity(gpu, oracle, move):
+    d = F.medium_box(move=move, spp=64, res=12)
+    if move == "box":
+        d.bsdfs = [("null",), ("null",)]
+    z = _parity(gpu, oracle, d, seeds=16, block=3)
+    assert_3sigma(z, what="medium box " + move)
+
+
+def test_cfg2_small_parity(gpu, oracle):
+    """Rough dielectric displaced sphere with shading normals in a medium, 8 lanes."""
+    d = scenes.config2(nder=8, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
+    z = _parity(gpu, oracle, d, seeds=12, block=4, sppe=128, max_bounces=6)
+    assert_3sigma(z, what="cfg2 small", max_mean_z2=1.8)
+
+
+def test_render_into_device_tensor(gpu):
+    import torch
+    d = scenes.config0(nder=1, res=16, spp=8)
+    sc = scenes.instantiate(gpu, d)
+    host = scenes.render(gpu, d, seed=2, scene=sc)
+    out = torch.zeros(d.out_shape(), dtype=torch.float32, device="cuda")
+    gpu.render_device(0, sc, gpu.RenderOptions(2, d.spp, d.max_bounces, d.sppe, d.sppse, True), out.data_ptr())
+    torch.cuda.synchronize()
+    dev = out.cpu().numpy()
+    assert np.allclose(host[0], dev[0], rtol=1e-5, atol=1e-6)
+    st = gpu.render_stats(sc)
+    assert st["queue_overflow"] == 0 and st["ad_samples"] == 16 * 16 * 8
