@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--spp", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the renderer runs on a single device (--gpus must be 1).", file=sys.stderr)
@@ -87,6 +89,15 @@ def main():
 
     import numpy as np
     import torch
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 18
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
 

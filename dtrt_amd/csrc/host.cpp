@@ -258,18 +258,30 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, BvhOut &out) {
             }
         }
         float leafCost = area(nlo, nhi) * t.count;
+        // levels a balanced split of this node still needs before its leaves fit
+        int need = 0;
+        for (int c = t.count; c > MAXLEAF; c = (c + 1) / 2) ++need;
+        const bool tight = t.depth + need >= DTRT_BVH_STACK - 2;
+        if (tight && t.depth + need > DTRT_BVH_STACK - 2) return fail("bvh: too many triangles for the traversal stack");
         int mid;
-        if (bestAxis < 0 || (!mustSplit && bestCost >= leafCost) || t.depth >= DTRT_BVH_STACK - 2) {
-            if (!mustSplit || (bestAxis < 0 && t.count <= MAXLEAF)) { nd.first = t.first; nd.count = t.count; continue; }
-            if (t.depth >= DTRT_BVH_STACK - 2) return fail("bvh: tree deeper than the traversal stack");
-            mid = t.first + t.count / 2;   // identical centroids: split by index
+        if (tight && mustSplit) {
+            // depth budget exhausted: median split along the widest centroid axis keeps
+            // every leaf below at depth <= DTRT_BVH_STACK - 2
+            int ax = 0;
+            for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+            mid = t.first + (t.count + 1) / 2;
+            std::nth_element(idx.begin() + t.first, idx.begin() + mid, idx.begin() + t.first + t.count,
+                             [&](int p, int q) { return cen[3 * p + ax] < cen[3 * q + ax]; });
+        } else if (bestAxis < 0 || (!mustSplit && bestCost >= leafCost)) {
+            if (!mustSplit) { nd.first = t.first; nd.count = t.count; continue; }
+            mid = t.first + (t.count + 1) / 2;   // identical centroids: split by index
         } else {
             float ext = chi[bestAxis] - clo[bestAxis];
             auto it = std::partition(idx.begin() + t.first, idx.begin() + t.first + t.count, [&](int p) {
                 return std::min(NB - 1, (int)((cen[3 * p + bestAxis] - clo[bestAxis]) / ext * NB)) <= bestSplit;
             });
             mid = (int)(it - idx.begin());
-            if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+            if (mid == t.first || mid == t.first + t.count) mid = t.first + (t.count + 1) / 2;
         }
         int l = (int)bn.size();
         bn.push_back(BuildNode());
@@ -482,7 +494,7 @@ struct Scene : Obj {
     ~Scene() override {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
-        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf}) if (b->p) (void)hipFree(b->p);
+        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1]}) if (b->p) (void)hipFree(b->p);
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -495,7 +507,7 @@ struct Scene : Obj {
     std::vector<Medium *> media;
     DScene S;
     std::vector<DevBuf> bufs;
-    DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf;
+    DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf, ovf[2];
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;
     // last render
@@ -504,6 +516,7 @@ struct Scene : Obj {
     int64_t kernel_launches[4] = {0};
     uint64_t work_counts[12] = {0};
     int edge_ebase_total = 0;
+    size_t queue_budget = size_t(6) << 30;   // bytes of queued child records
 
     template <class T> const T *upload(const std::vector<T> &v) {
         if (v.empty()) return nullptr;
@@ -735,7 +748,8 @@ bool Scene::initEdges(const float *weights) {
     return true;
 }
 
-constexpr size_t QUEUE_BUDGET = size_t(6) << 30;   // bytes of child records per chunk
+constexpr uint64_t MAX_CHUNK = uint64_t(1) << 22;    // AD samples per launch
+constexpr uint64_t FIRST_CHUNK = uint64_t(1) << 16;  // before the children-per-sample rate is known
 
 bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device) {
     RenderParams P{};
@@ -746,8 +760,21 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
     if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
     P.rec_stride = (int)((sizeof(ChildHdr) + 12 * nder + 15) / 16 * 16);
     const size_t npix = P.npix, img = (size_t)(nder + 1) * npix * 3;
+    const uint64_t total = (uint64_t)npix * std::max(0, P.spp);
+    uint64_t pe_total = 0;
+    if (P.sppe > 0 && S.npe > 0) pe_total = (uint64_t)std::ceil(S.pe_sum * P.sppe / 128.0) * 128;
+    P.has_pe = pe_total > 0;
+    // One queue of child records for both passes. AD samples are launched in chunks
+    // sized from the children-per-sample rate seen so far; a sample whose pushes
+    // overflow is replayed exactly (see k_ad_paths). A primary-edge sample pushes at
+    // most 2 records, so its chunks never overflow.
+    const uint64_t cap = std::max<uint64_t>(128, std::min<uint64_t>(queue_budget / P.rec_stride, 0xFFFFFFF0ull));
+    const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
+    const uint64_t max_chunk = std::min<uint64_t>(MAX_CHUNK, std::max<uint64_t>(total, 64));
+    P.qcap = (unsigned)cap;
     if (!ensure(acc, img * 4) || !ensure(acc_pe, (size_t)nder * npix * 3 * 4) || !ensure(qcount, 64) ||
-        !ensure(stats, 64 * 4) || !ensure(work, 16 * 8))
+        !ensure(stats, 64 * 4) || !ensure(work, 16 * 8) || !ensure(queue, (size_t)cap * P.rec_stride) ||
+        !ensure(ovf[0], (size_t)max_chunk * 16) || !ensure(ovf[1], (size_t)max_chunk * 16))
         return false;
     HIPCHK(hipMemsetAsync(acc.p, 0, img * 4, stream));
     HIPCHK(hipMemsetAsync(acc_pe.p, 0, (size_t)nder * npix * 3 * 4, stream));
@@ -759,24 +786,6 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
         dout = static_cast<float *>(outbuf.p);
         HIPCHK(hipMemcpyAsync(dout, out, img * 4, hipMemcpyHostToDevice, stream));
     }
-    // child capacity per AD path: 2 sub-paths per edge sample and up to 100 Lins
-    // samples at every vertex (volpathAD.cpp:212-303, 334, 519-614)
-    const bool edges_on = P.sppse > 0 && (P.mode & (4 | 32));
-    const int lins = (P.mode & 16) ? (P.ddist > 0 ? 100 : 1) : 0;
-    const uint64_t perPath = std::max<uint64_t>(1, (uint64_t)std::max(P.max_bounces, 1) * ((edges_on ? 2 * P.sppse : 0) + lins));
-    const uint64_t total = (uint64_t)npix * std::max(0, P.spp);
-    uint64_t chunk = std::max<uint64_t>(64, QUEUE_BUDGET / (perPath * P.rec_stride) / 64 * 64);
-    chunk = std::min<uint64_t>(chunk, (uint64_t)1 << 22);
-    uint64_t pe_total = 0;
-    if (P.sppe > 0 && S.npe > 0) pe_total = (uint64_t)std::ceil(S.pe_sum * P.sppe / 128.0) * 128;
-    P.has_pe = pe_total > 0;
-    const uint64_t pe_chunk = std::min<uint64_t>(std::max<uint64_t>(64, QUEUE_BUDGET / (2 * (uint64_t)P.rec_stride) / 64 * 64),
-                                                 (uint64_t)1 << 23);
-    uint64_t cap = std::max(std::min(chunk, total) * perPath, std::min(pe_chunk, pe_total) * 2);
-    cap = std::max<uint64_t>(cap, 64);
-    if (cap > 0xFFFFFFF0ull) return fail("render: child queue too large");
-    P.qcap = (unsigned)cap;
-    if (!ensure(queue, (size_t)cap * P.rec_stride)) return false;
     if (ev.empty()) {
         ev.resize(2);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
@@ -798,22 +807,54 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
         kernel_launches[slot]++;
         return true;
     };
-    unsigned *qc = static_cast<unsigned *>(qcount.p);
-    unsigned *st = static_cast<unsigned *>(stats.p);
+    unsigned *qc = static_cast<unsigned *>(qcount.p);   // [0] queued records, [1] overflowed samples
+    unsigned *st = static_cast<unsigned *>(stats.p);    // [0..STAT_COUNT) counters, [32..) discarded (replays)
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
     float *A = static_cast<float *>(acc.p), *APE = static_cast<float *>(acc_pe.p);
     char *Q = static_cast<char *>(queue.p);
-    for (uint64_t s0 = 0; s0 < total; s0 += chunk) {
-        uint64_t n = std::min(chunk, total - s0);
-        HIPCHK(hipMemsetAsync(qc, 0, 4, stream));
-        if (!timed(0, [&] { return launch_ad_paths(S, P, s0, n, A, Q, qc, st, wk, stream); })) return false;
+    uint4 *OV[2] = {static_cast<uint4 *>(ovf[0].p), static_cast<uint4 *>(ovf[1].p)};
+    uint64_t children = 0, replayed = 0;
+    auto counts = [&](unsigned h[2]) -> bool {
+        HIPCHK(hipMemcpyAsync(h, qc, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return true;
+    };
+    uint64_t chunk = std::min<uint64_t>(FIRST_CHUNK, max_chunk);
+    for (uint64_t s0 = 0; s0 < total;) {
+        const uint64_t n = std::min(chunk, total - s0);
+        uint64_t delivered = 0;
+        unsigned h[2];
+        HIPCHK(hipMemsetAsync(qc, 0, 8, stream));
+        if (!timed(0, [&] { return launch_ad_paths(S, P, s0, n, nullptr, A, Q, qc, st, OV[0], qc + 1, wk, stream); })) return false;
         if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
+        if (!counts(h)) return false;
+        delivered += std::min<uint64_t>(h[0], cap);
+        int in = 0;
+        for (unsigned nov = h[1]; nov > 0; in ^= 1) {   // each round delivers >= 1 record: terminates
+            replayed += nov;
+            HIPCHK(hipMemsetAsync(qc, 0, 8, stream));
+            if (!timed(0, [&] { return launch_ad_paths(S, P, 0, nov, OV[in], A, Q, qc, st + 32, OV[in ^ 1], qc + 1, wk, stream); }))
+                return false;
+            if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
+            if (!counts(h)) return false;
+            delivered += std::min<uint64_t>(h[0], cap);
+            nov = h[1];
+        }
+        children += delivered;
+        s0 += n;
+        // next chunk: fill ~3/4 of the queue at the observed rate, in whole waves
+        const double rate = (double)delivered / (double)n;
+        uint64_t next = rate > 0 ? (uint64_t)(0.75 * (double)cap / rate) : max_chunk;
+        chunk = std::max<uint64_t>(64, std::min(next, max_chunk) / 64 * 64);
     }
     for (uint64_t s0 = 0; s0 < pe_total; s0 += pe_chunk) {
         uint64_t n = std::min(pe_chunk, pe_total - s0);
         HIPCHK(hipMemsetAsync(qc, 0, 4, stream));
         if (!timed(2, [&] { return launch_primary(S, P, s0, n, pe_total, Q, qc, st, wk, stream); })) return false;
         if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
+        unsigned h[2];
+        if (!counts(h)) return false;
+        children += std::min<uint64_t>(h[0], cap);
     }
     if (!timed(3, [&] { return launch_finalize(S, P, A, APE, dout, st, stream); })) return false;
     if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, img * 4, hipMemcpyDeviceToHost, stream));
@@ -822,12 +863,13 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
     HIPCHK(hipMemcpyAsync(hst, st, sizeof(hst), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hw, wk, sizeof(hw), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    for (int i = 0; i < 7; ++i) last_stats[i] = hst[i];
+    for (int i = 0; i < 6; ++i) last_stats[i] = hst[i];
+    last_stats[6] = (int64_t)replayed;
     last_stats[7] = (int64_t)total;
-    last_stats[8] = 0;
+    last_stats[8] = (int64_t)children;
     last_stats[9] = (int64_t)pe_total;
     for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
-    if (hst[STAT_QUEUE_OVERFLOW]) return fail("render: child queue overflow");
+    if (hst[STAT_QUEUE_OVERFLOW]) return fail("render: primary-edge child queue overflow");
     return true;
 }
 
@@ -1167,6 +1209,13 @@ int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
 }
 
 // ------------------------------------------------------------- test / bench hooks
+int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    s->queue_budget = bytes ? (size_t)bytes : (size_t(6) << 30);
+    return 0;
+}
+
 int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, uint64_t *work12) {
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s) return -1;

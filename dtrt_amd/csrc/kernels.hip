@@ -26,13 +26,37 @@ DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
     return (b % X) * (nb / X) + b / X;
 }
 
-DI void pushChild(const RenderParams &P, char *q, unsigned *qcount, unsigned *stats, const ChildHdr &h, const float *w, int nw) {
+DI bool writeRecord(const RenderParams &P, char *q, unsigned *qcount, const ChildHdr &h, const float *w, int nw) {
     unsigned idx = atomicAdd(qcount, 1u);
-    if (idx >= P.qcap) { atomicAdd(stats + STAT_QUEUE_OVERFLOW, 1u); return; }
+    if (idx >= P.qcap) return false;
     char *rec = q + (size_t)idx * P.rec_stride;
     *reinterpret_cast<ChildHdr *>(rec) = h;
     float *wd = reinterpret_cast<float *>(rec + sizeof(ChildHdr));
     for (int i = 0; i < nw; ++i) wd[i] = w[i];
+    return true;
+}
+
+// Child pushes of one AD path sample. Children are numbered 1, 2, ... in push order
+// and child k draws from RNG stream key + k. When a push finds the queue full the
+// sample stops pushing, holds back its own contribution and is listed with the
+// number of children already delivered; the replay pass re-runs the sample (same
+// streams, hence the same path) and skips those, so every child and every sample
+// contribution is counted exactly once.
+struct Pusher {
+    char *q;
+    unsigned *qcount;
+    uint64_t key;
+    uint32_t child, skip, failed_at;   // failed_at: children delivered before the first failed push, or NO_FAIL
+    static constexpr uint32_t NO_FAIL = 0xFFFFFFFFu;
+};
+
+DI void push(const RenderParams &P, Pusher &u, ChildHdr &h, const float *w, int nw) {
+    const uint32_t k = ++u.child;
+    if (k <= u.skip || u.failed_at != Pusher::NO_FAIL) return;
+    const uint64_t ck = u.key + k;
+    h.key_lo = (uint32_t)ck;
+    h.key_hi = (uint32_t)(ck >> 32);
+    if (!writeRecord(P, u.q, u.qcount, h, w, nw)) u.failed_at = k - 1;
 }
 
 // Wave-reduce the per-thread work counters and add them to work[base..base+3]
@@ -87,7 +111,7 @@ DI int isSilhouette(const DScene &S, int4 e, f3 p, bool hasNormals) {
 template <int N>
 DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const Dv<N> &x, const Hit *its0,
                  const DBsdf *b, const DPhase *ph, f3 rd_in, f3 thrv, f3 betav, int med, int depth, int pixel,
-                 uint64_t key, uint32_t &child, char *q, unsigned *qcount, unsigned *stats) {
+                 Pusher &u, unsigned *stats) {
     const DScene &S = T.S;
     if (!S.edges_valid) return;
     const int ne = P.sppse;
@@ -155,27 +179,37 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
         for (int s = 0; s < 2; ++s) {
             f3 w2 = normalize(s == 0 ? wv - nrm * ANGLE_EPS : wv + nrm * ANGLE_EPS);
             h.dir[0] = w2.x; h.dir[1] = w2.y; h.dir[2] = w2.z;
-            uint64_t ck = key + (++child);
-            h.key_lo = (uint32_t)ck; h.key_hi = (uint32_t)(ck >> 32);
             if (s == 1)
 #pragma unroll
                 for (int k2 = 0; k2 < N * 3; ++k2) W[k2] = -W[k2];
-            pushChild(P, q, qcount, stats, h, W, N * 3);
+            push(P, u, h, W, N * 3);
         }
     }
 }
 
+// Normal pass: samples s0 .. s0+nsamp-1. Replay pass (replay != nullptr): entries
+// {sid lo, sid hi, children delivered, 0} of samples whose pushes overflowed.
+// Samples that overflow (again) are appended to ovf.
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp, float *acc,
-                                                    char *q, unsigned *qcount, unsigned *stats, unsigned long long *work) {
+__global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
+                                                    const uint4 *replay, float *acc, char *q, unsigned *qcount,
+                                                    unsigned *stats, uint4 *ovf, unsigned *ovf_count,
+                                                    unsigned long long *work) {
     __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
     Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
     Ds<N> L;
-    const uint64_t sid = s0 + li;
     const bool active = li < nsamp;
+    uint64_t sid = s0 + li;
+    uint32_t skip = 0;
+    if (replay) {
+        uint4 e = active ? replay[li] : make_uint4(0, 0, 0, 0);
+        sid = (uint64_t)e.x | ((uint64_t)e.y << 32);
+        skip = e.z;
+    }
     const int pixel = (int)(sid / (uint64_t)P.spp);
+    Pusher u{q, qcount, (sid + 1) << 20, 0, skip, Pusher::NO_FAIL};
     Rng r;
     r.init(P.seed, sid);
     if (active) {
@@ -183,8 +217,6 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
         int iy = P.cropped ? P.oy + pixel / P.cw : pixel / P.width;
         float jx = 0.5f, jy = 0.5f;
         if (P.sppe >= 0) { jx = r.next(); jy = r.next(); }
-        const uint64_t key = (sid + 1) << 20;
-        uint32_t child = 0;
         Dv<N> ro, rd;
         cameraRayAD<N>(S.cam, ix + jx, iy + jy, ro, rd);
         HitAD<N> its;
@@ -228,8 +260,8 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
                         cont = true;
                     }
                     if (P.sppse > 0 && (mode & 32))
-                        edgeTerm<N>(T, P, r, true, ro, nullptr, nullptr, &ph, rdin, thr.val(), beta.val(), med, depth, pixel, key,
-                                    child, q, qcount, stats);
+                        edgeTerm<N>(T, P, r, true, ro, nullptr, nullptr, &ph, rdin, thr.val(), beta.val(), med, depth, pixel, u,
+                                    stats);
                     if (!cont) break;
                     beta = beta * thr1;
                     rd = Dv<N>(wo);
@@ -266,11 +298,7 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
                         h.face = its0.face;
                         h.gn[0] = its0.gn.x; h.gn[1] = its0.gn.y; h.gn[2] = its0.gn.z;
                         h.sn[0] = its0.sn.x; h.sn[1] = its0.sn.y; h.sn[2] = its0.sn.z;
-                        for (int s = 0; s < ns; ++s) {
-                            uint64_t ck = key + (++child);
-                            h.key_lo = (uint32_t)ck; h.key_hi = (uint32_t)(ck >> 32);
-                            pushChild(P, q, qcount, stats, h, W, N * 3);
-                        }
+                        for (int s = 0; s < ns; ++s) push(P, u, h, W, N * 3);
                     }
                     FrameAD<N> shf(its.sn);
                     if (!isnull && (mode & 2)) {   // surface NEE with tangents
@@ -319,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
                     }
                     if (!isnull && P.sppse > 0 && (mode & 4))
                         edgeTerm<N>(T, P, r, false, its.p, &its0, &b, nullptr, f3{0, 0, 0}, thr.val(), beta.val(), med, depth,
-                                    pixel, key, child, q, qcount, stats);
+                                    pixel, u, stats);
                     if (!cont) break;
                     beta = beta * thr1;
                     ro = no;
@@ -332,14 +360,17 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
             }
         }
     }
+    const bool done = u.failed_at == Pusher::NO_FAIL;
+    if (active && !done) ovf[atomicAdd(ovf_count, 1u)] = make_uint4((uint32_t)sid, (uint32_t)(sid >> 32), u.failed_at, 0u);
     // accumulate: one atomic per wave when the wave covers one pixel, else per lane
     const int nd = P.npix;
-    if (P.spp % BLOCK == 0) {
+    const bool put = active && done;
+    if (P.spp % BLOCK == 0 && !replay) {
         float v[3 * (N + 1)];
         for (int c = 0; c < 3; ++c) {
-            v[c] = L.c[c].v;
+            v[c] = put ? L.c[c].v : 0.f;
 #pragma unroll
-            for (int j = 0; j < N; ++j) v[3 + 3 * j + c] = L.c[c].d[j];
+            for (int j = 0; j < N; ++j) v[3 + 3 * j + c] = put ? L.c[c].d[j] : 0.f;
         }
 #pragma unroll
         for (int k = 0; k < 3 * (N + 1); ++k)
@@ -347,7 +378,7 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
         if (threadIdx.x == 0 && active)
             for (int k = 0; k < 3 * (N + 1); ++k)
                 if (v[k] != 0.f) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + pixel) * 3 + (k % 3), v[k]);
-    } else if (active) {
+    } else if (put) {
         for (int c = 0; c < 3; ++c) {
             if (L.c[c].v != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + c, L.c[c].v);
 #pragma unroll
@@ -366,39 +397,75 @@ __global__ __launch_bounds__(BLOCK) void k_children(DScene S, RenderParams P, co
     const unsigned n = min(*qcount, P.qcap);
     const int nd = P.npix;
     uint32_t nlk = 0;
-    for (unsigned i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        const char *rec = q + (size_t)i * P.rec_stride;
-        const ChildHdr h = *reinterpret_cast<const ChildHdr *>(rec);
-        const float *W = reinterpret_cast<const float *>(rec + sizeof(ChildHdr));
-        Rng r;
-        r.init(P.seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
-        int kind = h.meta & 3, depth = (h.meta >> 4) & 255, med = ((h.meta >> 12) & 255) - 1;
-        bool prim = (h.meta & 8) != 0;
-        f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
-        f3 Lv;
-        if (kind == CHILD_LI) {
-            Lv = T.Li(r, org, dir, depth, med, true);
-        } else {
-            Hit its0;
-            its0.face = h.face;
-            its0.p = org;
-            its0.gn = f3{h.gn[0], h.gn[1], h.gn[2]};
-            its0.sn = f3{h.sn[0], h.sn[1], h.sn[2]};
-            its0.wi = Frame(its0.sn).toLocal(-dir);
-            its0.t = 0.f;
-            Lv = T.LinsOnce(r, its0, dir, med, depth, kind == CHILD_LINS_NULL);
-        }
-        float *dst = prim ? acc_pe : acc + (size_t)nd * 3;
-        for (int c = 0; c < 3; ++c) {
-            float lc = Lv[c];
-            if (lc == 0.f) continue;
+    const unsigned stride = gridDim.x * BLOCK;
+    // every lane of a wave runs the same number of iterations (idle lanes carry zero
+    // weights), so the wave-wide reductions below see all 64 lanes
+    const unsigned wave0 = (blockIdx.x * BLOCK + threadIdx.x) & ~63u;
+    for (unsigned base = wave0; base < n; base += stride) {
+        const unsigned i = base + (threadIdx.x & 63);
+        const bool live = i < n;
+        f3 Lv{0.f, 0.f, 0.f};
+        float W[N * 3];
+        int pix = -1;
+        bool prim = false;
+        if (live) {
+            const char *rec = q + (size_t)i * P.rec_stride;
+            const ChildHdr h = *reinterpret_cast<const ChildHdr *>(rec);
+            const float *Wg = reinterpret_cast<const float *>(rec + sizeof(ChildHdr));
 #pragma unroll
-            for (int j = 0; j < N; ++j) {
-                float w = W[3 * j + c] * lc;
-                if (w != 0.f) unsafeAtomicAdd(dst + ((size_t)j * nd + h.pixel) * 3 + c, w);
+            for (int k = 0; k < N * 3; ++k) W[k] = Wg[k];
+            Rng r;
+            r.init(P.seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
+            int kind = h.meta & 3, depth = (h.meta >> 4) & 255, med = ((h.meta >> 12) & 255) - 1;
+            prim = (h.meta & 8) != 0;
+            pix = h.pixel;
+            f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
+            if (kind == CHILD_LI) {
+                Lv = T.Li(r, org, dir, depth, med, true);
+            } else {
+                Hit its0;
+                its0.face = h.face;
+                its0.p = org;
+                its0.gn = f3{h.gn[0], h.gn[1], h.gn[2]};
+                its0.sn = f3{h.sn[0], h.sn[1], h.sn[2]};
+                its0.wi = Frame(its0.sn).toLocal(-dir);
+                its0.t = 0.f;
+                Lv = T.LinsOnce(r, its0, dir, med, depth, kind == CHILD_LINS_NULL);
+            }
+            nlk += r.nlk;
+        }
+        // lanes that share lane 0's pixel and target image are summed over the wave
+        // and added with one atomic per channel; the others add their own
+        const int pix0 = __shfl(pix, 0);
+        const bool prim0 = __shfl(prim, 0);
+        const bool grp = live && pix == pix0 && prim == prim0;
+        float v[N * 3];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            v[3 * j] = W[3 * j] * Lv.x;
+            v[3 * j + 1] = W[3 * j + 1] * Lv.y;
+            v[3 * j + 2] = W[3 * j + 2] * Lv.z;
+        }
+        if (live && !grp) {
+            float *dst = prim ? acc_pe : acc + (size_t)nd * 3;
+#pragma unroll
+            for (int k = 0; k < N * 3; ++k)
+                if (v[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix) * 3 + (k % 3), v[k]);
+        }
+        if (pix0 >= 0) {
+            float sv[N * 3];
+#pragma unroll
+            for (int k = 0; k < N * 3; ++k) {
+                sv[k] = grp ? v[k] : 0.f;
+                for (int off = 32; off > 0; off >>= 1) sv[k] += __shfl_xor(sv[k], off);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                float *dst = prim0 ? acc_pe : acc + (size_t)nd * 3;
+#pragma unroll
+                for (int k = 0; k < N * 3; ++k)
+                    if (sv[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix0) * 3 + (k % 3), sv[k]);
             }
         }
-        nlk += r.nlk;
     }
     flushWork(work, 4, T.nbox, T.ntri, nlk, T.nseg);
 }
@@ -480,7 +547,8 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t 
         if (s == 1)
 #pragma unroll
             for (int k = 0; k < N * 3; ++k) W[k] = -W[k];
-        pushChild(P, q, qcount, stats, h, W, N * 3);
+        // the queue holds two records per primary-edge sample of the launch: cannot fail
+        if (!writeRecord(P, q, qcount, h, W, N * 3)) atomicAdd(stats + STAT_QUEUE_OVERFLOW, 1u);
     }
 }
 
@@ -583,10 +651,11 @@ __global__ void k_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uin
 
 hipError_t setFrameRotation(const float *m9) { return hipMemcpyToSymbol(HIP_SYMBOL(c_frame_rot), m9, sizeof(FrameRot)); }
 
-template <int N> static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc, char *q,
-                                      unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+template <int N> static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,
+                                      float *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,
+                                      unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_ad_paths<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, q, qc, st, wk);
+    hipLaunchKernelGGL(k_ad_paths<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf, ovf_count, wk);
 }
 template <int N> static void launchChildren(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc,
                                             float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
@@ -617,9 +686,10 @@ template <int N> static void launchGrid(const DScene &S, int med, const float *p
     }                                                 \
     return hipGetLastError();
 
-hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc, char *q, unsigned *qc,
-                           unsigned *st, unsigned long long *wk, hipStream_t sm) {
-    DT_DISPATCH(launchAd, S, P, s0, n, acc, q, qc, st, wk, sm)
+hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
+                           char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *wk,
+                           hipStream_t sm) {
+    DT_DISPATCH(launchAd, S, P, s0, n, replay, acc, q, qc, st, ovf, ovf_count, wk, sm)
 }
 hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc, float *acc, float *acc_pe,
                            unsigned long long *wk, unsigned grid, hipStream_t sm) {

@@ -33,8 +33,12 @@ constexpr double PRIMARY_DELTA = 1e-4;
 hipError_t setFrameRotation(const float *m9);
 // work[0..3] AD paths, [4..7] child sub-paths, [8..11] primary-edge samples:
 // box tests, triangle tests, density lookups, traced segments.
-hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc, char *q, unsigned *qc,
-                           unsigned *st, unsigned long long *work, hipStream_t sm);
+// AD samples s0 .. s0+n-1, or (replay != nullptr) the n listed {sid lo, sid hi,
+// children delivered, 0} entries; samples whose child pushes overflow the queue are
+// appended to ovf[*ovf_count] (capacity >= n) and contribute nothing in this launch.
+hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
+                           char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *work,
+                           hipStream_t sm);
 hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc, float *acc, float *acc_pe,
                            unsigned long long *work, unsigned grid, hipStream_t sm);
 hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,

@@ -9,7 +9,9 @@
 #include <cstdint>
 
 #define DTRT_NDER_MAX 16
-#define DTRT_BVH_STACK 64
+// Traversal stack entries per thread (LDS, 4 B each): the builder keeps every leaf
+// at depth <= DTRT_BVH_STACK - 2, so traversal never pushes more than that.
+#define DTRT_BVH_STACK 32
 
 namespace dt {
 

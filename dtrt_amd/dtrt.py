@@ -61,6 +61,7 @@ if "DTRT_NDER" in os.environ:
 _P = C.c_void_p
 _lib.dtrt_test_kernel_times.argtypes = [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
 _lib.dtrt_test_scene_info.argtypes = [_P, C.POINTER(C.c_int64)]
+_lib.dtrt_test_set_queue_budget.argtypes = [_P, C.c_uint64]
 _lib.dtrt_test_shape_edges.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]
 _lib.dtrt_test_trace.argtypes = [_P, _P, C.c_int, _P, _P]
 _lib.dtrt_test_li.argtypes = [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, _P]
@@ -85,11 +86,16 @@ def kernel_times(scene):
     return np.array(ms[:]), np.array(ln[:]), np.array(wk[:], dtype=np.uint64)
 
 
+def set_queue_budget(scene, nbytes):
+    """Child-record queue bytes for this scene's renders (0 restores the default)."""
+    _chk(_lib.dtrt_test_set_queue_budget(scene._h, int(nbytes)), "set_queue_budget")
+
+
 def render_stats(scene):
     a = (C.c_int64 * 10)()
     _chk(_lib.dtrt_render_stats(scene._h, a), "render_stats")
     keys = ["neg_pixels", "nonfinite_der", "dt_cap", "bad_edge_weight", "nonortho", "med_inconsistent",
-            "queue_overflow", "ad_samples", "children", "pe_samples"]
+            "replayed", "ad_samples", "children", "pe_samples"]
     return dict(zip(keys, a[:]))
 
 

@@ -129,7 +129,8 @@ int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_opt
 /* Counters of the last render on `scene` (the warnings the reference prints):
  * out[0] negative pixels, [1] non-finite tangents, [2] delta-tracking caps,
  * [3] invalid edge weights, [4] non-orthogonal edge tangents, [5] medium
- * inconsistencies, [6] child-queue overflows, [7] AD path samples,
+ * inconsistencies, [6] AD samples re-run after their child pushes overflowed the
+ * queue (exact replay; the image is unaffected), [7] AD path samples,
  * [8] child sub-paths, [9] primary-edge samples. */
 int dtrt_render_stats(dtrt_object *scene, int64_t *out10);
 

@@ -23,6 +23,9 @@ extern "C" {
  * [8..11] primary-edge kernel; each = box tests, triangle tests, grid lookups,
  * traced segments. */
 int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, uint64_t *work12);
+/* Bytes of child-record queue the scene's renders may use (0 = default 6 GiB).
+ * A small budget forces the exact overflow-replay path. */
+int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes);
 /* [0] BVH nodes, [1] triangles, [2] edges, [3] primary edges, [4] primary-edge
  * length sum (double bits), [5] secondary edges usable, [6] vertices, [7] faces. */
 int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8);
