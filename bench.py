@@ -134,10 +134,14 @@ def main():
     dom = int(np.argmax(ms))
     name = KERNELS[dom]
     achieved = alg[name] / (ms[dom] * 1e-3) / 1e9
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE profile of this workload
+    # (copy-calibrated); null when the workload differs from the profiled one
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "ad_render_heterogeneous_512x512_128spp_nder3_hbm_traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(name)
+        t = json.load(open(tfile))
+        if (t["width"], t["height"], t["spp"], t["nder"]) == (desc.width, desc.height, desc.spp, desc.nder):
+            traffic = t.get(name)
     rec = {
         "metric": "AD path samples/s at 1 GPU",
         "value": value,

@@ -28,12 +28,10 @@ DI f3 xform33(const float *m, f3 p) {
 }
 
 // Per-thread traversal stack in LDS, laid out [depth][lane] so that the lanes of a
-// wave touch consecutive banks.
-struct LStack {
-    int *base;
-    int stride;
-    DI int &at(int i) { return base[i * stride]; }
-};
+// wave touch consecutive banks. Every kernel that traces rays runs blocks of
+// exactly TRACE_BLOCK threads.
+constexpr int TRACE_BLOCK = 64;
+__shared__ int s_bvh_stack[DTRT_BVH_STACK * TRACE_BLOCK];
 
 // ---------------------------------------------------------------- BVH
 DI bool boxHit(const float4 &lo, const float4 &hi, f3 o, f3 inv, float tnear, float tfar, float &tentry) {
@@ -46,19 +44,30 @@ DI bool boxHit(const float4 &lo, const float4 &hi, f3 o, f3 inv, float tnear, fl
     return t0 <= t1;
 }
 
-// Closest hit in [tnear, tfar]; returns the global face id or -1.
-DI int traceClosest(const DScene &S, f3 o, f3 d, float tnear, float tfar, float &thit, LStack stk, uint32_t &nbox,
-                      uint32_t &ntri) {
-    if (S.nnodes == 0) return -1;
+// Closest hit in [tnear, tfar]: global face id (or -1), its t, and the box and
+// triangle tests done. Kept out of line so that every call site shares one copy of
+// the traversal code (the callers' code stays within the instruction cache).
+struct TraceHit {
+    int face;
+    float t;
+    uint32_t nbox, ntri;
+};
+
+__device__ __noinline__ TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3 o, f3 d,
+                                              float tnear, float tfar) {
+    TraceHit res{-1, tfar, 0u, 0u};
+    if (nnodes == 0) return res;
+    int *stk = s_bvh_stack + threadIdx.x;
     const float BEPS = 1e-6f;
     f3 inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float best = tfar;
     int face = -1;
     int sp = 0;
     int node = 0;
+    uint32_t nbox = 0, ntri = 0;
     while (true) {
         if (node >= 0) {
-            const float4 *n = S.nodes + 4 * (size_t)node;
+            const float4 *n = nodes + 4 * (size_t)node;
             float4 a = n[0], b = n[1], c = n[2], e = n[3];
             float tl, tr;
             nbox += 2;
@@ -67,7 +76,7 @@ DI int traceClosest(const DScene &S, f3 o, f3 d, float tnear, float tfar, float 
             int cl = __float_as_int(a.w), cr = __float_as_int(b.w);
             if (hl && hr) {
                 if (tr < tl) { int t = cl; cl = cr; cr = t; }
-                stk.at(sp++) = cr;
+                stk[TRACE_BLOCK * sp++] = cr;
                 node = cl;
                 continue;
             } else if (hl) {
@@ -82,7 +91,7 @@ DI int traceClosest(const DScene &S, f3 o, f3 d, float tnear, float tfar, float 
             int first = enc >> 4, cnt = enc & 15;
             ntri += cnt;
             for (int k = first; k < first + cnt; ++k) {
-                const float4 *tp = S.tris + 3 * (size_t)k;
+                const float4 *tp = tris + 3 * (size_t)k;
                 float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
                 f3 E1{e1.x, e1.y, e1.z}, E2{e2.x, e2.y, e2.z};
                 f3 pv = cross(d, E2);
@@ -100,10 +109,13 @@ DI int traceClosest(const DScene &S, f3 o, f3 d, float tnear, float tfar, float 
             }
         }
         if (sp == 0) break;
-        node = stk.at(--sp);
+        node = stk[TRACE_BLOCK * --sp];
     }
-    thit = best;
-    return face;
+    res.face = face;
+    res.t = best;
+    res.nbox = nbox;
+    res.ntri = ntri;
+    return res;
 }
 
 // ------------------------------------------------------------ hit records

@@ -16,7 +16,7 @@ namespace dt {
 
 __constant__ FrameRot c_frame_rot;
 
-constexpr int BLOCK = 64;
+constexpr int BLOCK = TRACE_BLOCK;
 
 DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
     // Blocks are dealt round-robin over the 8 XCDs; give each XCD a contiguous run
@@ -195,10 +195,9 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
                                                     const uint4 *replay, float *acc, char *q, unsigned *qcount,
                                                     unsigned *stats, uint4 *ovf, unsigned *ovf_count,
                                                     unsigned long long *work) {
-    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
-    Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    Tracer<N> T(S);
     Ds<N> L;
     const bool active = li < nsamp;
     uint64_t sid = s0 + li;
@@ -392,8 +391,7 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
                                                     float *acc, float *acc_pe, unsigned long long *work) {
-    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
-    Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    Tracer<N> T(S);
     const unsigned n = min(*qcount, P.qcap);
     const int nd = P.npix;
     uint32_t nlk = 0;
@@ -555,8 +553,7 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t 
 template <int N>
 __global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp, uint64_t total,
                                                    char *q, unsigned *qcount, unsigned *stats, unsigned long long *work) {
-    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
-    Tracer<N> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    Tracer<N> T(S);
     const uint64_t li = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t sid = s0 + li;
     Rng r;
@@ -593,8 +590,7 @@ __global__ void k_finalize(RenderParams P, const float *acc, const float *acc_pe
 
 // ---------------------------------------------------------------- test hooks
 __global__ void k_trace(DScene S, const float *rays, int n, int *out_face, float *out_t) {
-    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
-    Tracer<1> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    Tracer<1> T(S);
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const float *r = rays + (size_t)i * 8;
@@ -606,8 +602,7 @@ __global__ void k_trace(DScene S, const float *rays, int n, int *out_face, float
     else out_t[i] = FINF;
 }
 __global__ void k_li(DScene S, const float *rays, int n, int depth, int med, uint64_t seed, float *out) {
-    __shared__ int lds_stack[DTRT_BVH_STACK * BLOCK];
-    Tracer<1> T(S, LStack{lds_stack + threadIdx.x, BLOCK});
+    Tracer<1> T(S);
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const float *rr = rays + (size_t)i * 6;

@@ -12,13 +12,16 @@ DI float misPower(float a, float b) { a *= a; b *= b; return a / (a + b); }   //
 
 template <int N> struct Tracer {
     const DScene &S;
-    LStack stk;
     uint32_t nbox = 0, ntri = 0, nseg = 0;   // work counters: box tests, triangle tests, traced segments
-    DI Tracer(const DScene &s, LStack st) : S(s), stk(st) {}
+    DI explicit Tracer(const DScene &s) : S(s) {}
 
     DI int trace(f3 o, f3 d, float tnear, float tfar, float &t) {
         nseg++;
-        return traceClosest(S, o, d, tnear, tfar, t, stk, nbox, ntri);
+        TraceHit h = traceClosest(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar);
+        nbox += h.nbox;
+        ntri += h.ntri;
+        t = h.t;
+        return h.face;
     }
 
     // scene.cpp:538-575
