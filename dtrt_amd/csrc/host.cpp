@@ -807,7 +807,7 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
         kernel_launches[slot]++;
         return true;
     };
-    unsigned *qc = static_cast<unsigned *>(qcount.p);   // [0] queued records, [1] overflowed samples
+    unsigned *qc = static_cast<unsigned *>(qcount.p);   // [0] queued records, [1] overflowed samples, [2] consumed records
     unsigned *st = static_cast<unsigned *>(stats.p);    // [0..STAT_COUNT) counters, [32..) discarded (replays)
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
     float *A = static_cast<float *>(acc.p), *APE = static_cast<float *>(acc_pe.p);
@@ -824,7 +824,7 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
         const uint64_t n = std::min(chunk, total - s0);
         uint64_t delivered = 0;
         unsigned h[2];
-        HIPCHK(hipMemsetAsync(qc, 0, 8, stream));
+        HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
         if (!timed(0, [&] { return launch_ad_paths(S, P, s0, n, nullptr, A, Q, qc, st, OV[0], qc + 1, wk, stream); })) return false;
         if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
         if (!counts(h)) return false;
@@ -832,7 +832,7 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
         int in = 0;
         for (unsigned nov = h[1]; nov > 0; in ^= 1) {   // each round delivers >= 1 record: terminates
             replayed += nov;
-            HIPCHK(hipMemsetAsync(qc, 0, 8, stream));
+            HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
             if (!timed(0, [&] { return launch_ad_paths(S, P, 0, nov, OV[in], A, Q, qc, st + 32, OV[in ^ 1], qc + 1, wk, stream); }))
                 return false;
             if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
@@ -849,7 +849,7 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
     }
     for (uint64_t s0 = 0; s0 < pe_total; s0 += pe_chunk) {
         uint64_t n = std::min(pe_chunk, pe_total - s0);
-        HIPCHK(hipMemsetAsync(qc, 0, 4, stream));
+        HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
         if (!timed(2, [&] { return launch_primary(S, P, s0, n, pe_total, Q, qc, st, wk, stream); })) return false;
         if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
         unsigned h[2];

@@ -388,82 +388,117 @@ __global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, ui
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
 }
 
+// Adds the finished sub-paths of this wave: lanes with fin set add W * Lv to their
+// record's pixel. Lanes that share the first finishing lane's pixel and image are
+// summed over the wave first (one atomic per weight); the others add their own.
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
-                                                    float *acc, float *acc_pe, unsigned long long *work) {
-    Tracer<N> T(S);
-    const unsigned n = min(*qcount, P.qcap);
+DI void scatterFinished(const RenderParams &P, const char *q, float *acc, float *acc_pe, bool fin, unsigned rec, f3 Lv) {
+    const uint64_t m = __ballot(fin);
+    if (!m) return;
     const int nd = P.npix;
-    uint32_t nlk = 0;
-    const unsigned stride = gridDim.x * BLOCK;
-    // every lane of a wave runs the same number of iterations (idle lanes carry zero
-    // weights), so the wave-wide reductions below see all 64 lanes
-    const unsigned wave0 = (blockIdx.x * BLOCK + threadIdx.x) & ~63u;
-    for (unsigned base = wave0; base < n; base += stride) {
-        const unsigned i = base + (threadIdx.x & 63);
-        const bool live = i < n;
-        f3 Lv{0.f, 0.f, 0.f};
-        float W[N * 3];
-        int pix = -1;
-        bool prim = false;
-        if (live) {
-            const char *rec = q + (size_t)i * P.rec_stride;
-            const ChildHdr h = *reinterpret_cast<const ChildHdr *>(rec);
-            const float *Wg = reinterpret_cast<const float *>(rec + sizeof(ChildHdr));
+    int pix = -1;
+    bool prim = false;
+    float v[N * 3];
 #pragma unroll
-            for (int k = 0; k < N * 3; ++k) W[k] = Wg[k];
-            Rng r;
-            r.init(P.seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
-            int kind = h.meta & 3, depth = (h.meta >> 4) & 255, med = ((h.meta >> 12) & 255) - 1;
-            prim = (h.meta & 8) != 0;
-            pix = h.pixel;
-            f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
-            if (kind == CHILD_LI) {
-                Lv = T.Li(r, org, dir, depth, med, true);
-            } else {
-                Hit its0;
-                its0.face = h.face;
-                its0.p = org;
-                its0.gn = f3{h.gn[0], h.gn[1], h.gn[2]};
-                its0.sn = f3{h.sn[0], h.sn[1], h.sn[2]};
-                its0.wi = Frame(its0.sn).toLocal(-dir);
-                its0.t = 0.f;
-                Lv = T.LinsOnce(r, its0, dir, med, depth, kind == CHILD_LINS_NULL);
-            }
-            nlk += r.nlk;
-        }
-        // lanes that share lane 0's pixel and target image are summed over the wave
-        // and added with one atomic per channel; the others add their own
-        const int pix0 = __shfl(pix, 0);
-        const bool prim0 = __shfl(prim, 0);
-        const bool grp = live && pix == pix0 && prim == prim0;
-        float v[N * 3];
+    for (int k = 0; k < N * 3; ++k) v[k] = 0.f;
+    if (fin) {
+        const char *r = q + (size_t)rec * P.rec_stride;
+        const ChildHdr *h = reinterpret_cast<const ChildHdr *>(r);
+        const float *W = reinterpret_cast<const float *>(r + sizeof(ChildHdr));
+        pix = h->pixel;
+        prim = (h->meta & 8) != 0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             v[3 * j] = W[3 * j] * Lv.x;
             v[3 * j + 1] = W[3 * j + 1] * Lv.y;
             v[3 * j + 2] = W[3 * j + 2] * Lv.z;
         }
-        if (live && !grp) {
-            float *dst = prim ? acc_pe : acc + (size_t)nd * 3;
+    }
+    const int lead = __ffsll((unsigned long long)m) - 1;
+    const int pix0 = __shfl(pix, lead);
+    const bool prim0 = __shfl((int)prim, lead) != 0;
+    const bool grp = fin && pix == pix0 && prim == prim0;
+    if (fin && !grp) {
+        float *dst = prim ? acc_pe : acc + (size_t)nd * 3;
 #pragma unroll
-            for (int k = 0; k < N * 3; ++k)
-                if (v[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix) * 3 + (k % 3), v[k]);
-        }
-        if (pix0 >= 0) {
-            float sv[N * 3];
+        for (int k = 0; k < N * 3; ++k)
+            if (v[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix) * 3 + (k % 3), v[k]);
+    }
 #pragma unroll
-            for (int k = 0; k < N * 3; ++k) {
-                sv[k] = grp ? v[k] : 0.f;
-                for (int off = 32; off > 0; off >>= 1) sv[k] += __shfl_xor(sv[k], off);
+    for (int k = 0; k < N * 3; ++k) {
+        float sv = grp ? v[k] : 0.f;
+        for (int off = 32; off > 0; off >>= 1) sv += __shfl_xor(sv, off);
+        v[k] = sv;
+    }
+    if ((int)(threadIdx.x & 63) == lead) {
+        float *dst = prim0 ? acc_pe : acc + (size_t)nd * 3;
+#pragma unroll
+        for (int k = 0; k < N * 3; ++k)
+            if (v[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix0) * 3 + (k % 3), v[k]);
+    }
+}
+
+// Persistent: each lane holds one primal sub-path and advances it one bounce per
+// iteration; a lane whose path has ended takes the next queued record (one atomic
+// per wave on qnext), so the lanes of a wave stay busy until the queue is drained.
+template <int N>
+__global__ __launch_bounds__(BLOCK) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
+                                                    unsigned *qnext, float *acc, float *acc_pe, unsigned long long *work) {
+    Tracer<N> T(S);
+    const unsigned n = min(*qcount, P.qcap);
+    const int lane = threadIdx.x & 63;
+    uint32_t nlk = 0;
+    bool alive = false, exhausted = false;
+    unsigned rec = 0;
+    LiPath path;
+    Rng r;
+    f3 pre{0, 0, 0}, cw{0, 0, 0};
+    while (true) {
+        bool fin = false;
+        f3 Lv{0, 0, 0};
+        if (!exhausted) {
+            const uint64_t need = __ballot(!alive);
+            if (need) {
+                const int lead = __ffsll((unsigned long long)need) - 1;
+                unsigned base = 0;
+                if (lane == lead) base = atomicAdd(qnext, (unsigned)__popcll(need));
+                base = __shfl(base, lead);
+                if (base + __popcll(need) >= n) exhausted = true;
+                const unsigned my = base + (unsigned)__popcll(need & ((1ull << lane) - 1ull));
+                if (!alive && my < n) {
+                    rec = my;
+                    const ChildHdr h = *reinterpret_cast<const ChildHdr *>(q + (size_t)rec * P.rec_stride);
+                    r.init(P.seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
+                    const int kind = h.meta & 3, depth = (h.meta >> 4) & 255, med = ((h.meta >> 12) & 255) - 1;
+                    const f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
+                    bool has = true;
+                    if (kind == CHILD_LI) {
+                        pre = f3{0, 0, 0};
+                        cw = f3{1, 1, 1};
+                        T.liStart(path, org, dir, depth, med, true);
+                    } else {
+                        Hit its0;
+                        its0.face = h.face;
+                        its0.p = org;
+                        its0.gn = f3{h.gn[0], h.gn[1], h.gn[2]};
+                        its0.sn = f3{h.sn[0], h.sn[1], h.sn[2]};
+                        its0.wi = Frame(its0.sn).toLocal(-dir);
+                        its0.t = 0.f;
+                        pre = T.linsStart(r, its0, dir, med, depth, kind == CHILD_LINS_NULL, path, cw, has);
+                    }
+                    alive = has;
+                    if (!has) { fin = true; Lv = pre; nlk += r.nlk; }
+                }
             }
-            if ((threadIdx.x & 63) == 0) {
-                float *dst = prim0 ? acc_pe : acc + (size_t)nd * 3;
-#pragma unroll
-                for (int k = 0; k < N * 3; ++k)
-                    if (sv[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix0) * 3 + (k % 3), sv[k]);
-            }
         }
+        if (alive && !T.liStep(path, r)) {
+            alive = false;
+            fin = true;
+            Lv = pre + cw * path.ret;
+            nlk += r.nlk;
+        }
+        scatterFinished<N>(P, q, acc, acc_pe, fin, rec, Lv);
+        if (exhausted && !__any(alive)) break;
     }
     flushWork(work, 4, T.nbox, T.ntri, nlk, T.nseg);
 }
@@ -652,9 +687,9 @@ template <int N> static void launchAd(const DScene &S, const RenderParams &P, ui
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_ad_paths<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf, ovf_count, wk);
 }
-template <int N> static void launchChildren(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc,
+template <int N> static void launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
                                             float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
-    hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, acc, acc_pe, wk);
+    hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
 }
 template <int N> static void launchPrimary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total,
                                            char *q, unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
@@ -686,7 +721,7 @@ hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, 
                            hipStream_t sm) {
     DT_DISPATCH(launchAd, S, P, s0, n, replay, acc, q, qc, st, ovf, ovf_count, wk, sm)
 }
-hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc, float *acc, float *acc_pe,
+hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc, float *acc_pe,
                            unsigned long long *wk, unsigned grid, hipStream_t sm) {
     DT_DISPATCH(launchChildren, S, P, q, qc, acc, acc_pe, wk, grid, sm)
 }

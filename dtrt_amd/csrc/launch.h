@@ -39,7 +39,8 @@ hipError_t setFrameRotation(const float *m9);
 hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
                            char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *work,
                            hipStream_t sm);
-hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, const unsigned *qc, float *acc, float *acc_pe,
+// Drains the qc[0] queued records (qc[2] = 0 on entry: the consumption counter).
+hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc, float *acc_pe,
                            unsigned long long *work, unsigned grid, hipStream_t sm);
 hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,
                           unsigned *st, unsigned long long *work, hipStream_t sm);

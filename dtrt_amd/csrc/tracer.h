@@ -10,6 +10,14 @@ namespace dt {
 
 DI float misPower(float a, float b) { a *= a; b *= b; return a / (a + b); }   // volpathAD.cpp:43-47
 
+// State of one primal path between bounces (see Tracer::liStep).
+struct LiPath {
+    f3 ro, rd, thr, ret;
+    Hit its;
+    int depth, max_depth, med;
+    bool incEmission;
+};
+
 template <int N> struct Tracer {
     const DScene &S;
     uint32_t nbox = 0, ntri = 0, nseg = 0;   // work counters: box tests, triangle tests, traced segments
@@ -245,73 +253,104 @@ template <int N> struct Tracer {
     }
 
     // volpathAD.cpp:50-142 — primal path tracer used by every edge / boundary sub-path
-    DI f3 Li(Rng &r, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {
-        Hit its;
-        f3 ret{0, 0, 0}, thr{1, 1, 1};
-        rayIntersect(ro, rd, true, its);
-        for (int depth = 0; depth <= max_depth; ++depth) {
-            int max_inter = max_depth - depth - 1;
-            float r1a = r.next(), r1b = r.next();
-            float r2[4] = {r.next(), r.next(), r.next(), r.next()};
-            float r3[3] = {r.next(), r.next(), r.next()};
-            bool inside = false;
-            if (med >= 0) {
-                f3 ps;
-                inside = sampleDistance(S.media[med], ro, rd, its.t, r1a, r1b, r, ps, thr);
-                if (inside) ro = ps;
-            }
-            if (inside) {
-                if (depth >= max_depth) break;
-                if (allZero(thr, EPS)) break;
-                const DPhase &ph = S.phases[S.media[med].phase_id];
-                f3 wo;
-                float pdf_nee;
-                f3 value = neePoint(ro, r2, r, med, max_inter, wo, pdf_nee);
-                if (!allZero(value, EPS)) {
-                    float pv = phaseEval(ph, -rd, wo);
-                    if (pv > EPS) ret = ret + thr * value * (misPower(pdf_nee, pv) * pv);
-                }
-                wo = phaseSample(ph, -rd, r3[0], r3[1]);
-                rd = wo;
-                f3 att = lookForEmitter(ro, rd, false, r, med, max_inter, its, pdf_nee);
-                if (!allZero(att, EPS)) ret = ret + thr * att * misPower(phaseEval(ph, -rd, wo), pdf_nee);
-                incEmission = false;
-            } else {
-                if (its.face < 0) break;
-                const DShape &sh = shapeOf(S, its.face);
-                if (sh.light_id >= 0 && incEmission) ret = ret + thr * emitterEval(S.emitters[sh.light_id], its.gn, its.sn, -rd);
-                if (depth >= max_depth) break;
-                const DBsdf &b = bsdfOf(S, sh);
-                Frame shf(its.sn);
-                f3 wo;
-                float pdf_nee;
-                if (b.type != BSDF_NULL) {
-                    f3 value = neeSurface(its, shf, r2, r, med, max_inter, wo, pdf_nee, false);
-                    if (!allZero(value, EPS))
-                        ret = ret + thr * value * bsdfEval(b, its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, its.wi, wo));
-                }
-                float bpdf = 0.f;
-                f3 bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
-                if (allZero(bw, EPS)) break;
-                wo = shf.toWorld(wo);
-                ro = its.p;
-                rd = wo;
-                if (isTransition(sh)) med = targetMedium(sh, its.gn, wo);
-                if (b.type == BSDF_NULL) {
-                    rayIntersect(ro, rd, true, its);
-                    continue;
-                }
-                f3 att = lookForEmitter(ro, rd, true, r, med, max_inter, its, pdf_nee);
-                thr = thr * bw;
-                if (!allZero(att, 1e-12f)) ret = ret + thr * att * misPower(bpdf, pdf_nee);
-                incEmission = false;
-            }
-        }
-        return ret;
+    // The primal estimator as a resumable state machine: liStart traces the first
+    // segment, each liStep runs one iteration of the bounce loop of
+    // volpathAD.cpp:50-142 (same random-number order), and returns false once the
+    // path has ended; the estimate is then in p.ret.
+    DI void liStartAt(LiPath &p, f3 ro, f3 rd, int max_depth, int med, bool incEmission, const Hit &first) {
+        p.ro = ro;
+        p.rd = rd;
+        p.thr = f3{1, 1, 1};
+        p.ret = f3{0, 0, 0};
+        p.depth = 0;
+        p.max_depth = max_depth;
+        p.med = med;
+        p.incEmission = incEmission;
+        p.its = first;
+    }
+    DI void liStart(LiPath &p, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {
+        Hit h;
+        rayIntersect(ro, rd, true, h);
+        liStartAt(p, ro, rd, max_depth, med, incEmission, h);
     }
 
-    // volpathAD.cpp:338-441 — one sample of Lins at a boundary hit (ALTERNATIVE_LINS_BOUNDARY_SAMPLING)
-    DI f3 LinsOnce(Rng &r, const Hit &its0, f3 rdv, int med, int max_depth, bool null_bsdf) {
+    DI bool liStep(LiPath &p, Rng &r) {
+        if (p.depth > p.max_depth) return false;
+        const int max_inter = p.max_depth - p.depth - 1;
+        float r1a = r.next(), r1b = r.next();
+        float r2[4] = {r.next(), r.next(), r.next(), r.next()};
+        float r3[3] = {r.next(), r.next(), r.next()};
+        bool inside = false;
+        if (p.med >= 0) {
+            f3 ps;
+            inside = sampleDistance(S.media[p.med], p.ro, p.rd, p.its.t, r1a, r1b, r, ps, p.thr);
+            if (inside) p.ro = ps;
+        }
+        if (inside) {
+            if (p.depth >= p.max_depth) return false;
+            if (allZero(p.thr, EPS)) return false;
+            const DPhase &ph = S.phases[S.media[p.med].phase_id];
+            f3 wo;
+            float pdf_nee;
+            f3 value = neePoint(p.ro, r2, r, p.med, max_inter, wo, pdf_nee);
+            if (!allZero(value, EPS)) {
+                float pv = phaseEval(ph, -p.rd, wo);
+                if (pv > EPS) p.ret = p.ret + p.thr * value * (misPower(pdf_nee, pv) * pv);
+            }
+            wo = phaseSample(ph, -p.rd, r3[0], r3[1]);
+            p.rd = wo;
+            f3 att = lookForEmitter(p.ro, p.rd, false, r, p.med, max_inter, p.its, pdf_nee);
+            if (!allZero(att, EPS)) p.ret = p.ret + p.thr * att * misPower(phaseEval(ph, -p.rd, wo), pdf_nee);
+            p.incEmission = false;
+        } else {
+            if (p.its.face < 0) return false;
+            const DShape &sh = shapeOf(S, p.its.face);
+            if (sh.light_id >= 0 && p.incEmission)
+                p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], p.its.gn, p.its.sn, -p.rd);
+            if (p.depth >= p.max_depth) return false;
+            const DBsdf &b = bsdfOf(S, sh);
+            Frame shf(p.its.sn);
+            f3 wo;
+            float pdf_nee;
+            if (b.type != BSDF_NULL) {
+                f3 value = neeSurface(p.its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
+                if (!allZero(value, EPS))
+                    p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, p.its.wi, wo));
+            }
+            float bpdf = 0.f;
+            f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+            if (allZero(bw, EPS)) return false;
+            wo = shf.toWorld(wo);
+            p.ro = p.its.p;
+            p.rd = wo;
+            if (isTransition(sh)) p.med = targetMedium(sh, p.its.gn, wo);
+            if (b.type == BSDF_NULL) {
+                rayIntersect(p.ro, p.rd, true, p.its);
+            } else {
+                f3 att = lookForEmitter(p.ro, p.rd, true, r, p.med, max_inter, p.its, pdf_nee);
+                p.thr = p.thr * bw;
+                if (!allZero(att, 1e-12f)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
+                p.incEmission = false;
+            }
+        }
+        p.depth++;
+        return p.depth <= p.max_depth;
+    }
+
+    DI f3 Li(Rng &r, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {   // volpathAD.cpp:50-142
+        LiPath p;
+        liStart(p, ro, rd, max_depth, med, incEmission);
+        while (liStep(p, r)) {}
+        return p.ret;
+    }
+
+    // volpathAD.cpp:338-441 — one sample of Lins at a boundary hit (ALTERNATIVE_LINS_BOUNDARY_SAMPLING).
+    // Returns the terms computed at the hit; every branch ends in at most one primal
+    // sub-path, which is returned as `cont` (started, weight `cw`) for the caller to
+    // run: the sample is the returned value + cw * cont.ret. `has_cont` = false: none.
+    DI f3 linsStart(Rng &r, const Hit &its0, f3 rdv, int med, int max_depth, bool null_bsdf, LiPath &cont, f3 &cw,
+                    bool &has_cont) {
+        has_cont = false;
         const DShape &shp = shapeOf(S, its0.face);
         const DBsdf &b = bsdfOf(S, shp);
         const DPhase &ph = S.phases[S.media[med].phase_id];
@@ -340,7 +379,9 @@ template <int N> struct Tracer {
                 Hit h1;
                 f3 att = lookForEmitter(its0.p, wo, true, r, med1, md1, h1, pdf_nee);
                 if (!allZero(att, EPS)) Lins = Lins + att * (pv * misPower(phaseEval(ph, -rdv, wo), pdf_nee));
-                Lins = Lins + Li(r, its0.p, wo, md1, med1, false) * pv;
+                liStartAt(cont, its0.p, wo, md1, med1, false, h1);   // h1 = first hit of this ray
+                cw = f3{pv, pv, pv};
+                has_cont = true;
             }
         } else {
             f3 att = neeSurface(its0, shf, r4, r, med, max_depth - 1, wo, pdf_nee, false);
@@ -359,7 +400,9 @@ template <int N> struct Tracer {
                 Hit h1;
                 f3 att2 = lookForEmitter(its0.p, wo, true, r, med, max_depth - 1, h1, pdf_nee);
                 if (!allZero(att2, EPS)) Lins = Lins + att2 * (pv * misPower(phaseEval(ph, -rdv, wo), pdf_nee));
-                Lins = Lins + Li(r, its0.p, wo, max_depth - 1, med, false) * pv;
+                liStartAt(cont, its0.p, wo, max_depth - 1, med, false, h1);
+                cw = f3{pv, pv, pv};
+                has_cont = true;
             } else if (wdn > EPS) {
                 if (shp.light_id >= 0) Lins = Lins + emitterEval(S.emitters[shp.light_id], its0.gn, its0.sn, -wo) * pv;
                 if (max_depth > 1) {
@@ -378,12 +421,26 @@ template <int N> struct Tracer {
                         Hit itsc;
                         f3 att4 = lookForEmitter(itsb.p, wow, true, r, med1, max_depth - 2, itsc, pdf_nee);
                         if (!allZero(att4, 1e-12f)) Lins = Lins + thr1 * att4 * misPower(bpdf, pdf_nee);
-                        Lins = Lins + thr1 * Li(r, itsb.p, wow, max_depth - 2, med1, false);
+                        liStartAt(cont, itsb.p, wow, max_depth - 2, med1, false, itsc);
+                        cw = thr1;
+                        has_cont = true;
                     }
                 }
             }
         }
         return Lins;
+    }
+
+    DI f3 LinsOnce(Rng &r, const Hit &its0, f3 rdv, int med, int max_depth, bool null_bsdf) {
+        LiPath c;
+        f3 cw{0, 0, 0};
+        bool has;
+        f3 L = linsStart(r, its0, rdv, med, max_depth, null_bsdf, c, cw, has);
+        if (has) {
+            while (liStep(c, r)) {}
+            L = L + cw * c.ret;
+        }
+        return L;
     }
 };
 
