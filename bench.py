@@ -64,7 +64,7 @@ def cpu_baseline(desc, target_s=15.0):
 
     side = 8
     t, n = run(side)
-    side = int(min(96, max(8, side * (target_s / max(t, 1e-3)) ** 0.5)))
+    side = int(min(160, max(8, side * (target_s / max(t, 1e-3)) ** 0.5)))
     t, n = run(side)
     return {"value": n / t, "unit": "AD path samples/s", "cores": threads, "kind": "port",
             "sample": "CPU oracle (fp64 restatement of volpathAD/integratorAD, OpenMP) on a centred %dx%d crop of the "

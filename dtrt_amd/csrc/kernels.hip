@@ -17,6 +17,14 @@ namespace dt {
 __constant__ FrameRot c_frame_rot;
 
 constexpr int BLOCK = TRACE_BLOCK;
+// Minimum resident waves per SIMD requested from the register allocator (HIP's second
+// __launch_bounds__ argument): bounds the VGPRs of the two path kernels.
+#ifndef CHILD_WAVES_PER_SIMD
+#define CHILD_WAVES_PER_SIMD 4
+#endif
+#ifndef AD_WAVES_PER_SIMD
+#define AD_WAVES_PER_SIMD 2
+#endif
 
 DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
     // Blocks are dealt round-robin over the 8 XCDs; give each XCD a contiguous run
@@ -191,7 +199,7 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
 // {sid lo, sid hi, children delivered, 0} of samples whose pushes overflowed.
 // Samples that overflow (again) are appended to ovf.
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
+__global__ __launch_bounds__(BLOCK, AD_WAVES_PER_SIMD) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
                                                     const uint4 *replay, float *acc, char *q, unsigned *qcount,
                                                     unsigned *stats, uint4 *ovf, unsigned *ovf_count,
                                                     unsigned long long *work) {
@@ -442,7 +450,7 @@ DI void scatterFinished(const RenderParams &P, const char *q, float *acc, float 
 // iteration; a lane whose path has ended takes the next queued record (one atomic
 // per wave on qnext), so the lanes of a wave stay busy until the queue is drained.
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
+__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
                                                     unsigned *qnext, float *acc, float *acc_pe, unsigned long long *work) {
     Tracer<N> T(S);
     const unsigned n = min(*qcount, P.qcap);
