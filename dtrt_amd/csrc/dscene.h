@@ -247,7 +247,62 @@ template <int N> DI Dv<N> toGridAD(const DMedium &m, const Dv<N> &p, bool albedo
     }
     return g;
 }
-template <int N> DI Df<N> lookupDensityAD(const DMedium &m, const Dv<N> &p) { return trilinearAD<N>(m.density, m.res, toGridAD<N>(m, p, false), 0, 1); }
+// Cell of a node-centred grid coordinate, with the bounds test of trilinear().
+struct GridCell {
+    int x, y, z;
+    float fx, fy, fz;
+};
+DI bool gridCell(const int *res, f3 g, GridCell &c) {
+    float fx0 = floorf(g.x), fy0 = floorf(g.y), fz0 = floorf(g.z);
+    if (!(fx0 >= 0.f && fy0 >= 0.f && fz0 >= 0.f && fx0 < (float)(res[0] - 1) && fy0 < (float)(res[1] - 1) && fz0 < (float)(res[2] - 1)))
+        return false;
+    c.x = (int)fx0; c.y = (int)fy0; c.z = (int)fz0;
+    c.fx = g.x - fx0; c.fy = g.y - fy0; c.fz = g.z - fz0;
+    return true;
+}
+
+template <int N> DI Df<N> lookupDensityAD(const DMedium &m, const Dv<N> &p) {
+    Df<N> r = trilinearAD<N>(m.density, m.res, toGridAD<N>(m, p, false), 0, 1);
+    if (m.vox_any) {   // d(density at p) / d(node v) = trilinear weight of v at p
+        GridCell c;
+        if (gridCell(m.res, toGrid(m, p.val()), c)) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const int v = m.vox_lane[j];
+                if (v < 0) continue;
+                const int dx = v % m.res[0] - c.x, dy = (v / m.res[0]) % m.res[1] - c.y, dz = v / (m.res[0] * m.res[1]) - c.z;
+                if ((unsigned)dx > 1u || (unsigned)dy > 1u || (unsigned)dz > 1u) continue;
+                r.d[j] += (dx ? c.fx : 1.f - c.fx) * (dy ? c.fy : 1.f - c.fy) * (dz ? c.fz : 1.f - c.fz);
+            }
+        }
+    }
+    return r;
+}
+
+// Reverse-mode hook of the medium estimators (path-replay adjoint). When set, every
+// density-dependent weight of the estimator adds w * d(log weight)/d(node density)
+// into grad[node] for the grid of medium *pm; the forward kernels pass none.
+struct ScatterCtx {
+    float *grad;
+    const DMedium *pm;
+    float w;
+};
+// grad[v] += w * (trilinear weight of node v at p) over the 8 nodes of p's cell
+DI void scatterDensity(const DMedium &m, f3 p, float w, float *grad) {
+    GridCell c;
+    if (w == 0.f || !gridCell(m.res, toGrid(m, p), c)) return;
+    const size_t sy = (size_t)m.res[0], sz = sy * m.res[1];
+    float *b = grad + (size_t)c.z * sz + (size_t)c.y * sy + c.x;
+    const float _fx = 1.f - c.fx, _fy = 1.f - c.fy, _fz = 1.f - c.fz;
+    unsafeAtomicAdd(b, w * _fx * _fy * _fz);
+    unsafeAtomicAdd(b + 1, w * c.fx * _fy * _fz);
+    unsafeAtomicAdd(b + sy, w * _fx * c.fy * _fz);
+    unsafeAtomicAdd(b + sy + 1, w * c.fx * c.fy * _fz);
+    unsafeAtomicAdd(b + sz, w * _fx * _fy * c.fz);
+    unsafeAtomicAdd(b + sz + 1, w * c.fx * _fy * c.fz);
+    unsafeAtomicAdd(b + sz + sy, w * _fx * c.fy * c.fz);
+    unsafeAtomicAdd(b + sz + sy + 1, w * c.fx * c.fy * c.fz);
+}
 template <int N> DI Ds<N> lookupAlbedoAD(const DMedium &m, const Dv<N> &p) {
     Dv<N> g = toGridAD<N>(m, p, true);
     Ds<N> r;
@@ -282,15 +337,19 @@ DI bool aabbIntersect(const float *lo, const float *hi, f3 o, f3 d, float &nearT
 }
 
 // heterogeneous.cpp:121-141 — derivative of the optical depth along a segment
+// With `sc` (adjoint), each of the 10 points also scatters -w * len/10 * scale.
 template <int N>
-DI void intSigT(const DMedium &m, const Dv<N> &o, const Dv<N> &d, float tmin, const Df<N> &tmax, Rng &r, float *out) {
+DI void intSigT(const DMedium &m, const Dv<N> &o, const Dv<N> &d, float tmin, const Df<N> &tmax, Rng &r, float *out,
+                const ScatterCtx *scx = nullptr) {
 #pragma unroll
     for (int j = 0; j < N; ++j) out[j] = 0.f;
     const float len = tmax.v - tmin;
     Df<N> sc = scaleAD<N>(m);
     for (int i = 0; i < 10; ++i) {
         float t = tmin + len * ((i + r.next()) * 0.1f);
-        Df<N> s = sc * lookupDensityAD<N>(m, o + d * Df<N>(t));
+        Dv<N> x = o + d * Df<N>(t);
+        Df<N> s = sc * lookupDensityAD<N>(m, x);
+        if (scx && &m == scx->pm) scatterDensity(m, x.val(), -scx->w * len * 0.1f * m.scale, scx->grad);
         r.nlk++;
 #pragma unroll
         for (int j = 0; j < N; ++j) out[j] += s.d[j];
@@ -345,9 +404,11 @@ DI bool sampleDistance(const DMedium &m, f3 o, f3 d, float tmax, float r0, float
 }
 
 // heterogeneous.cpp:40-75, homogeneous.cpp:27-49
+// With `sc` (adjoint): on a real collision at p the weight's log-derivative is
+// sigma_t'(p)/sigma_t(p) - int sigma_t', on a pass-through -int sigma_t'.
 template <int N>
 DI bool sampleDistanceAD(const DMedium &m, const Dv<N> &o, const Dv<N> &d, const Df<N> &tmax, float r0, float r1, Rng &r,
-                         Dv<N> &ps, Ds<N> &thr) {
+                         Dv<N> &ps, Ds<N> &thr, const ScatterCtx *scx = nullptr) {
     if (m.type == MED_HOMOGENEOUS) {
         Df<N> st = sigmaTAD<N>(m);
         if (m.sampling_weight > EPS) {
@@ -386,7 +447,8 @@ DI bool sampleDistanceAD(const DMedium &m, const Dv<N> &o, const Dv<N> &d, const
             ps = p;
             Ds<N> alb = m.albedo_grid ? lookupAlbedoAD<N>(m, p) : albedoAD<N>(m);
             Df<N> sigT = scaleAD<N>(m) * dens;
-            intSigT<N>(m, o, d, 0.f, Df<N>(t), r, is);
+            if (scx && &m == scx->pm) scatterDensity(m, p.val(), scx->w / dens.v, scx->grad);
+            intSigT<N>(m, o, d, 0.f, Df<N>(t), r, is, scx);
             // sigT * (1, -intSigT) / sigT.v
             Df<N> w(1.f);
             float iv = 1.0f / sigT.v;
@@ -396,7 +458,7 @@ DI bool sampleDistanceAD(const DMedium &m, const Dv<N> &o, const Dv<N> &d, const
             return true;
         }
     }
-    intSigT<N>(m, o, d, 0.f, tmax, r, is);
+    intSigT<N>(m, o, d, 0.f, tmax, r, is, scx);
     Df<N> w(1.f);
 #pragma unroll
     for (int j = 0; j < N; ++j) w.d[j] = -is[j];
@@ -416,12 +478,14 @@ DI float mediumTransmittance(const DMedium &m, f3 o, f3 d, float tmin, float tma
     }
     return ret;
 }
+// With `sc` (adjoint): the estimate's log-derivative is -int sigma_t'.
 template <int N>
-DI Df<N> mediumTransmittanceAD(const DMedium &m, const Dv<N> &o, const Dv<N> &d, float tmin, const Df<N> &tmax, Rng &r) {
+DI Df<N> mediumTransmittanceAD(const DMedium &m, const Dv<N> &o, const Dv<N> &d, float tmin, const Df<N> &tmax, Rng &r,
+                               const ScatterCtx *scx = nullptr) {
     if (m.type == MED_HOMOGENEOUS) return dexp((Df<N>(tmin) - tmax) * sigmaTAD<N>(m));
     Df<N> ret(mediumTransmittance(m, o.val(), d.val(), tmin, tmax.v, r));
     float is[N];
-    intSigT<N>(m, o, d, tmin, tmax, r, is);
+    intSigT<N>(m, o, d, tmin, tmax, r, is, scx);
 #pragma unroll
     for (int j = 0; j < N; ++j) ret.d[j] = -ret.v * is[j];
     return ret;

@@ -508,6 +508,7 @@ struct Scene : Obj {
     DScene S;
     std::vector<DevBuf> bufs;
     DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf, ovf[2];
+    std::vector<DMedium> hmed;   // host copy of the device media records
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;
     // last render
@@ -537,6 +538,21 @@ struct Scene : Obj {
     bool build();
     bool initEdges(const float *weights);
     bool render(const dtrt_render_options &o, float *out, bool out_is_device);
+    bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
+    bool setDensity(int med, const float *density, bool on_device);
+    bool setVoxelLanes(int med, const int *vox, int n);
+    bool uploadMedia() {
+        if (hmed.empty()) return true;
+        HIPCHK(hipMemcpyAsync(const_cast<DMedium *>(S.media), hmed.data(), hmed.size() * sizeof(DMedium), hipMemcpyHostToDevice,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return true;
+    }
+    bool gridMedium(int med) const {
+        if (med < 0 || med >= (int)hmed.size() || hmed[med].type != MED_HETEROGENEOUS || !hmed[med].density)
+            return fail("medium index does not name a heterogeneous grid medium of this scene");
+        return true;
+    }
 };
 
 bool frameRotationReady() {
@@ -644,9 +660,12 @@ bool Scene::build() {
         DMedium d = m->d;
         d.density = m->density.empty() ? nullptr : upload(m->density);
         d.albedo_grid = m->albedo.empty() ? nullptr : upload(m->albedo);
+        for (int j = 0; j < DTRT_NDER_MAX; ++j) d.vox_lane[j] = -1;
+        d.vox_any = 0;
         dm.push_back(d);
     }
     S.media = upload(dm);
+    hmed = dm;
     // lights (scene.cpp:45-98)
     int nl = (int)emitters.size();
     std::vector<double> lcdf(nl), pool;
@@ -870,6 +889,81 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
     last_stats[9] = (int64_t)pe_total;
     for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
     if (hst[STAT_QUEUE_OVERFLOW]) return fail("render: primary-edge child queue overflow");
+    return true;
+}
+
+// Replace the density grid of medium `med` (same resolution): negative values are
+// clamped to 0 as the VOL reader does, and the majorant is recomputed.
+bool Scene::setDensity(int med, const float *density, bool on_device) {
+    if (!gridMedium(med) || !density) return fail("set_density: bad arguments");
+    DMedium &d = hmed[med];
+    const uint64_t n = (uint64_t)d.res[0] * d.res[1] * d.res[2];
+    float *dst = const_cast<float *>(d.density);
+    if (!ensure(stats, 64 * 4)) return false;
+    unsigned *mx = static_cast<unsigned *>(stats.p) + 48;
+    HIPCHK(hipMemsetAsync(mx, 0, 4, stream));
+    if (!on_device) HIPCHK(hipMemcpyAsync(dst, density, n * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(launch_load_density(dst, on_device ? density : dst, n, mx, stream));
+    unsigned bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, mx, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    float m;
+    std::memcpy(&m, &bits, 4);
+    if (!(m > 0.f)) return fail("set_density: density grid is all zero");
+    d.maxDensity = m;
+    d.maxInvDensity = (float)(1.0 / ((double)d.scale * m));
+    return uploadMedia();
+}
+
+bool Scene::setVoxelLanes(int med, const int *vox, int n) {
+    if (!gridMedium(med) || n < 0 || n > nder || (n > 0 && !vox)) return fail("voxel lanes: bad arguments");
+    DMedium &d = hmed[med];
+    const int64_t nv = (int64_t)d.res[0] * d.res[1] * d.res[2];
+    d.vox_any = 0;
+    for (int j = 0; j < DTRT_NDER_MAX; ++j) {
+        int v = j < n ? vox[j] : -1;
+        if (v >= nv) return fail("voxel lanes: node index outside the grid");
+        d.vox_lane[j] = v < 0 ? -1 : v;
+        d.vox_any |= v >= 0;
+    }
+    return uploadMedia();
+}
+
+// Path-replay adjoint of the interior terms (emission, surface, volume) w.r.t. the
+// node densities of medium `med`; dimg = d loss / d image value [npix][3], grad
+// accumulates [z][y][x]. Both are device pointers.
+bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad) {
+    if (!gridMedium(med) || !dimg || !grad) return false;
+    RenderParams P{};
+    P.seed = o.seed; P.spp = o.spp; P.max_bounces = o.max_bounces; P.sppe = o.sppe; P.sppse = 0;
+    P.mode = (o.mode >= 0 ? o.mode : 0x3F) & (1 | 2 | 8);
+    P.ddist = 0.f;
+    P.width = cam.w; P.height = cam.h; P.ox = cam.ox; P.oy = cam.oy; P.cw = cam.cw; P.ch = cam.ch; P.cropped = cam.cropped();
+    P.npix = P.cropped ? cam.cw * cam.ch : cam.w * cam.h;
+    if (P.spp <= 0 || P.max_bounces < 0 || P.max_bounces > 200) return fail("adjoint: bad options");
+    if (!ensure(stats, 64 * 4) || !ensure(work, 16 * 8)) return false;
+    HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
+    if (ev.empty()) {
+        ev.resize(2);
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    unsigned *st = static_cast<unsigned *>(stats.p) + 32;   // warnings of the replays are not reported
+    unsigned long long *wk = static_cast<unsigned long long *>(work.p);
+    const uint64_t total = (uint64_t)P.npix * P.spp, chunk = uint64_t(1) << 22;
+    std::memset(kernel_ms, 0, sizeof(kernel_ms));
+    std::memset(kernel_launches, 0, sizeof(kernel_launches));
+    HIPCHK(hipEventRecord(ev[0], stream));
+    for (uint64_t s0 = 0; s0 < total; s0 += chunk)
+        HIPCHK(launch_adjoint(S, P, s0, std::min(chunk, total - s0), dimg, grad, med, st, wk, stream));
+    HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    kernel_ms[0] = ms;
+    kernel_launches[0] = (int64_t)((total + chunk - 1) / chunk);
+    unsigned long long hw[12];
+    HIPCHK(hipMemcpy(hw, wk, sizeof(hw), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
     return true;
 }
 
@@ -1208,7 +1302,27 @@ int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
     return 0;
 }
 
+int dtrt_scene_set_density(dtrt_object *scene, int medium, const float *density, int on_device) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    return s->setDensity(medium, density, on_device != 0) ? 0 : -1;
+}
+
+int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
+                        const float *d_image_device, float *d_density_device) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !options) { g_err = "render_adjoint: bad arguments"; return -1; }
+    if (integrator != 0) { g_err = "render_adjoint: only VolPathTracerAD (0) is implemented"; return -1; }
+    return s->renderAdjoint(*options, medium, d_image_device, d_density_device) ? 0 : -1;
+}
+
 // ------------------------------------------------------------- test / bench hooks
+int dtrt_test_set_voxel_lanes(dtrt_object *scene, int medium, const int *voxels, int n) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    return s->setVoxelLanes(medium, voxels, n) ? 0 : -1;
+}
+
 int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes) {
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s) return -1;

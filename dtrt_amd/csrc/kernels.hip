@@ -25,6 +25,9 @@ constexpr int BLOCK = TRACE_BLOCK;
 #ifndef AD_WAVES_PER_SIMD
 #define AD_WAVES_PER_SIMD 2
 #endif
+// Wide duals (N >= 8 tangent lanes) take the whole 512-entry register file at one
+// wave per SIMD rather than spilling kilobytes per lane to scratch.
+template <int N> constexpr int adWaves() { return N >= 8 ? 1 : AD_WAVES_PER_SIMD; }
 
 DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
     // Blocks are dealt round-robin over the 8 XCDs; give each XCD a contiguous run
@@ -195,11 +198,228 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
     }
 }
 
+// Passes of the path-replay adjoint (k_adjoint); NONE in the forward kernel.
+enum class AdjPass { NONE, TOTAL, SCATTER };
+struct AdjCtx {
+    float *grad;         // d loss / d node density of the grid of *pm (accumulated)
+    const DMedium *pm;
+    f3 adj;              // d loss / d pixel value of this sample (already / spp)
+    f3 Ltot;             // the sample's radiance, from the TOTAL pass
+};
+
+// One AD path sample of pixel `pixel` (camera ray through the jittered pixel
+// position, then the dual-number estimator of volpathAD.cpp:145-620), accumulated
+// into L. A == SCATTER replays the path of a TOTAL pass with the same stream and adds
+// the reverse-mode density derivative of every contribution into ac->grad.
+template <int N, AdjPass A>
+DI void adPathSample(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, Ds<N> &L, Pusher &u, unsigned *stats,
+                     const AdjCtx *ac) {
+    const DScene &S = T.S;
+    int ix = P.cropped ? P.ox + pixel % P.cw : pixel % P.width;
+    int iy = P.cropped ? P.oy + pixel / P.cw : pixel / P.width;
+    float jx = 0.5f, jy = 0.5f;
+    if (P.sppe >= 0) { jx = r.next(); jy = r.next(); }
+    Dv<N> ro, rd;
+    cameraRayAD<N>(S.cam, ix + jx, iy + jy, ro, rd);
+    HitAD<N> its;
+    if (T.rayIntersectAD(ro, rd, false, its)) {
+        int med = S.cam.med_id, depth = P.max_bounces, mode = P.mode;
+        Ds<N> beta(f3{1, 1, 1});
+        while (mode != 0) {
+            float r1a = r.next(), r1b = r.next();
+            float r2[4] = {r.next(), r.next(), r.next(), r.next()};
+            float r3[3] = {r.next(), r.next(), r.next()};
+            Ds<N> thr(f3{1, 1, 1});
+            bool vol = false;
+            if (med >= 0) {
+                Dv<N> ps;
+                ScatterCtx fs{nullptr, nullptr, 0.f};
+                const ScatterCtx *fsp = nullptr;
+                if constexpr (A == AdjPass::SCATTER) {   // free flight weighs everything from here on
+                    fs = ScatterCtx{ac->grad, ac->pm, dot(ac->adj, ac->Ltot - L.val())};
+                    fsp = &fs;
+                }
+                vol = sampleDistanceAD<N>(S.media[med], ro, rd, its.t, r1a, r1b, r, ps, thr, fsp);
+                if (vol) ro = ps;
+            }
+            if (vol) {
+                if (depth == 0 || thr.isZero(EPS)) break;
+                const DPhase &ph = S.phases[S.media[med].phase_id];
+                f3 rdin = rd.val();
+                if (mode & 8) {   // volume NEE with tangents
+                    f3 wo;
+                    float pdf_nee;
+                    Rng rs;   // stream position before the walk (scatter pass only)
+                    if constexpr (A == AdjPass::SCATTER) rs = r;
+                    Ds<N> value = T.neePointAD(ro, r2, r, med, depth - 1, wo, pdf_nee);
+                    if (!value.isZero(EPS)) {
+                        Df<N> pv = phaseEvalAD<N>(ph, -rd, wo);
+                        if (pv.v > EPS) {
+                            Ds<N> c = beta * (thr * value * (pv * misPower(pdf_nee, phaseEval(ph, -rdin, wo))));
+                            L += c;
+                            if constexpr (A == AdjPass::SCATTER) {   // the shadow walk weighs this term only
+                                ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
+                                Rng r2w = rs;
+                                f3 wo2;
+                                float pdf2;
+                                T.neePointAD(ro, r2, r2w, med, depth - 1, wo2, pdf2, &sc);
+                            }
+                        }
+                    }
+                }
+                f3 wo = phaseSample(ph, -rdin, r3[0], r3[1]);
+                Df<N> pv = phaseSampleWeightAD<N>(ph, -rd, wo);
+                bool cont = false;
+                HitAD<N> its1;
+                Ds<N> thr1;
+                if (pv.v > EPS) {
+                    thr1 = thr * pv;
+                    float pdf_nee = 0.f;
+                    Rng rs;   // stream position before the walk (scatter pass only)
+                    if constexpr (A == AdjPass::SCATTER) rs = r;
+                    Ds<N> att = T.lookForEmitterAD(ro, Dv<N>(wo), false, r, med, depth - 1, its1, pdf_nee, nullptr, nullptr);
+                    if (!att.isZero(EPS) && (mode & 8)) {
+                        Ds<N> c = beta * (thr1 * att * misPower(phaseEval(ph, -wo, wo), pdf_nee));
+                        L += c;
+                        if constexpr (A == AdjPass::SCATTER) {
+                            ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
+                            Rng r2w = rs;
+                            HitAD<N> h2;
+                            float pdf2;
+                            T.lookForEmitterAD(ro, Dv<N>(wo), false, r2w, med, depth - 1, h2, pdf2, nullptr, nullptr, &sc);
+                        }
+                    }
+                    cont = true;
+                }
+                if (P.sppse > 0 && (mode & 32))
+                    edgeTerm<N>(T, P, r, true, ro, nullptr, nullptr, &ph, rdin, thr.val(), beta.val(), med, depth, pixel, u,
+                                stats);
+                if (!cont) break;
+                beta = beta * thr1;
+                rd = Dv<N>(wo);
+                its = its1;
+                depth -= 1;
+                mode &= ~1;
+            } else {
+                if (its.face < 0 || thr.isZero(EPS)) break;
+                const DShape &sh = shapeOf(S, its.face);
+                if (isTransition(sh) && targetMedium(sh, its.gn.val(), -rd.val()) != med) {
+                    atomicAdd(stats + STAT_MED_INCONSISTENT, 1u);
+                    break;
+                }
+                if (sh.light_id >= 0 && (mode & 1)) L += beta * (thr * emitterEvalAD<N>(S.emitters[sh.light_id], its, -rd));
+                if (depth == 0) break;
+                const DBsdf &b = bsdfOf(S, sh);
+                const bool isnull = b.type == BSDF_NULL;
+                Hit its0 = its.val();
+                // Lins boundary term (volpathAD.cpp:321-467): queued as primal sub-paths
+                if (med >= 0 && !its.t.gradZero(EPS) && (mode & 16)) {
+                    int ns = min(max((int)(P.ddist * its.t.gradAbsMax()), 1), 100);
+                    f3 n = dot(its0.gn, rd.val()) < 0.f ? -its0.gn : its0.gn;
+                    f3 sgs = sigS(S.media[med], its0.p - n * SHADOW_EPS);
+                    f3 base = beta.val() * thr.val() * sgs * (1.0f / ns);
+                    float W[N * 3];
+#pragma unroll
+                    for (int j = 0; j < N; ++j) { W[3 * j] = base.x * its.t.d[j]; W[3 * j + 1] = base.y * its.t.d[j]; W[3 * j + 2] = base.z * its.t.d[j]; }
+                    ChildHdr h;
+                    h.org[0] = its0.p.x; h.org[1] = its0.p.y; h.org[2] = its0.p.z;
+                    f3 rv = rd.val();
+                    h.dir[0] = rv.x; h.dir[1] = rv.y; h.dir[2] = rv.z;
+                    h.pixel = pixel;
+                    h.meta = (isnull ? CHILD_LINS_NULL : CHILD_LINS_SURF) | (depth << 4) | ((med + 1) << 12);
+                    h.face = its0.face;
+                    h.gn[0] = its0.gn.x; h.gn[1] = its0.gn.y; h.gn[2] = its0.gn.z;
+                    h.sn[0] = its0.sn.x; h.sn[1] = its0.sn.y; h.sn[2] = its0.sn.z;
+                    for (int s = 0; s < ns; ++s) push(P, u, h, W, N * 3);
+                }
+                FrameAD<N> shf(its.sn);
+                if (!isnull && (mode & 2)) {   // surface NEE with tangents
+                    Dv<N> woL;
+                    float pdf_nee;
+                    Rng rs;   // stream position before the walk (scatter pass only)
+                    if constexpr (A == AdjPass::SCATTER) rs = r;
+                    Ds<N> value = T.neeSurfaceAD(its, shf, r2, r, med, depth - 1, woL, pdf_nee);
+                    if (!value.isZero(EPS)) {
+                        Ds<N> bv = bsdfEvalAD<N>(b, its.wi, woL);
+                        Ds<N> c = beta * (thr * value * bv * misPower(pdf_nee, bsdfPdf(b, its0.wi, woL.val())));
+                        L += c;
+                        if constexpr (A == AdjPass::SCATTER) {
+                            ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
+                            Rng r2w = rs;
+                            Dv<N> wo2;
+                            float pdf2;
+                            T.neeSurfaceAD(its, shf, r2, r2w, med, depth - 1, wo2, pdf2, &sc);
+                        }
+                    }
+                }
+                // BSDF sampling (bsdf.cpp:11-21; null.cpp:18-23)
+                bool cont = false;
+                Dv<N> nd;
+                Ds<N> bw(f3{1, 1, 1});
+                float bpdf = 1.f;
+                if (isnull) {
+                    nd = rd;   // toWorld(-wi) == incoming direction, tangents included
+                    cont = true;
+                } else {
+                    f3 wo0;
+                    f3 w0 = bsdfSample(b, its0.wi, r3[0], r3[1], r3[2], wo0, bpdf);
+                    if (!allZero(w0, EPS) && bpdf > EPS) {
+                        Frame shv = shf.val();
+                        f3 ww = shv.toWorld(wo0);
+                        Dv<N> woL = shf.toLocal(Dv<N>(ww));
+                        bw = bsdfEvalAD<N>(b, its.wi, woL) * (1.0f / bpdf);
+                        if (!bw.isZero(EPS)) { nd = Dv<N>(ww); cont = true; }
+                    }
+                }
+                HitAD<N> its1;
+                Ds<N> thr1 = thr;
+                int med1 = med, mode1 = mode;
+                Dv<N> no = its.p;
+                if (cont) {
+                    med1 = isTransition(sh) ? targetMedium(sh, its0.gn, nd.val()) : med;
+                    if (isnull) {
+                        T.rayIntersectAD(no, nd, true, its1);
+                    } else {
+                        float pdf_nee = 0.f;
+                        Rng rs;   // stream position before the walk (scatter pass only)
+                        if constexpr (A == AdjPass::SCATTER) rs = r;
+                        Ds<N> att = T.lookForEmitterAD(no, nd, true, r, med1, depth - 1, its1, pdf_nee, nullptr, nullptr);
+                        thr1 = thr * bw;
+                        if (!att.isZero(EPS) && (mode & 2)) {
+                            Ds<N> c = beta * (thr1 * att * misPower(bpdf, pdf_nee));
+                            L += c;
+                            if constexpr (A == AdjPass::SCATTER) {
+                                ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
+                                Rng r2w = rs;
+                                HitAD<N> h2;
+                                float pdf2;
+                                T.lookForEmitterAD(no, nd, true, r2w, med1, depth - 1, h2, pdf2, nullptr, nullptr, &sc);
+                            }
+                        }
+                        mode1 = mode & ~1;
+                    }
+                }
+                if (!isnull && P.sppse > 0 && (mode & 4))
+                    edgeTerm<N>(T, P, r, false, its.p, &its0, &b, nullptr, f3{0, 0, 0}, thr.val(), beta.val(), med, depth,
+                                pixel, u, stats);
+                if (!cont) break;
+                beta = beta * thr1;
+                ro = no;
+                rd = nd;
+                its = its1;
+                med = med1;
+                mode = mode1;
+                depth -= 1;
+            }
+        }
+    }
+}
+
 // Normal pass: samples s0 .. s0+nsamp-1. Replay pass (replay != nullptr): entries
 // {sid lo, sid hi, children delivered, 0} of samples whose pushes overflowed.
 // Samples that overflow (again) are appended to ovf.
 template <int N>
-__global__ __launch_bounds__(BLOCK, AD_WAVES_PER_SIMD) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
+__global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
                                                     const uint4 *replay, float *acc, char *q, unsigned *qcount,
                                                     unsigned *stats, uint4 *ovf, unsigned *ovf_count,
                                                     unsigned long long *work) {
@@ -219,154 +439,7 @@ __global__ __launch_bounds__(BLOCK, AD_WAVES_PER_SIMD) void k_ad_paths(DScene S,
     Pusher u{q, qcount, (sid + 1) << 20, 0, skip, Pusher::NO_FAIL};
     Rng r;
     r.init(P.seed, sid);
-    if (active) {
-        int ix = P.cropped ? P.ox + pixel % P.cw : pixel % P.width;
-        int iy = P.cropped ? P.oy + pixel / P.cw : pixel / P.width;
-        float jx = 0.5f, jy = 0.5f;
-        if (P.sppe >= 0) { jx = r.next(); jy = r.next(); }
-        Dv<N> ro, rd;
-        cameraRayAD<N>(S.cam, ix + jx, iy + jy, ro, rd);
-        HitAD<N> its;
-        if (T.rayIntersectAD(ro, rd, false, its)) {
-            int med = S.cam.med_id, depth = P.max_bounces, mode = P.mode;
-            Ds<N> beta(f3{1, 1, 1});
-            while (mode != 0) {
-                float r1a = r.next(), r1b = r.next();
-                float r2[4] = {r.next(), r.next(), r.next(), r.next()};
-                float r3[3] = {r.next(), r.next(), r.next()};
-                Ds<N> thr(f3{1, 1, 1});
-                bool vol = false;
-                if (med >= 0) {
-                    Dv<N> ps;
-                    vol = sampleDistanceAD<N>(S.media[med], ro, rd, its.t, r1a, r1b, r, ps, thr);
-                    if (vol) ro = ps;
-                }
-                if (vol) {
-                    if (depth == 0 || thr.isZero(EPS)) break;
-                    const DPhase &ph = S.phases[S.media[med].phase_id];
-                    f3 rdin = rd.val();
-                    if (mode & 8) {   // volume NEE with tangents
-                        f3 wo;
-                        float pdf_nee;
-                        Ds<N> value = T.neePointAD(ro, r2, r, med, depth - 1, wo, pdf_nee);
-                        if (!value.isZero(EPS)) {
-                            Df<N> pv = phaseEvalAD<N>(ph, -rd, wo);
-                            if (pv.v > EPS) L += beta * (thr * value * (pv * misPower(pdf_nee, phaseEval(ph, -rdin, wo))));
-                        }
-                    }
-                    f3 wo = phaseSample(ph, -rdin, r3[0], r3[1]);
-                    Df<N> pv = phaseSampleWeightAD<N>(ph, -rd, wo);
-                    bool cont = false;
-                    HitAD<N> its1;
-                    Ds<N> thr1;
-                    if (pv.v > EPS) {
-                        thr1 = thr * pv;
-                        float pdf_nee = 0.f;
-                        Ds<N> att = T.lookForEmitterAD(ro, Dv<N>(wo), false, r, med, depth - 1, its1, pdf_nee, nullptr, nullptr);
-                        if (!att.isZero(EPS) && (mode & 8)) L += beta * (thr1 * att * misPower(phaseEval(ph, -wo, wo), pdf_nee));
-                        cont = true;
-                    }
-                    if (P.sppse > 0 && (mode & 32))
-                        edgeTerm<N>(T, P, r, true, ro, nullptr, nullptr, &ph, rdin, thr.val(), beta.val(), med, depth, pixel, u,
-                                    stats);
-                    if (!cont) break;
-                    beta = beta * thr1;
-                    rd = Dv<N>(wo);
-                    its = its1;
-                    depth -= 1;
-                    mode &= ~1;
-                } else {
-                    if (its.face < 0 || thr.isZero(EPS)) break;
-                    const DShape &sh = shapeOf(S, its.face);
-                    if (isTransition(sh) && targetMedium(sh, its.gn.val(), -rd.val()) != med) {
-                        atomicAdd(stats + STAT_MED_INCONSISTENT, 1u);
-                        break;
-                    }
-                    if (sh.light_id >= 0 && (mode & 1)) L += beta * (thr * emitterEvalAD<N>(S.emitters[sh.light_id], its, -rd));
-                    if (depth == 0) break;
-                    const DBsdf &b = bsdfOf(S, sh);
-                    const bool isnull = b.type == BSDF_NULL;
-                    Hit its0 = its.val();
-                    // Lins boundary term (volpathAD.cpp:321-467): queued as primal sub-paths
-                    if (med >= 0 && !its.t.gradZero(EPS) && (mode & 16)) {
-                        int ns = min(max((int)(P.ddist * its.t.gradAbsMax()), 1), 100);
-                        f3 n = dot(its0.gn, rd.val()) < 0.f ? -its0.gn : its0.gn;
-                        f3 sgs = sigS(S.media[med], its0.p - n * SHADOW_EPS);
-                        f3 base = beta.val() * thr.val() * sgs * (1.0f / ns);
-                        float W[N * 3];
-#pragma unroll
-                        for (int j = 0; j < N; ++j) { W[3 * j] = base.x * its.t.d[j]; W[3 * j + 1] = base.y * its.t.d[j]; W[3 * j + 2] = base.z * its.t.d[j]; }
-                        ChildHdr h;
-                        h.org[0] = its0.p.x; h.org[1] = its0.p.y; h.org[2] = its0.p.z;
-                        f3 rv = rd.val();
-                        h.dir[0] = rv.x; h.dir[1] = rv.y; h.dir[2] = rv.z;
-                        h.pixel = pixel;
-                        h.meta = (isnull ? CHILD_LINS_NULL : CHILD_LINS_SURF) | (depth << 4) | ((med + 1) << 12);
-                        h.face = its0.face;
-                        h.gn[0] = its0.gn.x; h.gn[1] = its0.gn.y; h.gn[2] = its0.gn.z;
-                        h.sn[0] = its0.sn.x; h.sn[1] = its0.sn.y; h.sn[2] = its0.sn.z;
-                        for (int s = 0; s < ns; ++s) push(P, u, h, W, N * 3);
-                    }
-                    FrameAD<N> shf(its.sn);
-                    if (!isnull && (mode & 2)) {   // surface NEE with tangents
-                        Dv<N> woL;
-                        float pdf_nee;
-                        Ds<N> value = T.neeSurfaceAD(its, shf, r2, r, med, depth - 1, woL, pdf_nee);
-                        if (!value.isZero(EPS)) {
-                            Ds<N> bv = bsdfEvalAD<N>(b, its.wi, woL);
-                            L += beta * (thr * value * bv * misPower(pdf_nee, bsdfPdf(b, its0.wi, woL.val())));
-                        }
-                    }
-                    // BSDF sampling (bsdf.cpp:11-21; null.cpp:18-23)
-                    bool cont = false;
-                    Dv<N> nd;
-                    Ds<N> bw(f3{1, 1, 1});
-                    float bpdf = 1.f;
-                    if (isnull) {
-                        nd = rd;   // toWorld(-wi) == incoming direction, tangents included
-                        cont = true;
-                    } else {
-                        f3 wo0;
-                        f3 w0 = bsdfSample(b, its0.wi, r3[0], r3[1], r3[2], wo0, bpdf);
-                        if (!allZero(w0, EPS) && bpdf > EPS) {
-                            Frame shv = shf.val();
-                            f3 ww = shv.toWorld(wo0);
-                            Dv<N> woL = shf.toLocal(Dv<N>(ww));
-                            bw = bsdfEvalAD<N>(b, its.wi, woL) * (1.0f / bpdf);
-                            if (!bw.isZero(EPS)) { nd = Dv<N>(ww); cont = true; }
-                        }
-                    }
-                    HitAD<N> its1;
-                    Ds<N> thr1 = thr;
-                    int med1 = med, mode1 = mode;
-                    Dv<N> no = its.p;
-                    if (cont) {
-                        med1 = isTransition(sh) ? targetMedium(sh, its0.gn, nd.val()) : med;
-                        if (isnull) {
-                            T.rayIntersectAD(no, nd, true, its1);
-                        } else {
-                            float pdf_nee = 0.f;
-                            Ds<N> att = T.lookForEmitterAD(no, nd, true, r, med1, depth - 1, its1, pdf_nee, nullptr, nullptr);
-                            thr1 = thr * bw;
-                            if (!att.isZero(EPS) && (mode & 2)) L += beta * (thr1 * att * misPower(bpdf, pdf_nee));
-                            mode1 = mode & ~1;
-                        }
-                    }
-                    if (!isnull && P.sppse > 0 && (mode & 4))
-                        edgeTerm<N>(T, P, r, false, its.p, &its0, &b, nullptr, f3{0, 0, 0}, thr.val(), beta.val(), med, depth,
-                                    pixel, u, stats);
-                    if (!cont) break;
-                    beta = beta * thr1;
-                    ro = no;
-                    rd = nd;
-                    its = its1;
-                    med = med1;
-                    mode = mode1;
-                    depth -= 1;
-                }
-            }
-        }
-    }
+    if (active) adPathSample<N, AdjPass::NONE>(T, P, r, pixel, L, u, stats, nullptr);
     const bool done = u.failed_at == Pusher::NO_FAIL;
     if (active && !done) ovf[atomicAdd(ovf_count, 1u)] = make_uint4((uint32_t)sid, (uint32_t)(sid >> 32), u.failed_at, 0u);
     // accumulate: one atomic per wave when the wave covers one pixel, else per lane
@@ -394,6 +467,59 @@ __global__ __launch_bounds__(BLOCK, AD_WAVES_PER_SIMD) void k_ad_paths(DScene S,
         }
     }
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
+// Path-replay adjoint of the density derivative of the interior terms: for each AD
+// sample a TOTAL pass computes its radiance, then a SCATTER pass replays the same
+// stream and adds d loss / d node density (loss gradient dimg[pixel] / spp) into
+// grad. The host restricts the terms to emission, surface and volume interior
+// (no edge or Lins sub-paths), so nothing is pushed.
+template <int N>
+__global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
+                                                                   const float *dimg, float *grad, int pmed,
+                                                                   unsigned *stats, unsigned long long *work) {
+    const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    Tracer<N> T(S);
+    const uint64_t sid = s0 + li;
+    const int pixel = (int)(sid / (uint64_t)P.spp);
+    Pusher u{nullptr, nullptr, 0, 0, 0, Pusher::NO_FAIL};
+    uint32_t nlk = 0;
+    if (li < nsamp) {
+        const float inv = 1.0f / P.spp;
+        const f3 adj{dimg[3 * (size_t)pixel] * inv, dimg[3 * (size_t)pixel + 1] * inv, dimg[3 * (size_t)pixel + 2] * inv};
+        if (adj.x != 0.f || adj.y != 0.f || adj.z != 0.f) {
+            Rng r;
+            r.init(P.seed, sid);
+            Ds<N> L;
+            adPathSample<N, AdjPass::TOTAL>(T, P, r, pixel, L, u, stats, nullptr);
+            nlk += r.nlk;
+            const f3 Lt = L.val();
+            if (Lt.x != 0.f || Lt.y != 0.f || Lt.z != 0.f) {
+                AdjCtx ac{grad, S.media + pmed, adj, Lt};
+                Rng r2;
+                r2.init(P.seed, sid);
+                Ds<N> L2;
+                adPathSample<N, AdjPass::SCATTER>(T, P, r2, pixel, L2, u, stats, &ac);
+                nlk += r2.nlk;
+            }
+        }
+    }
+    flushWork(work, 0, T.nbox, T.ntri, nlk, T.nseg);
+}
+
+// dst = max(src, 0) (NaN -> 0), and the maximum into *maxbits (non-negative floats
+// order like their bit patterns). dst may alias src.
+__global__ void k_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float v = 0.f;
+    if (i < n) {
+        v = fmaxf(src[i], 0.f);
+        dst[i] = v;
+    }
+    unsigned b = __float_as_uint(v);
+    for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, b);
 }
 
 // Adds the finished sub-paths of this wave: lanes with fin set add W * Lv to their
@@ -736,6 +862,19 @@ hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q
 hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,
                           unsigned *st, unsigned long long *wk, hipStream_t sm) {
     DT_DISPATCH(launchPrimary, S, P, s0, n, total, q, qc, st, wk, sm)
+}
+template <int N> static void launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,
+                                           float *grad, int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, dimg, grad, pmed, st, wk);
+}
+hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,
+                          int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    DT_DISPATCH(launchAdjoint, S, P, s0, n, dimg, grad, pmed, st, wk, sm)
+}
+hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm) {
+    hipLaunchKernelGGL(k_load_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sm, dst, src, n, maxbits);
+    return hipGetLastError();
 }
 hipError_t launch_finalize(const DScene &S, const RenderParams &P, const float *acc, const float *acc_pe, float *out, unsigned *st,
                            hipStream_t sm) {

@@ -46,6 +46,12 @@ hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, u
                           unsigned *st, unsigned long long *work, hipStream_t sm);
 hipError_t launch_finalize(const DScene &S, const RenderParams &P, const float *acc, const float *acc_pe, float *out, unsigned *st,
                            hipStream_t sm);
+// Path-replay adjoint over AD samples s0 .. s0+n-1 (interior terms only): adds
+// d loss / d node density of medium pmed's grid into grad, given d loss / d pixel.
+hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,
+                          int pmed, unsigned *st, unsigned long long *work, hipStream_t sm);
+// dst = max(src, 0) over n floats; atomicMax of the result's bits into *maxbits.
+hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm);
 hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);
 hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm);
 hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out, hipStream_t sm);

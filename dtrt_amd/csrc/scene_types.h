@@ -70,6 +70,10 @@ struct DMedium {
     float center[3], aabb_min[3], aabb_max[3];
     float maxDensity, maxInvDensity, scale;
     float scale_d[DTRT_NDER_MAX], trans_d[DTRT_NDER_MAX][3], rot_d[DTRT_NDER_MAX][3];
+    // optional per-voxel tangent lanes: lane j differentiates w.r.t. the density of
+    // grid node vox_lane[j] (flat [z][y][x] index, -1 = none); vox_any = any set
+    int vox_any;
+    int vox_lane[DTRT_NDER_MAX];
 };
 
 struct DScene {

@@ -109,7 +109,7 @@ template <int N> struct Tracer {
 
     // scene.cpp:703-793 — with tangents; reports the far hit's face and AD distance
     DI Ds<N> lookForEmitterAD(Dv<N> o, const Dv<N> &d, bool onSurface, Rng &r, int med, int max_inter, HitAD<N> &its,
-                              float &pdf_nee, int *far_face, float *far_t) {
+                              float &pdf_nee, int *far_face, float *far_t, const ScatterCtx *scx = nullptr) {
         float tmin = onSurface ? SHADOW_EPS : 0.f, th;
         f3 scat = o.val(), dv = d.val();
         int f = trace(scat, dv, tmin, FINF, th);
@@ -122,7 +122,7 @@ template <int N> struct Tracer {
             const DShape &sh = shapeOf(S, f);
             Df<N> tmax = nh.t;
             dist += nh.t;
-            if (med >= 0) T *= mediumTransmittanceAD<N>(S.media[med], o, d, tmin, tmax, r);
+            if (med >= 0) T *= mediumTransmittanceAD<N>(S.media[med], o, d, tmin, tmax, r, scx);
             if (sh.light_id >= 0) {
                 f3 dd = nh.p.val() - scat;
                 float G = nh.wi.z.v / dot(dd, dd);
@@ -204,7 +204,8 @@ template <int N> struct Tracer {
         return f3{0, 0, 0};
     }
     // scene.cpp:428-454
-    DI Ds<N> neePointAD(const Dv<N> &ps, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf) {
+    DI Ds<N> neePointAD(const Dv<N> &ps, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf,
+                        const ScatterCtx *scx = nullptr) {
         if (S.nlights == 0) return Ds<N>();
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
@@ -215,7 +216,7 @@ template <int N> struct Tracer {
             HitAD<N> inear;
             int ff = -1;
             float ft = 0.f;
-            Ds<N> ret = lookForEmitterAD(ps, Dv<N>(dir), false, r, med, max_inter, inear, pdf, &ff, &ft);
+            Ds<N> ret = lookForEmitterAD(ps, Dv<N>(dir), false, r, med, max_inter, inear, pdf, &ff, &ft, scx);
             if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
                 fabsf(ft - dist) < SHADOW_EPS) {
                 wo = dir;
@@ -226,7 +227,7 @@ template <int N> struct Tracer {
     }
     // scene.cpp:493-524
     DI Ds<N> neeSurfaceAD(const HitAD<N> &its, const FrameAD<N> &shf, const float *r4, Rng &r, int med, int max_inter,
-                          Dv<N> &wo, float &pdf) {
+                          Dv<N> &wo, float &pdf, const ScatterCtx *scx = nullptr) {
         if (S.nlights == 0) return Ds<N>();
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
@@ -242,7 +243,7 @@ template <int N> struct Tracer {
             HitAD<N> inear;
             int ff = -1;
             float ft = 0.f;
-            Ds<N> ret = lookForEmitterAD(its.p, Dv<N>(dir), true, r, med, max_inter, inear, pdf, &ff, &ft);
+            Ds<N> ret = lookForEmitterAD(its.p, Dv<N>(dir), true, r, med, max_inter, inear, pdf, &ff, &ft, scx);
             if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
                 fabsf(ft - dist) < SHADOW_EPS) {
                 wo = shf.toLocal(Dv<N>(dir));

@@ -62,6 +62,9 @@ _P = C.c_void_p
 _lib.dtrt_test_kernel_times.argtypes = [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
 _lib.dtrt_test_scene_info.argtypes = [_P, C.POINTER(C.c_int64)]
 _lib.dtrt_test_set_queue_budget.argtypes = [_P, C.c_uint64]
+_lib.dtrt_test_set_voxel_lanes.argtypes = [_P, C.c_int, C.POINTER(C.c_int), C.c_int]
+_lib.dtrt_scene_set_density.argtypes = [_P, C.c_int, _P, C.c_int]
+_lib.dtrt_render_adjoint.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
 _lib.dtrt_test_shape_edges.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]
 _lib.dtrt_test_trace.argtypes = [_P, _P, C.c_int, _P, _P]
 _lib.dtrt_test_li.argtypes = [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, _P]
@@ -84,6 +87,29 @@ def kernel_times(scene):
     wk = (C.c_uint64 * 12)()
     _chk(_lib.dtrt_test_kernel_times(scene._h, ms, ln, wk), "kernel_times")
     return np.array(ms[:]), np.array(ln[:]), np.array(wk[:], dtype=np.uint64)
+
+
+def set_density(scene, medium, density):
+    """``dtrt_scene_set_density``: a torch tensor (host or device) or numpy array [rz][ry][rx]."""
+    import numpy as np
+    if hasattr(density, "data_ptr"):
+        t = density.detach().contiguous().float()
+        _chk(_lib.dtrt_scene_set_density(scene._h, int(medium), t.data_ptr(), int(t.is_cuda)), "set_density")
+    else:
+        a = np.ascontiguousarray(density, dtype=np.float32)
+        _chk(_lib.dtrt_scene_set_density(scene._h, int(medium), a.ctypes.data, 0), "set_density")
+
+
+def render_adjoint(scene, options, medium, d_image_ptr, d_density_ptr, integrator=0):
+    """``dtrt_render_adjoint``: device pointers to d loss/d image [H][W][3] and the accumulated grid gradient."""
+    st = options._struct()
+    _chk(_lib.dtrt_render_adjoint(integrator, scene._h, C.byref(st), int(medium), d_image_ptr, d_density_ptr), "render_adjoint")
+
+
+def set_voxel_lanes(scene, medium, voxels):
+    """Forward-mode lanes w.r.t. single grid nodes (flat [z][y][x] indices; [] clears)."""
+    v = (C.c_int * max(1, len(voxels)))(*[int(x) for x in voxels])
+    _chk(_lib.dtrt_test_set_voxel_lanes(scene._h, int(medium), v, len(voxels)), "set_voxel_lanes")
 
 
 def set_queue_budget(scene, nbytes):

@@ -126,6 +126,27 @@ int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *o
 /* Same, with `image` in device memory of the current HIP device (no host copies). */
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);
 
+/* Replace the density grid of heterogeneous medium `medium` (index into the scene's
+ * media) with `density` float32[rz][ry][rx] of the same resolution, host memory
+ * (on_device = 0) or device memory (1). Negative values are clamped to 0, as the
+ * VOL reader does (gridvolume.h:88-132), and the majorant is recomputed. Only the
+ * scene's copy changes. No counterpart in the reference, whose grids are fixed at
+ * construction; it is how an optimiser updates the parameters between renders. */
+int dtrt_scene_set_density(dtrt_object *scene, int medium, const float *density, int on_device);
+
+/* Reverse-mode (path-replay) gradient of a loss w.r.t. every node density of
+ * medium `medium`: d_density[z][y][x] += sum over pixels of d loss/d image value
+ * times d image value/d node density. `d_image_device` is float32[H][W][3] (crop
+ * size if cropped), i.e. d loss / d channel 0 of the image dtrt_render writes. The
+ * derivative is that of the forward-mode estimator's interior terms (emission,
+ * surface, volume: EMode bits 1|2|8 of options->mode), with the reference's
+ * delta-tracking and ratio-tracking weights (heterogeneous.cpp:40-141); each AD
+ * sample is evaluated once and replayed from its random stream. Device pointers;
+ * synchronous. No counterpart in the reference, whose forward mode needs one
+ * tangent lane per parameter. */
+int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
+                        const float *d_image_device, float *d_density_device);
+
 /* Counters of the last render on `scene` (the warnings the reference prints):
  * out[0] negative pixels, [1] non-finite tangents, [2] delta-tracking caps,
  * [3] invalid edge weights, [4] non-orthogonal edge tangents, [5] medium

@@ -23,6 +23,11 @@ extern "C" {
  * [8..11] primary-edge kernel; each = box tests, triangle tests, grid lookups,
  * traced segments. */
 int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, uint64_t *work12);
+/* Forward-mode tangent lanes w.r.t. single grid nodes: lane j (< n) of medium
+ * `medium` differentiates w.r.t. the density of node voxels[j] (flat [z][y][x]
+ * index, -1 = none); n = 0 clears. Other tangent sources still add to the lanes.
+ * Used to check dtrt_render_adjoint lane by lane. */
+int dtrt_test_set_voxel_lanes(dtrt_object *scene, int medium, const int *voxels, int n);
 /* Bytes of child-record queue the scene's renders may use (0 = default 6 GiB).
  * A small budget forces the exact overflow-replay path. */
 int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes);

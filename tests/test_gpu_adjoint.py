@@ -1,0 +1,108 @@
+"""Path-replay adjoint of the density grid (dtrt_render_adjoint).
+
+The adjoint is the reverse mode of the forward-mode interior estimator. With the same
+seed it must reproduce, for every grid node, the forward-mode tangent of that node
+(voxel lanes) contracted with the loss gradient: a per-node identity that holds up to
+float summation order. Finite differences with common random numbers check the
+derivative itself.
+"""
+import numpy as np
+import pytest
+
+from dtrt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+INTERIOR = 1 | 2 | 8   # emission, surface interior, volume interior (integratorAD.h:12-24)
+
+
+def _density_scene(nder, res=64, crop=(16, 16, 32, 32), spp=8, grid_res=24):
+    """The heterogeneous-medium scene with every motion tangent removed."""
+    d = scenes.config1(nder=nder, res=res, spp=spp, grid_res=grid_res, crop=crop)
+    for s in d.shapes:
+        if s.velocities is not None:
+            s.velocities = np.zeros_like(s.velocities)
+    d.media[0].trans_vel = np.zeros_like(d.media[0].trans_vel)
+    d.media[0].rot_vel = np.zeros_like(d.media[0].rot_vel)
+    return d
+
+
+def _support_voxels(dens, k, seed):
+    rz, ry, rx = dens.shape
+    zz, yy, xx = np.meshgrid(np.arange(rz), np.arange(ry), np.arange(rx), indexing="ij")
+    ok = (dens > 0.3 * dens.max()) & (dens < 0.95 * dens.max())
+    ok &= (xx > 1) & (xx < rx - 2) & (yy > 1) & (yy < ry - 2) & (zz > 1) & (zz < rz - 2)
+    idx = np.flatnonzero(ok)
+    return np.random.default_rng(seed).choice(idx, size=k, replace=False)
+
+
+def _adjoint(gpu, sc, d, seed, dimg, spp=None):
+    import torch
+    grad = torch.zeros(d.media[0].density.shape, dtype=torch.float32, device="cuda")
+    g = torch.as_tensor(np.ascontiguousarray(dimg, np.float32), device="cuda")
+    opts = gpu.RenderOptions(seed, spp or d.spp, d.max_bounces, 0, 0, True, INTERIOR)
+    gpu.render_adjoint(sc, opts, 0, g.data_ptr(), grad.data_ptr())
+    torch.cuda.synchronize()
+    return grad.cpu().numpy().astype(np.float64)
+
+
+def test_adjoint_equals_forward_voxel_lanes(gpu):
+    d = _density_scene(nder=16)
+    sc = scenes.instantiate(gpu, d)
+    vox = _support_voxels(d.media[0].density, 16, seed=3)
+    gpu.set_voxel_lanes(sc, 0, vox)
+    img = scenes.render(gpu, d, seed=11, scene=sc, mode=INTERIOR, sppe=0, sppse=0).astype(np.float64)
+    dimg = np.random.default_rng(0).normal(size=img.shape[1:]).astype(np.float32)
+    expect = np.array([(dimg * img[1 + j]).sum() for j in range(16)])
+    grad = _adjoint(gpu, sc, d, 11, dimg).ravel()
+    got = grad[vox]
+    assert np.count_nonzero(np.abs(expect) > 1e-4) >= 8
+    scale = np.abs(expect).max()
+    assert np.allclose(got, expect, rtol=2e-3, atol=2e-4 * scale), np.c_[got, expect]
+    # the gradient is not concentrated on the probed nodes only
+    assert np.count_nonzero(grad) > 50 * len(vox)
+
+
+def test_adjoint_matches_crn_finite_differences(gpu):
+    d = _density_scene(nder=1, spp=1024)
+    sc = scenes.instantiate(gpu, d)
+    dens = d.media[0].density.astype(np.float32)
+    rng = np.random.default_rng(1)
+    dimg = rng.normal(size=(d.crop[3], d.crop[2], 3)).astype(np.float32)
+    grads = np.stack([_adjoint(gpu, sc, d, 100 + s, dimg).ravel() for s in range(4)])
+    g = grads.mean(0)
+    cand = _support_voxels(dens, 400, seed=5)
+    vox = cand[np.argsort(-np.abs(g[cand]))[:3]]
+    h = 0.05 * float(dens.max())
+
+    def loss(rho, seed):
+        gpu.set_density(sc, 0, rho)
+        img = scenes.render(gpu, d, seed=seed, scene=sc, mode=INTERIOR, sppe=0, sppse=0)
+        return float((dimg.astype(np.float64) * img[0]).sum())
+
+    for v in vox:
+        fds = []
+        for s in range(4):
+            up, dn = dens.copy(), dens.copy()
+            up.ravel()[v] += h
+            dn.ravel()[v] -= h
+            fds.append((loss(up, 200 + s) - loss(dn, 200 + s)) / (2 * h))
+        fd, se_fd = np.mean(fds), np.std(fds, ddof=1) / 2
+        se_g = grads[:, v].std(ddof=1) / 2
+        assert abs(fd - g[v]) < 4 * np.hypot(se_fd, se_g) + 0.05 * abs(g[v]), (v, fd, se_fd, g[v], se_g)
+    gpu.set_density(sc, 0, dens)
+
+
+def test_autograd_function_backward_is_the_adjoint(gpu):
+    import torch
+    from dtrt_amd.adjoint import render_density
+    d = _density_scene(nder=1, spp=16)
+    sc = scenes.instantiate(gpu, d)
+    rho = torch.tensor(d.media[0].density, device="cuda", requires_grad=True)
+    opts = gpu.RenderOptions(3, d.spp, d.max_bounces, 0, 0, True, INTERIOR)
+    img = render_density(rho, sc, opts, d.out_shape(), adjoint_seed=7)
+    w = torch.linspace(-1, 1, img.numel(), device="cuda").reshape(img.shape)
+    (img * w).sum().backward()
+    direct = _adjoint(gpu, sc, d, 7, w.cpu().numpy(), spp=d.spp)
+    assert np.allclose(rho.grad.cpu().numpy(), direct, rtol=1e-4, atol=1e-6 * np.abs(direct).max())
+    assert np.abs(direct).max() > 0

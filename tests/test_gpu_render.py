@@ -145,3 +145,17 @@ def test_queue_overflow_replay_is_exact(gpu):
     scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
     for lane in range(ref.shape[0]):
         assert np.allclose(got[lane], ref[lane], rtol=1e-3, atol=1e-4 * scale[lane]), lane
+
+
+def test_value_lane_independent_of_lane_count(gpu):
+    """Channel 0 does not depend on how many tangent lanes ride along: the same seed
+    gives the same image at every compiled lane count (all kernel instantiations)."""
+    ref = None
+    for n in (1, 2, 3, 4, 8, 16):
+        d = scenes.config1(nder=n, res=64, spp=8, grid_res=24, crop=(16, 16, 32, 32))
+        img = scenes.render(gpu, d, seed=11, sppe=256)[0].astype(np.float64)
+        assert img.sum() > 0
+        if ref is None:
+            ref = img
+        else:
+            assert np.allclose(img, ref, rtol=1e-5, atol=1e-6), n
