@@ -72,6 +72,69 @@ def cpu_baseline(desc, target_s=15.0):
                       % (side, side, n, t)}
 
 
+def adjoint_leg(spp=64, res=512, grid_res=128):
+    """Time to the full density gradient: one path-replay adjoint pass against forward mode.
+
+    Workload: the heterogeneous-medium scene with a 128^3 grid (2,097,152 densities),
+    512x512, 64 spp, depth 10, interior terms, L2 loss against a render of a different
+    seeded grid. The forward-mode time is one 16-lane pass (16 node lanes) times the
+    ceil(N / 16) passes the full gradient needs. Per sample both evaluate the same
+    derivative estimator (the adjoint is its reverse mode), so equal spp is equal variance.
+    """
+    import numpy as np
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    interior = 1 | 2 | 8
+
+    def desc(nder, seed):
+        d = scenes.config1(nder=nder, res=res, spp=spp, grid_res=grid_res, seed=seed)
+        for sh in d.shapes:
+            if sh.velocities is not None:
+                sh.velocities = np.zeros_like(sh.velocities)
+        d.media[0].trans_vel = np.zeros_like(d.media[0].trans_vel)
+        return d
+
+    def timed_render(sc, d, seed):
+        out = torch.zeros(d.out_shape(), dtype=torch.float32, device="cuda")
+        opts = G.RenderOptions(seed, spp, d.max_bounces, 0, 0, True, interior)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        G.render_device(0, sc, opts, out.data_ptr())
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t
+
+    d8 = desc(1, 8)
+    tgt, _ = timed_render(scenes.instantiate(G, d8), d8, 99)
+    d = desc(1, 7)
+    sc = scenes.instantiate(G, d)
+    img, t_fwd1 = timed_render(sc, d, 1)
+    dimg = (2.0 / img[0].numel()) * (img[0] - tgt[0])
+    grad = torch.zeros(d.media[0].density.shape, dtype=torch.float32, device="cuda")
+    opts = G.RenderOptions(2, spp, d.max_bounces, 0, 0, True, interior)
+    G.render_adjoint(sc, opts, 0, dimg.contiguous().data_ptr(), grad.data_ptr())   # warm-up
+    grad.zero_()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    G.render_adjoint(sc, opts, 0, dimg.contiguous().data_ptr(), grad.data_ptr())
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t
+    nparam = int(np.prod(d.media[0].density.shape))
+    d16 = desc(16, 7)
+    sc16 = scenes.instantiate(G, d16)
+    rng = np.random.default_rng(99)
+    G.set_voxel_lanes(sc16, 0, rng.choice(nparam, 16, replace=False))
+    timed_render(sc16, d16, 1)   # warm-up
+    _, t16 = timed_render(sc16, d16, 1)
+    passes = -(-nparam // 16)
+    return {"workload": "heterogeneous-medium scene, %d^3 density grid (%d params), %dx%d, %d spp, depth %d, "
+                        "interior terms, L2 loss vs a render of grid seed 8" % (grid_res, nparam, res, res, spp, d.max_bounces),
+            "adjoint_s": t_adj, "forward_render_s": t_fwd1, "forward_mode_pass_s_16_lanes": t16,
+            "forward_mode_passes": passes, "forward_mode_total_s": t16 * passes,
+            "adjoint_speedup": t16 * passes / t_adj, "grad_nonzero_nodes": int((grad != 0).sum().item()),
+            "equal_variance": "same per-sample derivative estimator in reverse and forward mode, equal spp"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +143,7 @@ def main():
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--spp", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward gradient timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -143,7 +207,7 @@ def main():
         if (t["width"], t["height"], t["spp"], t["nder"]) == (desc.width, desc.height, desc.spp, desc.nder):
             traffic = t.get(name)
     rec = {
-        "metric": "AD path samples/s at 1 GPU",
+        "metric": "AD path samples/s at 1 GPU; adjoint vs forward grad time at equal variance",
         "value": value,
         "unit": "AD path samples/s",
         "n_gpus": 1,
@@ -168,6 +232,9 @@ def main():
                           for k, b in (("ad_paths", 0), ("children", 4), ("primary_edges", 8))},
         "render_stats": {k: int(v) for k, v in st.items()},
     }
+    if not args.no_adjoint:
+        del scene
+        rec["adjoint"] = adjoint_leg()
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(desc)
         rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
