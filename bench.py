@@ -205,7 +205,7 @@ def main():
     if os.path.exists(tfile):
         t = json.load(open(tfile))
         if (t["width"], t["height"], t["spp"], t["nder"]) == (desc.width, desc.height, desc.spp, desc.nder):
-            traffic = t.get(name)
+            traffic = t.get("%s<%d>" % (name, desc.nder))
     rec = {
         "metric": "AD path samples/s at 1 GPU; adjoint vs forward grad time at equal variance",
         "value": value,
