@@ -135,6 +135,43 @@ def adjoint_leg(spp=64, res=512, grid_res=128):
             "equal_variance": "same per-sample derivative estimator in reverse and forward mode, equal spp"}
 
 
+def views_leg(nviews=32):
+    """Batched multi-view render (dtrt_render_views) against the same views one call each.
+
+    The heterogeneous-medium scene (64^3 grid, nder 3, depth 10, sppse 10, sppe 4096)
+    seen by 32 orbiting cameras. Resolutions are uneven, 128^2..768^2 in steps of 64, and
+    spp is one of 16, 32, 64, all drawn with PCG32 seed 5.
+    """
+    import numpy as np
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    d = scenes.config1(nder=3, res=512, spp=64)
+    sc = scenes.instantiate(G, d)
+    views = scenes.orbit_views(nviews)
+    cams = [G.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
+            for c2w, w, h, _ in views]
+    opts = [G.RenderOptions(100 + i, spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
+            for i, (_, _, _, spp) in enumerate(views)]
+    outs = [torch.zeros((d.nder + 1, h, w, 3), device="cuda") for _, w, h, _ in views]
+    samples = sum(w * h * spp for _, w, h, spp in views)
+    G.render_views(sc, cams, opts, outs)   # warm-up; projects and caches each camera's primary edges
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    G.render_views(sc, cams, opts, outs)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t
+    t = time.perf_counter()
+    for i in range(nviews):
+        G.render_views(sc, [cams[i]], [opts[i]], [outs[i]])
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    return {"workload": "%d orbit views of the heterogeneous-medium scene, res %d^2..%d^2, spp 16/32/64 (PCG32 seed 5), "
+                        "nder 3, depth 10, sppse 10, sppe 4096" % (nviews, min(v[1] for v in views), max(v[1] for v in views)),
+            "samples": samples, "batched_s": t_batch, "one_call_per_view_s": t_seq,
+            "batched_samples_per_s": samples / t_batch, "batching_speedup": t_seq / t_batch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +181,7 @@ def main():
     ap.add_argument("--spp", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward gradient timing")
+    ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -232,8 +270,10 @@ def main():
                           for k, b in (("ad_paths", 0), ("children", 4), ("primary_edges", 8))},
         "render_stats": {k: int(v) for k, v in st.items()},
     }
+    del scene
+    if not args.no_views:
+        rec["views"] = views_leg()
     if not args.no_adjoint:
-        del scene
         rec["adjoint"] = adjoint_leg()
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(desc)

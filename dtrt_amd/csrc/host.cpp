@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -81,8 +82,11 @@ bool invert4(const double m[4][4], double o[4][4]) {
 }
 
 // ------------------------------------------------------------------ objects
+uint64_t g_camera_uid = 0;   // identifies a camera configuration (primary edges are cached by it)
+
 struct Camera : Obj {
-    Camera() : Obj(T_CAMERA) {}
+    Camera() : Obj(T_CAMERA), uid(++g_camera_uid) {}
+    uint64_t uid;
     int w = 0, h = 0, ox = 0, oy = 0, cw = -1, ch = -1, med = -1, nder = 0;
     float clip_near = 0;
     double c2w[4][4], w2c[4][4], c2ndc[3][3], ndc2c[3][3];
@@ -484,6 +488,21 @@ int silhouetteHost(const Shape &s, const EdgeH &e, D3 p) {
 }
 
 // ------------------------------------------------------------------- scene
+// camera.cpp:8-83 data for the device (ray generation in float, tangents per lane)
+void fillDCamera(const Camera &cam, int nder, DCamera &c) {
+    std::memset(&c, 0, sizeof(c));
+    c.width = cam.w; c.height = cam.h; c.ox = cam.ox; c.oy = cam.oy; c.cw = cam.cw; c.ch = cam.ch;
+    c.cropped = cam.cropped(); c.med_id = cam.med;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) c.c2w[4 * i + j] = (float)cam.c2w[i][j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c.ndc2c[3 * i + j] = (float)cam.ndc2c[i][j];
+    c.fov = (float)cam.c2ndc[0][0];
+    c.aspect = (float)cam.w / (float)cam.h;
+    c.clip_near = cam.clip_near;
+    auto put = [](float *dst, D3 v) { dst[0] = (float)v.x; dst[1] = (float)v.y; dst[2] = (float)v.z; };
+    put(c.cpos, cam.cpos); put(c.cs, cam.cs); put(c.ct, cam.ct); put(c.cn, cam.cn);
+    for (int j = 0; j < nder; ++j) { put(c.cpos_d[j], cam.cpos_d[j]); put(c.cs_d[j], cam.cs_d[j]); put(c.ct_d[j], cam.ct_d[j]); put(c.cn_d[j], cam.cn_d[j]); }
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -494,7 +513,8 @@ struct Scene : Obj {
     ~Scene() override {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
-        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1]}) if (b->p) (void)hipFree(b->p);
+        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf})
+            if (b->p) (void)hipFree(b->p);
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -507,8 +527,10 @@ struct Scene : Obj {
     std::vector<Medium *> media;
     DScene S;
     std::vector<DevBuf> bufs;
-    DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf, ovf[2];
+    DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf, ovf[2], viewbuf;
     std::vector<DMedium> hmed;   // host copy of the device media records
+    struct PECache { int npe = 0; double pe_sum = 0; const double *cdf = nullptr; const float *pe = nullptr; };
+    std::map<uint64_t, PECache> pecache;   // primary edges per camera configuration (Camera::uid)
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;
     // last render
@@ -537,7 +559,15 @@ struct Scene : Obj {
     }
     bool build();
     bool initEdges(const float *weights);
-    bool render(const dtrt_render_options &o, float *out, bool out_is_device);
+    bool render(const dtrt_render_options &o, float *out, bool out_is_device) {
+        const Camera *c = &cam;
+        float *outs[1] = {out};
+        return renderViews(1, &c, &o, outs, out_is_device);
+    }
+    bool renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs, bool on_device);
+    bool primaryEdges(const Camera &c, PECache &out);
+    bool setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
+                    std::vector<uint64_t> &pe_total);
     bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
     bool setDensity(int med, const float *density, bool on_device);
     bool setVoxelLanes(int med, const int *vox, int n);
@@ -698,19 +728,6 @@ bool Scene::build() {
     S.light_areas = upload(lareas);
     S.area_cdf_begin = upload(abeg);
     S.area_cdf_pool = upload(pool);
-    // camera
-    DCamera &c = S.cam;
-    std::memset(&c, 0, sizeof(c));
-    c.width = cam.w; c.height = cam.h; c.ox = cam.ox; c.oy = cam.oy; c.cw = cam.cw; c.ch = cam.ch;
-    c.cropped = cam.cropped(); c.med_id = cam.med;
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) c.c2w[4 * i + j] = (float)cam.c2w[i][j];
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c.ndc2c[3 * i + j] = (float)cam.ndc2c[i][j];
-    c.fov = (float)cam.c2ndc[0][0];
-    c.aspect = (float)cam.w / (float)cam.h;
-    c.clip_near = cam.clip_near;
-    auto put = [](float *dst, D3 v) { dst[0] = (float)v.x; dst[1] = (float)v.y; dst[2] = (float)v.z; };
-    put(c.cpos, cam.cpos); put(c.cs, cam.cs); put(c.ct, cam.ct); put(c.cn, cam.cn);
-    for (int j = 0; j < nder; ++j) { put(c.cpos_d[j], cam.cpos_d[j]); put(c.cs_d[j], cam.cs_d[j]); put(c.ct_d[j], cam.ct_d[j]); put(c.cn_d[j], cam.cn_d[j]); }
     for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
     return initEdges(nullptr);
 }
@@ -729,7 +746,15 @@ bool Scene::initEdges(const float *weights) {
     S.edges_valid = s > 1e-7;
     if (S.edges_valid) { for (double &x : w) x /= s; w.back() = 1.0; }
     S.shape_edge_cdf = upload(w);
-    // primary edges
+    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
+    return true;
+}
+
+// edge_manager.cpp:6-134 for one camera: silhouette edges of non-null or light shapes,
+// projected and clipped to the camera's window, sampled in proportion to screen length.
+bool Scene::primaryEdges(const Camera &cam, PECache &out) {
+    auto it = pecache.find(cam.uid);
+    if (it != pecache.end()) { out = it->second; return true; }
     std::vector<PEdge> pe;
     std::vector<double> pw;
     for (const Shape *sh : shapes) {
@@ -743,11 +768,12 @@ bool Scene::initEdges(const float *weights) {
             projectEdge(cam, a, b, nder, pe, pw);
         }
     }
-    S.npe = (int)pe.size();
+    PECache c;
+    c.npe = (int)pe.size();
     std::vector<double> cdf(pe.size() + 1, 0.0);
     for (size_t i = 0; i < pw.size(); ++i) cdf[i + 1] = cdf[i] + pw[i];
-    S.pe_sum = cdf.back();
-    if (S.pe_sum > 0) { for (double &x : cdf) x /= S.pe_sum; cdf.back() = 1.0; }
+    c.pe_sum = cdf.back();
+    if (c.pe_sum > 0) { for (double &x : cdf) x /= c.pe_sum; cdf.back() = 1.0; }
     const int stride = 16 + 4 * nder;
     std::vector<float> packed(pe.size() * stride, 0.f);
     for (size_t i = 0; i < pe.size(); ++i) {
@@ -761,28 +787,88 @@ bool Scene::initEdges(const float *weights) {
             p[18 + 4 * j] = (float)e.s1.dx[j]; p[19 + 4 * j] = (float)e.s1.dy[j];
         }
     }
-    S.pe_cdf = upload(cdf);
-    S.pe = packed.empty() ? nullptr : upload(packed);
-    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
+    c.cdf = upload(cdf);
+    c.pe = packed.empty() ? nullptr : upload(packed);
+    if (!c.cdf || (!packed.empty() && !c.pe)) return fail("scene: device allocation failed");
+    pecache[cam.uid] = c;
+    out = c;
     return true;
 }
 
 constexpr uint64_t MAX_CHUNK = uint64_t(1) << 22;    // AD samples per launch
+
+// Per-view device records: camera, sample and pixel offsets, seed, primary edges.
+bool Scene::setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
+                       std::vector<uint64_t> &pe_total) {
+    hv.assign(nv, DView{});
+    pe_total.assign(nv, 0);
+    uint64_t s0 = 0;
+    int64_t pix0 = 0;
+    for (int v = 0; v < nv; ++v) {
+        const Camera &c = *cams[v];
+        const dtrt_render_options &o = opts[v];
+        if (c.nder != nder) return fail("render: a view's camera was created with another nder");
+        if (c.med >= (int)media.size()) return fail("render: a view's camera medium is out of range");
+        if (o.spp < 0) return fail("render: negative spp");
+        DView &d = hv[v];
+        fillDCamera(c, nder, d.cam);
+        d.npix = c.cropped() ? c.cw * c.ch : c.w * c.h;
+        d.pix0 = (int)pix0;
+        d.s0 = s0;
+        d.seed = o.seed;
+        d.spp = o.spp;
+        d.sppe = o.sppe;
+        PECache pe;
+        if (o.sppe > 0) {
+            if (!primaryEdges(c, pe)) return false;
+            d.npe = pe.npe;
+            d.pe_sum = pe.pe_sum;
+            d.pe_cdf = pe.cdf;
+            d.pe = pe.pe;
+            if (pe.npe > 0) pe_total[v] = (uint64_t)std::ceil(pe.pe_sum * o.sppe / 128.0) * 128;
+        }
+        s0 += (uint64_t)d.npix * o.spp;
+        pix0 += d.npix;
+        if (pix0 > 0x7FFFFFFF / 4) return fail("render: too many pixels in one call");
+    }
+    if (!ensure(viewbuf, (size_t)nv * sizeof(DView))) return false;
+    HIPCHK(hipMemcpyAsync(viewbuf.p, hv.data(), (size_t)nv * sizeof(DView), hipMemcpyHostToDevice, stream));
+    S.views = static_cast<const DView *>(viewbuf.p);
+    S.nviews = nv;
+    return true;
+}
+
 constexpr uint64_t FIRST_CHUNK = uint64_t(1) << 16;  // before the children-per-sample rate is known
 
-bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device) {
+// All views in one pass: their AD samples form one global sample space (launched in
+// the same chunks), their primary-edge samples follow view by view, and each view's
+// pixels occupy their own range of the accumulators. Every view's image equals what
+// a render of that view alone gives (same streams), up to float summation order.
+bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs,
+                        bool out_is_device) {
+    if (nv <= 0) return fail("render: no views");
+    const dtrt_render_options &o0 = opts[0];
+    for (int v = 1; v < nv; ++v)
+        if (opts[v].max_bounces != o0.max_bounces || opts[v].mode != o0.mode || opts[v].sppse != o0.sppse ||
+            opts[v].ddist_coeff != o0.ddist_coeff)
+            return fail("render: views of one call must share max_bounces, mode, sppse and ddistCoeff");
     RenderParams P{};
-    P.seed = o.seed; P.spp = o.spp; P.max_bounces = o.max_bounces; P.sppe = o.sppe; P.sppse = std::max(0, o.sppse);
-    P.mode = o.mode >= 0 ? o.mode : 0x3F; P.ddist = o.ddist_coeff;
-    P.width = cam.w; P.height = cam.h; P.ox = cam.ox; P.oy = cam.oy; P.cw = cam.cw; P.ch = cam.ch; P.cropped = cam.cropped();
-    P.npix = P.cropped ? cam.cw * cam.ch : cam.w * cam.h;
+    P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
+    P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
     P.rec_stride = (int)((sizeof(ChildHdr) + 12 * nder + 15) / 16 * 16);
+    std::vector<DView> hv;
+    std::vector<uint64_t> pe_total;
+    if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
+    int64_t npix_all = 0;
+    uint64_t total = 0, pe_all = 0;
+    for (int v = 0; v < nv; ++v) {
+        npix_all += hv[v].npix;
+        total += (uint64_t)hv[v].npix * hv[v].spp;
+        pe_all += pe_total[v];
+    }
+    P.npix = (int)npix_all;
     const size_t npix = P.npix, img = (size_t)(nder + 1) * npix * 3;
-    const uint64_t total = (uint64_t)npix * std::max(0, P.spp);
-    uint64_t pe_total = 0;
-    if (P.sppe > 0 && S.npe > 0) pe_total = (uint64_t)std::ceil(S.pe_sum * P.sppe / 128.0) * 128;
-    P.has_pe = pe_total > 0;
     // One queue of child records for both passes. AD samples are launched in chunks
     // sized from the children-per-sample rate seen so far; a sample whose pushes
     // overflow is replayed exactly (see k_ad_paths). A primary-edge sample pushes at
@@ -799,11 +885,19 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
     HIPCHK(hipMemsetAsync(acc_pe.p, 0, (size_t)nder * npix * 3 * 4, stream));
     HIPCHK(hipMemsetAsync(stats.p, 0, 64 * 4, stream));
     HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
-    float *dout = out;
-    if (!out_is_device) {
+    // per-view output images in device memory (staged through outbuf for host outputs)
+    std::vector<float *> dout(nv);
+    if (out_is_device) {
+        for (int v = 0; v < nv; ++v) dout[v] = outs[v];
+    } else {
         if (!ensure(outbuf, img * 4)) return false;
-        dout = static_cast<float *>(outbuf.p);
-        HIPCHK(hipMemcpyAsync(dout, out, img * 4, hipMemcpyHostToDevice, stream));
+        size_t off = 0;
+        for (int v = 0; v < nv; ++v) {
+            dout[v] = static_cast<float *>(outbuf.p) + off;
+            const size_t n = (size_t)(nder + 1) * hv[v].npix * 3;
+            HIPCHK(hipMemcpyAsync(dout[v], outs[v], n * 4, hipMemcpyHostToDevice, stream));
+            off += n;
+        }
     }
     if (ev.empty()) {
         ev.resize(2);
@@ -866,17 +960,24 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
         uint64_t next = rate > 0 ? (uint64_t)(0.75 * (double)cap / rate) : max_chunk;
         chunk = std::max<uint64_t>(64, std::min(next, max_chunk) / 64 * 64);
     }
-    for (uint64_t s0 = 0; s0 < pe_total; s0 += pe_chunk) {
-        uint64_t n = std::min(pe_chunk, pe_total - s0);
-        HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
-        if (!timed(2, [&] { return launch_primary(S, P, s0, n, pe_total, Q, qc, st, wk, stream); })) return false;
-        if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
-        unsigned h[2];
-        if (!counts(h)) return false;
-        children += std::min<uint64_t>(h[0], cap);
-    }
-    if (!timed(3, [&] { return launch_finalize(S, P, A, APE, dout, st, stream); })) return false;
-    if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, img * 4, hipMemcpyDeviceToHost, stream));
+    for (int v = 0; v < nv; ++v)
+        for (uint64_t s0 = 0; s0 < pe_total[v]; s0 += pe_chunk) {
+            uint64_t n = std::min(pe_chunk, pe_total[v] - s0);
+            HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
+            if (!timed(2, [&] { return launch_primary(S, P, v, s0, n, pe_total[v], Q, qc, st, wk, stream); })) return false;
+            if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
+            unsigned h[2];
+            if (!counts(h)) return false;
+            children += std::min<uint64_t>(h[0], cap);
+        }
+    for (int v = 0; v < nv; ++v)
+        if (!timed(3, [&] {
+                return launch_finalize(S, P, hv[v].npix, hv[v].pix0, hv[v].spp, pe_total[v] > 0, A, APE, dout[v], st, stream);
+            }))
+            return false;
+    if (!out_is_device)
+        for (int v = 0; v < nv; ++v)
+            HIPCHK(hipMemcpyAsync(outs[v], dout[v], (size_t)(nder + 1) * hv[v].npix * 3 * 4, hipMemcpyDeviceToHost, stream));
     unsigned hst[STAT_COUNT];
     unsigned long long hw[12];
     HIPCHK(hipMemcpyAsync(hst, st, sizeof(hst), hipMemcpyDeviceToHost, stream));
@@ -886,7 +987,7 @@ bool Scene::render(const dtrt_render_options &o, float *out, bool out_is_device)
     last_stats[6] = (int64_t)replayed;
     last_stats[7] = (int64_t)total;
     last_stats[8] = (int64_t)children;
-    last_stats[9] = (int64_t)pe_total;
+    last_stats[9] = (int64_t)pe_all;
     for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
     if (hst[STAT_QUEUE_OVERFLOW]) return fail("render: primary-edge child queue overflow");
     return true;
@@ -935,12 +1036,17 @@ bool Scene::setVoxelLanes(int med, const int *vox, int n) {
 bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad) {
     if (!gridMedium(med) || !dimg || !grad) return false;
     RenderParams P{};
-    P.seed = o.seed; P.spp = o.spp; P.max_bounces = o.max_bounces; P.sppe = o.sppe; P.sppse = 0;
+    P.max_bounces = o.max_bounces; P.sppse = 0;
     P.mode = (o.mode >= 0 ? o.mode : 0x3F) & (1 | 2 | 8);
     P.ddist = 0.f;
-    P.width = cam.w; P.height = cam.h; P.ox = cam.ox; P.oy = cam.oy; P.cw = cam.cw; P.ch = cam.ch; P.cropped = cam.cropped();
-    P.npix = P.cropped ? cam.cw * cam.ch : cam.w * cam.h;
-    if (P.spp <= 0 || P.max_bounces < 0 || P.max_bounces > 200) return fail("adjoint: bad options");
+    if (o.spp <= 0 || o.max_bounces < 0 || o.max_bounces > 200) return fail("adjoint: bad options");
+    dtrt_render_options ov = o;
+    ov.sppe = o.sppe < 0 ? -1 : 0;   // pixel jitter as in the forward render; no primary-edge pass
+    const Camera *c = &cam;
+    std::vector<DView> hv;
+    std::vector<uint64_t> pe_total;
+    if (!setupViews(1, &c, &ov, hv, pe_total)) return false;
+    P.npix = hv[0].npix;
     if (!ensure(stats, 64 * 4) || !ensure(work, 16 * 8)) return false;
     HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
     if (ev.empty()) {
@@ -949,7 +1055,7 @@ bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *di
     }
     unsigned *st = static_cast<unsigned *>(stats.p) + 32;   // warnings of the replays are not reported
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
-    const uint64_t total = (uint64_t)P.npix * P.spp, chunk = uint64_t(1) << 22;
+    const uint64_t total = (uint64_t)P.npix * o.spp, chunk = uint64_t(1) << 22;
     std::memset(kernel_ms, 0, sizeof(kernel_ms));
     std::memset(kernel_launches, 0, sizeof(kernel_launches));
     HIPCHK(hipEventRecord(ev[0], stream));
@@ -1102,6 +1208,7 @@ int dtrt_camera_set_rect(dtrt_object *cam, int ox, int oy, int w, int h) {
     Camera *c = as<Camera>(cam, T_CAMERA);
     if (!c) return -1;
     c->ox = ox; c->oy = oy; c->cw = w; c->ch = h;
+    c->uid = ++g_camera_uid;   // the crop clips the primary edges
     if (c->cropped() && (ox < 0 || oy < 0 || ox + w > c->w || oy + h > c->h)) { g_err = "set_rect: crop outside the image"; return -1; }
     return 0;
 }
@@ -1278,6 +1385,29 @@ int dtrt_scene_init_edges(dtrt_object *scene, const float *w) {
     return s->initEdges(w) ? 0 : -1;
 }
 
+int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                      const dtrt_render_options *options, float *const *images, int images_on_device) {
+    if (integrator != 0) { g_err = "render_views: only VolPathTracerAD (integrator 0) is provided"; return -2; }
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || nviews <= 0 || !options || !images) { if (s) g_err = "render_views: bad arguments"; return -1; }
+    std::vector<const Camera *> cams(nviews);
+    for (int v = 0; v < nviews; ++v) {
+        if (!images[v]) { g_err = "render_views: null image"; return -1; }
+        if (cameras && cameras[v]) {
+            cams[v] = as<Camera>(cameras[v], T_CAMERA);
+            if (!cams[v]) return -1;
+        } else {
+            cams[v] = &s->cam;
+        }
+    }
+    try {
+        return s->renderViews(nviews, cams.data(), options, images, images_on_device != 0) ? 0 : -1;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
     if (integrator != 0) { g_err = "render: only VolPathTracerAD (integrator 0) is provided"; return -2; }
     Scene *s = as<Scene>(scene, T_SCENE);
@@ -1342,8 +1472,10 @@ int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8) {
     if (!s) return -1;
     int64_t ne = 0;
     for (Shape *sh : s->shapes) ne += (int64_t)sh->edges.size();
-    out8[0] = s->S.nnodes; out8[1] = s->S.ntris; out8[2] = ne; out8[3] = s->S.npe;
-    std::memcpy(&out8[4], &s->S.pe_sum, 8);
+    Scene::PECache pe;
+    if (!s->primaryEdges(s->cam, pe)) return -1;
+    out8[0] = s->S.nnodes; out8[1] = s->S.ntris; out8[2] = ne; out8[3] = pe.npe;
+    std::memcpy(&out8[4], &pe.pe_sum, 8);
     out8[5] = s->S.edges_valid; out8[6] = s->S.NV; out8[7] = s->S.NF;
     return 0;
 }

@@ -37,6 +37,17 @@ DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
     return (b % X) * (nb / X) + b / X;
 }
 
+// View of global AD sample `sid` (views are ordered by their first sample, s0 = 0).
+DI int viewOf(const DScene &S, uint64_t sid) {
+    int lo = 0, hi = S.nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.views[mid].s0 <= sid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 DI bool writeRecord(const RenderParams &P, char *q, unsigned *qcount, const ChildHdr &h, const float *w, int nw) {
     unsigned idx = atomicAdd(qcount, 1u);
     if (idx >= P.qcap) return false;
@@ -56,6 +67,7 @@ DI bool writeRecord(const RenderParams &P, char *q, unsigned *qcount, const Chil
 struct Pusher {
     char *q;
     unsigned *qcount;
+    int gpix, view;     // global pixel and view of the sample (written into every record)
     uint64_t key;
     uint32_t child, skip, failed_at;   // failed_at: children delivered before the first failed push, or NO_FAIL
     static constexpr uint32_t NO_FAIL = 0xFFFFFFFFu;
@@ -67,6 +79,8 @@ DI void push(const RenderParams &P, Pusher &u, ChildHdr &h, const float *w, int 
     const uint64_t ck = u.key + k;
     h.key_lo = (uint32_t)ck;
     h.key_hi = (uint32_t)(ck >> 32);
+    h.pixel = u.gpix;
+    h.view = u.view;
     if (!writeRecord(P, u.q, u.qcount, h, w, nw)) u.failed_at = k - 1;
 }
 
@@ -184,7 +198,6 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
         ChildHdr h;
         f3 org = x.val();
         h.org[0] = org.x; h.org[1] = org.y; h.org[2] = org.z;
-        h.pixel = pixel;
         h.meta = CHILD_LI | ((depth - 1) << 4) | ((med1 + 1) << 12);
         h.face = -1;
         for (int s = 0; s < 2; ++s) {
@@ -212,18 +225,19 @@ struct AdjCtx {
 // into L. A == SCATTER replays the path of a TOTAL pass with the same stream and adds
 // the reverse-mode density derivative of every contribution into ac->grad.
 template <int N, AdjPass A>
-DI void adPathSample(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, Ds<N> &L, Pusher &u, unsigned *stats,
-                     const AdjCtx *ac) {
+DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, Ds<N> &L, Pusher &u,
+                     unsigned *stats, const AdjCtx *ac) {
     const DScene &S = T.S;
-    int ix = P.cropped ? P.ox + pixel % P.cw : pixel % P.width;
-    int iy = P.cropped ? P.oy + pixel / P.cw : pixel / P.width;
+    const DCamera &cam = vw.cam;
+    int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
+    int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
     float jx = 0.5f, jy = 0.5f;
-    if (P.sppe >= 0) { jx = r.next(); jy = r.next(); }
+    if (vw.sppe >= 0) { jx = r.next(); jy = r.next(); }
     Dv<N> ro, rd;
-    cameraRayAD<N>(S.cam, ix + jx, iy + jy, ro, rd);
+    cameraRayAD<N>(cam, ix + jx, iy + jy, ro, rd);
     HitAD<N> its;
     if (T.rayIntersectAD(ro, rd, false, its)) {
-        int med = S.cam.med_id, depth = P.max_bounces, mode = P.mode;
+        int med = cam.med_id, depth = P.max_bounces, mode = P.mode;
         Ds<N> beta(f3{1, 1, 1});
         while (mode != 0) {
             float r1a = r.next(), r1b = r.next();
@@ -325,7 +339,6 @@ DI void adPathSample(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, Ds<
                     h.org[0] = its0.p.x; h.org[1] = its0.p.y; h.org[2] = its0.p.z;
                     f3 rv = rd.val();
                     h.dir[0] = rv.x; h.dir[1] = rv.y; h.dir[2] = rv.z;
-                    h.pixel = pixel;
                     h.meta = (isnull ? CHILD_LINS_NULL : CHILD_LINS_SURF) | (depth << 4) | ((med + 1) << 12);
                     h.face = its0.face;
                     h.gn[0] = its0.gn.x; h.gn[1] = its0.gn.y; h.gn[2] = its0.gn.z;
@@ -435,35 +448,39 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
         sid = (uint64_t)e.x | ((uint64_t)e.y << 32);
         skip = e.z;
     }
-    const int pixel = (int)(sid / (uint64_t)P.spp);
-    Pusher u{q, qcount, (sid + 1) << 20, 0, skip, Pusher::NO_FAIL};
+    const int v = viewOf(S, sid);
+    const DView &vw = S.views[v];
+    const uint64_t ls = sid - vw.s0;   // sample index within the view: keys its random stream
+    const int pixel = (int)(ls / (uint64_t)vw.spp), gpix = vw.pix0 + pixel;
+    Pusher u{q, qcount, gpix, v, (ls + 1) << 20, 0, skip, Pusher::NO_FAIL};
     Rng r;
-    r.init(P.seed, sid);
-    if (active) adPathSample<N, AdjPass::NONE>(T, P, r, pixel, L, u, stats, nullptr);
+    r.init(vw.seed, ls);
+    if (active) adPathSample<N, AdjPass::NONE>(T, P, vw, r, pixel, L, u, stats, nullptr);
     const bool done = u.failed_at == Pusher::NO_FAIL;
     if (active && !done) ovf[atomicAdd(ovf_count, 1u)] = make_uint4((uint32_t)sid, (uint32_t)(sid >> 32), u.failed_at, 0u);
     // accumulate: one atomic per wave when the wave covers one pixel, else per lane
     const int nd = P.npix;
     const bool put = active && done;
-    if (P.spp % BLOCK == 0 && !replay) {
-        float v[3 * (N + 1)];
+    const int gp0 = __shfl(gpix, 0);
+    if (!replay && __all(!active || gpix == gp0)) {   // the wave covers one pixel
+        float sum[3 * (N + 1)];
         for (int c = 0; c < 3; ++c) {
-            v[c] = put ? L.c[c].v : 0.f;
+            sum[c] = put ? L.c[c].v : 0.f;
 #pragma unroll
-            for (int j = 0; j < N; ++j) v[3 + 3 * j + c] = put ? L.c[c].d[j] : 0.f;
+            for (int j = 0; j < N; ++j) sum[3 + 3 * j + c] = put ? L.c[c].d[j] : 0.f;
         }
 #pragma unroll
         for (int k = 0; k < 3 * (N + 1); ++k)
-            for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+            for (int off = 32; off > 0; off >>= 1) sum[k] += __shfl_xor(sum[k], off);
         if (threadIdx.x == 0 && active)
             for (int k = 0; k < 3 * (N + 1); ++k)
-                if (v[k] != 0.f) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + pixel) * 3 + (k % 3), v[k]);
+                if (sum[k] != 0.f) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + gpix) * 3 + (k % 3), sum[k]);
     } else if (put) {
         for (int c = 0; c < 3; ++c) {
-            if (L.c[c].v != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + c, L.c[c].v);
+            if (L.c[c].v != 0.f) unsafeAtomicAdd(acc + (size_t)gpix * 3 + c, L.c[c].v);
 #pragma unroll
             for (int j = 0; j < N; ++j)
-                if (L.c[c].d[j] != 0.f) unsafeAtomicAdd(acc + ((size_t)(j + 1) * nd + pixel) * 3 + c, L.c[c].d[j]);
+                if (L.c[c].d[j] != 0.f) unsafeAtomicAdd(acc + ((size_t)(j + 1) * nd + gpix) * 3 + c, L.c[c].d[j]);
         }
     }
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
@@ -482,25 +499,26 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
     Tracer<N> T(S);
     const uint64_t sid = s0 + li;
-    const int pixel = (int)(sid / (uint64_t)P.spp);
-    Pusher u{nullptr, nullptr, 0, 0, 0, Pusher::NO_FAIL};
+    const DView &vw = S.views[0];
+    const int pixel = (int)(sid / (uint64_t)vw.spp);
+    Pusher u{nullptr, nullptr, pixel, 0, 0, 0, 0, Pusher::NO_FAIL};
     uint32_t nlk = 0;
     if (li < nsamp) {
-        const float inv = 1.0f / P.spp;
+        const float inv = 1.0f / vw.spp;
         const f3 adj{dimg[3 * (size_t)pixel] * inv, dimg[3 * (size_t)pixel + 1] * inv, dimg[3 * (size_t)pixel + 2] * inv};
         if (adj.x != 0.f || adj.y != 0.f || adj.z != 0.f) {
             Rng r;
-            r.init(P.seed, sid);
+            r.init(vw.seed, sid);
             Ds<N> L;
-            adPathSample<N, AdjPass::TOTAL>(T, P, r, pixel, L, u, stats, nullptr);
+            adPathSample<N, AdjPass::TOTAL>(T, P, vw, r, pixel, L, u, stats, nullptr);
             nlk += r.nlk;
             const f3 Lt = L.val();
             if (Lt.x != 0.f || Lt.y != 0.f || Lt.z != 0.f) {
                 AdjCtx ac{grad, S.media + pmed, adj, Lt};
                 Rng r2;
-                r2.init(P.seed, sid);
+                r2.init(vw.seed, sid);
                 Ds<N> L2;
-                adPathSample<N, AdjPass::SCATTER>(T, P, r2, pixel, L2, u, stats, &ac);
+                adPathSample<N, AdjPass::SCATTER>(T, P, vw, r2, pixel, L2, u, stats, &ac);
                 nlk += r2.nlk;
             }
         }
@@ -602,7 +620,7 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene
                 if (!alive && my < n) {
                     rec = my;
                     const ChildHdr h = *reinterpret_cast<const ChildHdr *>(q + (size_t)rec * P.rec_stride);
-                    r.init(P.seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
+                    r.init(S.views[h.view].seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
                     const int kind = h.meta & 3, depth = (h.meta >> 4) & 255, med = ((h.meta >> 12) & 255) - 1;
                     const f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
                     bool has = true;
@@ -639,14 +657,13 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene
 
 // edge_manager.cpp:112-134 + integratorAD.cpp:126-144 for one primary-edge sample
 template <int N>
-DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t sid, uint64_t total, char *q,
-                          unsigned *qcount, unsigned *stats) {
-    const DScene &S = T.S;
-    const DCamera &c = S.cam;
+DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, const DView &vw, int view, Rng &r, uint64_t sid,
+                          uint64_t total, char *q, unsigned *qcount, unsigned *stats) {
+    const DCamera &c = vw.cam;
     float u = r.next();
-    int idx = sampleReuse(S.pe_cdf, S.npe, u, nullptr);   // edge_manager.cpp:112-134
+    int idx = sampleReuse(vw.pe_cdf, vw.npe, u, nullptr);   // edge_manager.cpp:112-134
     const int stride = 16 + 4 * N;
-    const float *pe = S.pe + (size_t)idx * stride;
+    const float *pe = vw.pe + (size_t)idx * stride;
     double px_ = pe[0] + (double)u * (pe[2] - pe[0]), py_ = pe[1] + (double)u * (pe[3] - pe[1]);
     float pd[N][2];
     bool nz = false;
@@ -689,12 +706,12 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t 
     f3 org{c.c2w[3], c.c2w[7], c.c2w[11]};
     float trans = T.evalTransmittance(org, camdir(px_, py_), false, c.med_id, maxD, r, P.max_bounces - 1);
     if (!(trans > EPS)) return;
-    bool inside = P.cropped ? (ipx >= P.ox && ipx < P.ox + P.cw && ipy >= P.oy && ipy < P.oy + P.ch)
-                            : (ipx >= 0 && ipx < P.width && ipy >= 0 && ipy < P.height);
+    bool inside = c.cropped ? (ipx >= c.ox && ipx < c.ox + c.cw && ipy >= c.oy && ipy < c.oy + c.ch)
+                            : (ipx >= 0 && ipx < c.width && ipy >= 0 && ipy < c.height);
     if (!inside) return;
-    int pix = P.cropped ? (ipx - P.ox) + P.cw * (ipy - P.oy) : ipx + P.width * ipy;
+    int pix = vw.pix0 + (c.cropped ? (ipx - c.ox) + c.cw * (ipy - c.oy) : ipx + c.width * ipy);
     float W[N * 3];
-    float scale = (float)(S.pe_sum / (double)total);
+    float scale = (float)(vw.pe_sum / (double)total);
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         float w = (float)(nx * pd[j][0] + ny * pd[j][1]) * scale;
@@ -703,6 +720,7 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t 
     ChildHdr h;
     h.org[0] = org.x; h.org[1] = org.y; h.org[2] = org.z;
     h.pixel = pix;
+    h.view = view;
     h.meta = CHILD_LI | 8 | (P.max_bounces << 4) | ((c.med_id + 1) << 12);
     h.face = -1;
     for (int s = 0; s < 2; ++s) {
@@ -720,39 +738,43 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, Rng &r, uint64_t 
 }
 
 template <int N>
-__global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp, uint64_t total,
-                                                   char *q, unsigned *qcount, unsigned *stats, unsigned long long *work) {
+__global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, int view, uint64_t s0, uint64_t nsamp,
+                                                   uint64_t total, char *q, unsigned *qcount, unsigned *stats,
+                                                   unsigned long long *work) {
     Tracer<N> T(S);
     const uint64_t li = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t sid = s0 + li;
     Rng r;
-    r.init(P.seed, (1ull << 62) | sid);
-    if (li < nsamp) primaryEdgeSample<N>(T, P, r, sid, total, q, qcount, stats);
+    const DView &vw = S.views[view];
+    r.init(vw.seed, (1ull << 62) | sid);
+    if (li < nsamp) primaryEdgeSample<N>(T, P, vw, view, r, sid, total, q, qcount, stats);
     flushWork(work, 8, T.nbox, T.ntri, r.nlk, T.nseg);
 }
 
+// One view: pixels pix0 .. pix0+npix of the accumulators into its image.
 template <int N>
-__global__ void k_finalize(RenderParams P, const float *acc, const float *acc_pe, float *out, unsigned *stats) {
+__global__ void k_finalize(RenderParams P, int npix, int pix0, int spp, int has_pe, const float *acc, const float *acc_pe,
+                           float *out, unsigned *stats) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.npix) return;
-    const size_t nd = P.npix;
-    if (P.spp > 0) {
-        float inv = 1.0f / P.spp;
-        float v0 = acc[p * 3] * inv, v1 = acc[p * 3 + 1] * inv, v2 = acc[p * 3 + 2] * inv;
+    if (p >= npix) return;
+    const size_t nd = P.npix, g = (size_t)pix0 + p;   // accumulator stride and pixel; output stride is npix
+    if (spp > 0) {
+        float inv = 1.0f / spp;
+        float v0 = acc[g * 3] * inv, v1 = acc[g * 3 + 1] * inv, v2 = acc[g * 3 + 2] * inv;
         if (fminf(v0, fminf(v1, v2)) < 0.f) atomicAdd(stats + STAT_NEG_PIXEL, 1u);
         else { out[p * 3] = v0; out[p * 3 + 1] = v1; out[p * 3 + 2] = v2; }
         for (int j = 0; j < N; ++j) {
-            const float *a = acc + ((j + 1) * nd + p) * 3;
+            const float *a = acc + ((j + 1) * nd + g) * 3;
             float g0 = a[0] * inv, g1 = a[1] * inv, g2 = a[2] * inv;
             if (!isfinite(g0) || !isfinite(g1) || !isfinite(g2)) { atomicAdd(stats + STAT_NONFINITE, 1u); continue; }
-            float *o = out + ((j + 1) * nd + p) * 3;
+            float *o = out + ((size_t)(j + 1) * npix + p) * 3;
             o[0] = g0; o[1] = g1; o[2] = g2;
         }
     }
-    if (P.has_pe)
+    if (has_pe)
         for (int j = 0; j < N; ++j) {
-            const float *a = acc_pe + (j * nd + p) * 3;
-            float *o = out + ((j + 1) * nd + p) * 3;
+            const float *a = acc_pe + (j * nd + g) * 3;
+            float *o = out + ((size_t)(j + 1) * npix + p) * 3;
             o[0] += a[0]; o[1] += a[1]; o[2] += a[2];
         }
 }
@@ -825,14 +847,16 @@ template <int N> static void launchChildren(const DScene &S, const RenderParams 
                                             float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
     hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
 }
-template <int N> static void launchPrimary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total,
-                                           char *q, unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+template <int N> static void launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,
+                                           uint64_t total, char *q, unsigned *qc, unsigned *st, unsigned long long *wk,
+                                           hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, total, q, qc, st, wk);
+    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, view, s0, n, total, q, qc, st, wk);
 }
-template <int N> static void launchFinalize(const RenderParams &P, const float *acc, const float *acc_pe, float *out,
-                                            unsigned *st, hipStream_t sm) {
-    hipLaunchKernelGGL(k_finalize<N>, dim3((P.npix + 255) / 256), dim3(256), 0, sm, P, acc, acc_pe, out, st);
+template <int N> static void launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
+                                            const float *acc_pe, float *out, unsigned *st, hipStream_t sm) {
+    hipLaunchKernelGGL(k_finalize<N>, dim3((npix + 255) / 256), dim3(256), 0, sm, P, npix, pix0, spp, has_pe, acc, acc_pe,
+                       out, st);
 }
 template <int N> static void launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
     hipLaunchKernelGGL(k_grid<N>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, med, pts, n, out);
@@ -859,9 +883,9 @@ hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q
                            unsigned long long *wk, unsigned grid, hipStream_t sm) {
     DT_DISPATCH(launchChildren, S, P, q, qc, acc, acc_pe, wk, grid, sm)
 }
-hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,
-                          unsigned *st, unsigned long long *wk, hipStream_t sm) {
-    DT_DISPATCH(launchPrimary, S, P, s0, n, total, q, qc, st, wk, sm)
+hipError_t launch_primary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total, char *q,
+                          unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
+    DT_DISPATCH(launchPrimary, S, P, view, s0, n, total, q, qc, st, wk, sm)
 }
 template <int N> static void launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,
                                            float *grad, int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm) {
@@ -876,9 +900,9 @@ hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigne
     hipLaunchKernelGGL(k_load_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sm, dst, src, n, maxbits);
     return hipGetLastError();
 }
-hipError_t launch_finalize(const DScene &S, const RenderParams &P, const float *acc, const float *acc_pe, float *out, unsigned *st,
-                           hipStream_t sm) {
-    DT_DISPATCH(launchFinalize, P, acc, acc_pe, out, st, sm)
+hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
+                           const float *acc_pe, float *out, unsigned *st, hipStream_t sm) {
+    DT_DISPATCH(launchFinalize, P, npix, pix0, spp, has_pe, acc, acc_pe, out, st, sm)
 }
 hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
     DT_DISPATCH(launchGrid, S, med, pts, n, out, sm)

@@ -7,11 +7,11 @@
 namespace dt {
 
 // Per-render parameters (RenderOptions, integrator.h:10-25, plus the image layout).
+// Settings shared by all views of a render (per-view ones are in DView).
 struct RenderParams {
-    uint64_t seed;
-    int spp, max_bounces, sppe, sppse, mode;
+    int max_bounces, sppse, mode;
     float ddist;
-    int npix, width, height, ox, oy, cw, ch, cropped, has_pe;
+    int npix;                // pixels of all views (accumulator stride)
     int rec_stride;          // bytes per child record (ChildHdr + 12*nder, 16-byte aligned)
     unsigned qcap;           // child records the queue can hold
 };
@@ -42,10 +42,12 @@ hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, 
 // Drains the qc[0] queued records (qc[2] = 0 on entry: the consumption counter).
 hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc, float *acc_pe,
                            unsigned long long *work, unsigned grid, hipStream_t sm);
-hipError_t launch_primary(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, uint64_t total, char *q, unsigned *qc,
-                          unsigned *st, unsigned long long *work, hipStream_t sm);
-hipError_t launch_finalize(const DScene &S, const RenderParams &P, const float *acc, const float *acc_pe, float *out, unsigned *st,
-                           hipStream_t sm);
+// Primary-edge samples s0 .. s0+n-1 (of `total`) of view `view`.
+hipError_t launch_primary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total, char *q,
+                          unsigned *qc, unsigned *st, unsigned long long *work, hipStream_t sm);
+// One view: accumulator pixels pix0 .. pix0+npix into `out` (that view's image).
+hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
+                           const float *acc_pe, float *out, unsigned *st, hipStream_t sm);
 // Path-replay adjoint over AD samples s0 .. s0+n-1 (interior terms only): adds
 // d loss / d node density of medium pmed's grid into grad, given d loss / d pixel.
 hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,

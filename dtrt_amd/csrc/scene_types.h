@@ -103,12 +103,23 @@ struct DScene {
     const int4 *edges;          // v0, v1, f0, f1 (global ids; f1 = -1 for boundary edges)
     const double *edge_cdf;     // per shape: ne+1 entries starting at ebase + shape index
     const double *shape_edge_cdf;   // nshapes+1
-    int edges_valid, npe;
-    // primary edges (edge_manager.cpp:6-134)
-    const double *pe_cdf;       // npe+1
-    const float *pe;            // [npe][16 + 4*nder]
-    double pe_sum;
+    int edges_valid;
+    int nviews;
+    const struct DView *views;  // the cameras of the render (one for dtrt_render)
+};
+
+// One view of a render: camera and output window, its share of the global AD sample
+// and pixel index spaces, and the primary edges projected for its camera.
+struct DView {
     DCamera cam;
+    uint64_t s0;                // first global AD sample (samples of view v: s0 .. s0 + npix*spp)
+    uint64_t seed;              // render seed of this view
+    int spp, sppe;
+    int npix, pix0;             // pixels of the output window; first global (accumulator) pixel
+    int npe, pad;
+    double pe_sum;
+    const double *pe_cdf;       // npe+1 (edge_manager.cpp:6-134)
+    const float *pe;            // [npe][16 + 4*nder]
 };
 
 // Child primal sub-path records written by the AD-path kernel and the
@@ -123,7 +134,7 @@ struct ChildHdr {
     uint32_t key_lo, key_hi;    // RNG stream of this child
     int face;                   // LINS: global face id of the boundary hit
     float gn[3], sn[3];         // LINS: geometric and shading normals at the hit
-    float pad;
+    int view;                   // view of the sample that pushed the record (its seed keys the stream)
 };
 // followed by float w[nder][3] (weights applied to the child's RGB radiance)
 

@@ -64,6 +64,8 @@ _lib.dtrt_test_scene_info.argtypes = [_P, C.POINTER(C.c_int64)]
 _lib.dtrt_test_set_queue_budget.argtypes = [_P, C.c_uint64]
 _lib.dtrt_test_set_voxel_lanes.argtypes = [_P, C.c_int, C.POINTER(C.c_int), C.c_int]
 _lib.dtrt_scene_set_density.argtypes = [_P, C.c_int, _P, C.c_int]
+_lib.dtrt_render_views.argtypes = [C.c_int, _P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
+                                   C.POINTER(_P), C.c_int]
 _lib.dtrt_render_adjoint.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
 _lib.dtrt_test_shape_edges.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]
 _lib.dtrt_test_trace.argtypes = [_P, _P, C.c_int, _P, _P]
@@ -87,6 +89,21 @@ def kernel_times(scene):
     wk = (C.c_uint64 * 12)()
     _chk(_lib.dtrt_test_kernel_times(scene._h, ms, ln, wk), "kernel_times")
     return np.array(ms[:]), np.array(ln[:]), np.array(wk[:], dtype=np.uint64)
+
+
+def render_views(scene, cameras, options, images, integrator=0):
+    """``dtrt_render_views``: one pass over several views.
+
+    cameras: list of Camera objects (None = the scene's camera); options: list of
+    RenderOptions; images: list of zeroed torch CUDA tensors, or numpy float32 arrays
+    [(nder+1), h, w, 3], all of one kind.
+    """
+    n = len(options)
+    cams = (_P * n)(*[None if c is None else c._h for c in (cameras or [None] * n)])
+    opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
+    on_dev = hasattr(images[0], "data_ptr")
+    ptrs = (_P * n)(*[im.data_ptr() if on_dev else im.ctypes.data for im in images])
+    _chk(_lib.dtrt_render_views(integrator, scene._h, n, cams, opts, ptrs, int(on_dev)), "render_views")
 
 
 def set_density(scene, medium, density):

@@ -417,6 +417,22 @@ def config1(nder=3, res=512, spp=128, grid_res=64, crop=None, seed=7):
                      spp=spp, max_bounces=10, sppe=4096, sppse=10, ddist=0.0, crop=crop, name="cfg1")
 
 
+def orbit_views(nviews=32, seed=5, res_choices=tuple(range(128, 769, 64)), spp_choices=(16, 32, 64), radius=3.0):
+    """Seeded orbit of cameras around the origin for batched rendering: per view
+    (cam_to_world, width, height, spp). Resolutions and spp are drawn with PCG32(seed)."""
+    g = Pcg32(seed, 0)
+    views = []
+    for i in range(nviews):
+        u = g.uniform(4)
+        phi = 2.0 * np.pi * (i + u[0]) / nviews
+        elev = 0.6 * (u[1] - 0.5)
+        eye = (radius * np.cos(elev) * np.sin(phi), radius * np.sin(elev), radius * np.cos(elev) * np.cos(phi))
+        res = res_choices[min(int(u[2] * len(res_choices)), len(res_choices) - 1)]
+        spp = spp_choices[min(int(u[3] * len(spp_choices)), len(spp_choices) - 1)]
+        views.append((look_at(eye, (0, 0, 0), (0, 1, 0)), res, res, spp))
+    return views
+
+
 def config2(nder=8, res=512, spp=256, freq=71, grid_res=128, crop=None):
     """Displaced geodesic sphere (rough dielectric, IOR 1.5) inside a 128^3 fBm medium; 8 deformation lanes."""
     s_v, s_f = geodesic_sphere(freq, 1.0)

@@ -126,6 +126,18 @@ int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *o
 /* Same, with `image` in device memory of the current HIP device (no host copies). */
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);
 
+/* Several views of one scene in one pass. View v renders cameras[v] (NULL entry or
+ * NULL array: the scene's camera) with options[v] into images[v], laid out as for
+ * dtrt_render for that camera's size/crop. Views may differ in camera, resolution,
+ * crop, spp, sppe and seed; max_bounces, mode, sppse and ddist_coeff must be equal.
+ * Each image equals a dtrt_render of that view alone with the same options (same
+ * random streams), up to float summation order. Their AD samples share the
+ * launches, so small views fill the GPU together. Cameras must have been created
+ * with the scene's nder. images_on_device: 1 = device pointers, 0 = host memory.
+ * No counterpart in the reference (one camera per Scene). */
+int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                      const dtrt_render_options *options, float *const *images, int images_on_device);
+
 /* Replace the density grid of heterogeneous medium `medium` (index into the scene's
  * media) with `density` float32[rz][ry][rx] of the same resolution, host memory
  * (on_device = 0) or device memory (1). Negative values are clamped to 0, as the

@@ -1,0 +1,52 @@
+"""Batched multi-view rendering (dtrt_render_views): one pass over views of uneven
+resolution, crop, spp, sppe and seed gives, per view, the image of rendering that view
+alone (same random streams, so equal up to float summation order)."""
+import numpy as np
+import pytest
+
+from dtrt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _camera(gpu, d, eye, res, crop=None, fov=45.0):
+    w, h = res
+    cam = gpu.Camera(w, h, scenes.look_at(eye, (0, 0.5, 0), (0, 1, 0)).astype(np.float32),
+                     scenes.cam_to_ndc(fov).astype(np.float32), d.clip_near, -1, None)
+    if crop:
+        cam.set_rect(*crop)
+    return cam
+
+
+def test_views_equal_single_renders(gpu):
+    import torch
+    d = scenes.config0(nder=1, res=32, spp=16)
+    sc = scenes.instantiate(gpu, d)
+    cams = [None, _camera(gpu, d, (1.5, 1.2, 3.0), (48, 40)), _camera(gpu, d, (-2.0, 2.0, 2.0), (64, 64), crop=(16, 8, 24, 32))]
+    shapes = [(2, 32, 32, 3), (2, 40, 48, 3), (2, 32, 24, 3)]
+    opts = [gpu.RenderOptions(1, 16, 5, 512, 4, True), gpu.RenderOptions(2, 40, 5, 256, 4, True),
+            gpu.RenderOptions(3, 8, 5, -1, 4, True)]
+    outs = [torch.zeros(s, device="cuda") for s in shapes]
+    gpu.render_views(sc, cams, opts, outs)
+    st = gpu.render_stats(sc)
+    assert st["ad_samples"] == 32 * 32 * 16 + 48 * 40 * 40 + 24 * 32 * 8
+    for v in range(3):
+        single = torch.zeros(shapes[v], device="cuda")
+        gpu.render_views(sc, [cams[v]], [opts[v]], [single])
+        a, b = outs[v].cpu().numpy(), single.cpu().numpy()
+        assert np.abs(a[0]).sum() > 0
+        for lane in range(2):
+            scale = np.abs(b[lane]).max() + 1e-12
+            assert np.allclose(a[lane], b[lane], rtol=1e-4, atol=1e-5 * scale), (v, lane)
+    # view 0 is the scene's own camera: dtrt_render gives the same image
+    host = scenes.render(gpu, d, seed=1, scene=sc, sppe=512, sppse=4)
+    assert np.allclose(host, outs[0].cpu().numpy(), rtol=1e-4, atol=1e-5 * np.abs(host).max())
+
+
+def test_views_reject_mixed_settings(gpu):
+    import torch
+    d = scenes.config0(nder=1, res=16, spp=4)
+    sc = scenes.instantiate(gpu, d)
+    outs = [torch.zeros((2, 16, 16, 3), device="cuda") for _ in range(2)]
+    with pytest.raises(RuntimeError, match="share"):
+        gpu.render_views(sc, None, [gpu.RenderOptions(0, 4, 5, 0, 4, True), gpu.RenderOptions(0, 4, 6, 0, 4, True)], outs)
