@@ -158,9 +158,11 @@ HDI u4 philox4x32_10(u4 ctr, uint32_t k0, uint32_t k1) {
 }
 
 // One random stream: key from the render seed; counter = (stream lo, stream hi, block, 0).
+// The 4 words of the current block are separate registers picked by a select chain
+// (an indexed array would be placed in scratch memory).
 struct Rng {
     uint32_t k0, k1, s0, s1, blk;
-    uint32_t buf[4];
+    uint32_t b0, b1, b2, b3;
     int pos;
     uint32_t nlk;   // density-grid lookups made along this stream's path (work counter)
     HDI void init(uint64_t seed, uint64_t stream) {
@@ -175,10 +177,11 @@ struct Rng {
     HDI float next() {
         if (pos == 4) {
             u4 o = philox4x32_10(u4{s0, s1, blk++, 0x9e3779b9u}, k0, k1);
-            buf[0] = o.a; buf[1] = o.b; buf[2] = o.c; buf[3] = o.d;
+            b0 = o.a; b1 = o.b; b2 = o.c; b3 = o.d;
             pos = 0;
         }
-        uint32_t x = buf[pos++];
+        uint32_t x = pos == 0 ? b0 : pos == 1 ? b1 : pos == 2 ? b2 : b3;
+        pos++;
         return (float)(x >> 8) * (1.0f / 16777216.0f);   // [0, 1) with 24 bits
     }
 };

@@ -4,7 +4,7 @@ import math
 import numpy as np
 
 from dtrt_amd import scenes
-from dtrt_amd.scenes import MediumDesc, SceneDesc, ShapeDesc
+from dtrt_amd.scenes import MediumDesc, SceneDesc, ShapeDesc, cam_to_ndc, cube, icosphere, look_at, quad
 
 
 def furnace(nder=1, heterogeneous=False, sigma=0.5, res=16, spp=16, depth=30):
@@ -207,3 +207,52 @@ def shifted(d, h):
             s.vertices = (s.vertices + h * s.velocities[0]).astype(np.float32)
             s.velocities = None
     return e
+
+
+def feature_sweep(nder=4, res=40, spp=16, crop=(8, 8, 24, 24)):
+    """One small scene that exercises the API surface the benchmark scenes leave out:
+    Phong (kd, ks, exponent tangents), rough dielectric (alpha, eta tangents) inside a
+    null cube holding a homogeneous HG medium (sigma_t, albedo, g tangents), an
+    AreaLightEx light (intensity, kappa tangents), a translating and rotating camera,
+    Lins with ddistCoeff > 0, and a cropped window."""
+    g_v, g_f = quad((0, -0.5, 0), 4.0, 1, +1)
+    c_v, c_f = cube(0.0, 1.0)
+    lo, hi = np.array([-0.7, -0.45, -0.7], np.float32), np.array([0.7, 0.8, 0.7], np.float32)
+    c_v = (lo + c_v * (hi - lo)).astype(np.float32)
+    s_v, s_f = icosphere(2, 0.3, (0.0, 0.05, 0.0))
+    l_v, l_f = quad((0.3, 1.6, 0.2), 0.8, 1, -1)
+    lane = lambda j, v: np.array([v if k == j else (0.0 * v) for k in range(nder)], np.float32)
+    phong = ("phong", (0.4, 0.35, 0.3), (0.3, 0.3, 0.3), 20.0, lane(0, np.array([1.0, 0.5, 0.0])),
+             lane(1, np.array([0.0, 1.0, 1.0])), lane(2, 5.0))
+    rough = ("roughdielectric", 0.25, 1.5, 1.0, lane(2, 1.0), lane(3, 1.0))
+    shapes = [ShapeDesc(g_v, g_f, bsdf_id=0), ShapeDesc(c_v, c_f, bsdf_id=1, med_int=0),
+              ShapeDesc(s_v, s_f, bsdf_id=2, med_ext=0), ShapeDesc(l_v, l_f, bsdf_id=3, light_id=0)]
+    med = MediumDesc("homogeneous", phase_id=0, sigma_t=0.8, albedo=(0.7, 0.7, 0.7),
+                     d_scale=lane(1, 1.0), d_albedo=lane(2, np.array([1.0, 0.0, 0.5])))
+    cam_vel = np.zeros((2, nder, 3), np.float32)
+    cam_vel[0, 1] = (1.0, 0.0, 0.0)      # lane 1: camera translation along x
+    cam_vel[1, 3] = (0.0, 1.0, 0.0)      # lane 3: camera rotation about y
+    return SceneDesc(nder=nder, width=res, height=res,
+                     cam_to_world=look_at((0.4, 1.0, 3.0), (0, 0.1, 0), (0, 1, 0)), cam_to_ndc=cam_to_ndc(40.0),
+                     shapes=shapes, bsdfs=[phong, ("null",), rough, ("diffuse", (0.0, 0.0, 0.0))],
+                     lights=[("areaEx", 3, (15.0, 15.0, 15.0), 2.0, lane(3, np.array([1.0, 1.0, 1.0])), lane(0, 1.0))],
+                     phases=[("hg", 0.4, lane(0, 1.0))], media=[med], cam_vel=cam_vel,
+                     spp=spp, max_bounces=6, sppe=256, sppse=4, ddist=2.0, crop=crop, name="features")
+
+
+def rotating_grid(nder=2, res=48, spp=16, grid_res=16, crop=(12, 12, 24, 24)):
+    """The heterogeneous-medium scene with the grid (and its null cube) rotating about y
+    in lane 0 and the density scale as lane 1."""
+    from dtrt_amd.scenes import config1
+    d = config1(nder=nder, res=res, spp=spp, grid_res=grid_res, crop=crop)
+    c_v = d.shapes[0].vertices
+    vel = np.zeros((nder, len(c_v), 3), np.float32)
+    vel[0] = np.cross(np.array([0.0, 1.0, 0.0]), c_v)   # omega x p, centre at the origin
+    d.shapes[0].velocities = vel
+    m = d.media[0]
+    m.trans_vel = np.zeros((nder, 3), np.float32)
+    m.rot_vel = np.zeros((nder, 3), np.float32)
+    m.rot_vel[0] = (0.0, 1.0, 0.0)
+    m.d_scale = np.array([0.0, 1.0] + [0.0] * (nder - 2), np.float32)
+    d.sppe, d.sppse, d.max_bounces = 256, 4, 6
+    return d

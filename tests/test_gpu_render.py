@@ -164,3 +164,18 @@ def test_value_lane_independent_of_lane_count(gpu):
             ref = img
         else:
             assert np.allclose(img, ref, rtol=1e-5, atol=1e-6), n
+
+
+@pytest.mark.parametrize("mode", [-1, EMISSION | SURF_MAIN | VOL_MAIN, SURF_BOUNDARY | VOL_BOUNDARY1 | VOL_BOUNDARY2])
+def test_feature_sweep_parity(gpu, oracle, mode):
+    """Phong, rough dielectric, AreaLightEx, homogeneous HG medium, camera motion and
+    Lins with ddistCoeff > 0, each with its own tangents."""
+    d = F.feature_sweep()
+    z = _parity(gpu, oracle, d, mode=mode, seeds=16, block=4)
+    assert_3sigma(z, what="feature sweep mode %d" % mode)
+
+
+def test_rotating_grid_parity(gpu, oracle):
+    d = F.rotating_grid()
+    z = _parity(gpu, oracle, d, seeds=16, block=4)
+    assert_3sigma(z, what="rotating grid")
