@@ -166,10 +166,32 @@ def views_leg(nviews=32):
         G.render_views(sc, [cams[i]], [opts[i]], [outs[i]])
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t
+    # small views: one captured HIP graph per optimisation step against the eager batch
+    small = scenes.orbit_views(8, res_choices=(128,), spp_choices=(16,))
+    scams = [G.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
+             for c2w, w, h, _ in small]
+    sopts = [G.RenderOptions(200 + i, spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
+             for i, (_, _, _, spp) in enumerate(small)]
+    souts = [torch.zeros((d.nder + 1, h, w, 3), device="cuda") for _, w, h, _ in small]
+    G.capture_views(sc, scams, sopts, souts)
+    reps = 10
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(reps):
+        G.render_views(sc, scams, sopts, souts)
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    fallbacks = sum(G.replay_views(sc, [300 + 8 * k + i for i in range(len(small))]) for k in range(reps))
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t) / reps
     return {"workload": "%d orbit views of the heterogeneous-medium scene, res %d^2..%d^2, spp 16/32/64 (PCG32 seed 5), "
                         "nder 3, depth 10, sppse 10, sppe 4096" % (nviews, min(v[1] for v in views), max(v[1] for v in views)),
             "samples": samples, "batched_s": t_batch, "one_call_per_view_s": t_seq,
-            "batched_samples_per_s": samples / t_batch, "batching_speedup": t_seq / t_batch}
+            "batched_samples_per_s": samples / t_batch, "batching_speedup": t_seq / t_batch,
+            "small_batch": {"views": len(small), "res": 128, "spp": 16, "eager_ms": t_eager * 1e3,
+                            "graph_replay_ms": t_graph * 1e3, "graph_speedup": t_eager / t_graph,
+                            "replays_that_fell_back": int(fallbacks)}}
 
 
 def main():

@@ -512,6 +512,7 @@ struct Scene : Obj {
     Scene() : Obj(T_SCENE) {}
     ~Scene() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        graph.reset();
         for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
         for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf})
             if (b->p) (void)hipFree(b->p);
@@ -530,6 +531,24 @@ struct Scene : Obj {
     DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf, ovf[2], viewbuf;
     std::vector<DMedium> hmed;   // host copy of the device media records
     struct PECache { int npe = 0; double pe_sum = 0; const double *cdf = nullptr; const float *pe = nullptr; };
+    std::vector<uint64_t> last_chunks;   // AD chunk sizes of the last eager renderViews
+    uint64_t last_replayed = 0;
+    // A captured launch sequence of renderViews (fixed chunk schedule), replayed with new seeds.
+    struct Graph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        std::vector<Camera> cams;
+        std::vector<dtrt_render_options> opts;
+        std::vector<float *> images;
+        std::vector<DView> hv;
+        std::vector<void *> bufs;   // device buffers the graph was captured with
+        void reset() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            x = nullptr;
+            g = nullptr;
+        }
+    } graph;
     std::map<uint64_t, PECache> pecache;   // primary edges per camera configuration (Camera::uid)
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;
@@ -566,6 +585,11 @@ struct Scene : Obj {
     }
     bool renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs, bool on_device);
     bool primaryEdges(const Camera &c, PECache &out);
+    bool captureViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *images);
+    int replayViews(const uint64_t *seeds);
+    std::vector<void *> graphBuffers() const {
+        return {viewbuf.p, acc.p, acc_pe.p, queue.p, qcount.p, stats.p, work.p, ovf[0].p, ovf[1].p};
+    }
     bool setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
                     std::vector<uint64_t> &pe_total);
     bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
@@ -933,8 +957,10 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
         return true;
     };
     uint64_t chunk = std::min<uint64_t>(FIRST_CHUNK, max_chunk);
+    last_chunks.clear();
     for (uint64_t s0 = 0; s0 < total;) {
         const uint64_t n = std::min(chunk, total - s0);
+        last_chunks.push_back(n);
         uint64_t delivered = 0;
         unsigned h[2];
         HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
@@ -985,12 +1011,138 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     HIPCHK(hipStreamSynchronize(stream));
     for (int i = 0; i < 6; ++i) last_stats[i] = hst[i];
     last_stats[6] = (int64_t)replayed;
+    last_replayed = replayed;
     last_stats[7] = (int64_t)total;
     last_stats[8] = (int64_t)children;
     last_stats[9] = (int64_t)pe_all;
     for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
     if (hst[STAT_QUEUE_OVERFLOW]) return fail("render: primary-edge child queue overflow");
     return true;
+}
+
+// Render the views eagerly once (which fixes the adaptive chunk schedule), then
+// capture the same launch sequence as a hipGraph: clears as fill kernels, per chunk
+// AD paths + children, per view primary edges + finalize into `images` (device).
+// An overflowing sample cannot be replayed inside a graph, so every chunk's overflow
+// count is summed into qcount[3]; replayViews checks it and falls back to the eager
+// path, so a replay is always exact.
+bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *images) {
+    graph.reset();
+    if (!renderViews(nv, cams, opts, images, true)) return false;
+    if (last_replayed) return fail("capture: the schedule overflowed the child queue; render in smaller batches");
+    RenderParams P{};
+    const dtrt_render_options &o0 = opts[0];
+    P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
+    P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
+    P.rec_stride = (int)((sizeof(ChildHdr) + 12 * nder + 15) / 16 * 16);
+    std::vector<DView> hv;
+    std::vector<uint64_t> pe_total;
+    if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
+    HIPCHK(hipStreamSynchronize(stream));
+    int64_t npix_all = 0;
+    for (int v = 0; v < nv; ++v) npix_all += hv[v].npix;
+    P.npix = (int)npix_all;
+    const size_t npix = P.npix, img = (size_t)(nder + 1) * npix * 3;
+    const uint64_t cap = std::max<uint64_t>(128, std::min<uint64_t>(queue_budget / P.rec_stride, 0xFFFFFFF0ull));
+    const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
+    P.qcap = (unsigned)cap;
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned childGrid = (unsigned)std::max(1, ncu) * 32;
+    unsigned *qc = static_cast<unsigned *>(qcount.p);
+    unsigned *st = static_cast<unsigned *>(stats.p);
+    unsigned long long *wk = static_cast<unsigned long long *>(work.p);
+    float *A = static_cast<float *>(acc.p), *APE = static_cast<float *>(acc_pe.p);
+    char *Q = static_cast<char *>(queue.p);
+    uint4 *OV = static_cast<uint4 *>(ovf[0].p);
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    auto issue = [&]() -> hipError_t {
+        hipError_t e;
+#define GCHK(x) if ((e = (x)) != hipSuccess) return e
+        GCHK(launch_fill32(A, img, 0u, stream));
+        GCHK(launch_fill32(APE, (size_t)nder * npix * 3, 0u, stream));
+        GCHK(launch_fill32(st, 64, 0u, stream));
+        GCHK(launch_fill32(wk, 32, 0u, stream));
+        for (int v = 0; v < nv; ++v) GCHK(launch_fill32(images[v], (size_t)(nder + 1) * hv[v].npix * 3, 0u, stream));
+        uint64_t s0 = 0;
+        for (uint64_t n : last_chunks) {
+            GCHK(launch_fill32(qc, 3, 0u, stream));
+            GCHK(launch_ad_paths(S, P, s0, n, nullptr, A, Q, qc, st, OV, qc + 1, wk, stream));
+            GCHK(launch_add_u32(qc + 3, qc + 1, stream));
+            GCHK(launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream));
+            s0 += n;
+        }
+        for (int v = 0; v < nv; ++v)
+            for (uint64_t p0 = 0; p0 < pe_total[v]; p0 += pe_chunk) {
+                GCHK(launch_fill32(qc, 3, 0u, stream));
+                GCHK(launch_primary(S, P, v, p0, std::min(pe_chunk, pe_total[v] - p0), pe_total[v], Q, qc, st, wk, stream));
+                GCHK(launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream));
+            }
+        for (int v = 0; v < nv; ++v)
+            GCHK(launch_finalize(S, P, hv[v].npix, hv[v].pix0, hv[v].spp, pe_total[v] > 0, A, APE, images[v], st, stream));
+#undef GCHK
+        return hipSuccess;
+    };
+    hipError_t e = issue();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(std::string("capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    graph.g = g;
+    HIPCHK(hipGraphInstantiate(&graph.x, g, nullptr, nullptr, 0));
+    graph.cams.clear();
+    for (int v = 0; v < nv; ++v) graph.cams.push_back(*cams[v]);
+    graph.opts.assign(opts, opts + nv);
+    graph.images.assign(images, images + nv);
+    graph.hv = hv;
+    graph.bufs = graphBuffers();
+    return true;
+}
+
+// Replay the captured views with new seeds (one per view). Returns 0 after a graph
+// replay, 1 if the fixed schedule overflowed and the views were rendered eagerly
+// instead, -1 on error.
+int Scene::replayViews(const uint64_t *seeds) {
+    if (!graph.x) { g_err = "replay: no captured views"; return -1; }
+    if (graphBuffers() != graph.bufs) { g_err = "replay: an eager render reallocated the buffers; capture again"; return -1; }
+    const int nv = (int)graph.hv.size();
+    for (int v = 0; v < nv; ++v) {
+        graph.hv[v].seed = seeds[v];
+        graph.opts[v].seed = seeds[v];
+    }
+    unsigned *qc = static_cast<unsigned *>(qcount.p);
+    if (hipMemcpyAsync(viewbuf.p, graph.hv.data(), (size_t)nv * sizeof(DView), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemsetAsync(qc + 3, 0, 4, stream) != hipSuccess || hipEventRecord(ev[0], stream) != hipSuccess ||
+        hipGraphLaunch(graph.x, stream) != hipSuccess || hipEventRecord(ev[1], stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        g_err = "replay: graph launch failed";
+        return -1;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    unsigned flag = 0;
+    unsigned hst[STAT_COUNT];
+    if (hipMemcpy(&flag, qc + 3, 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hst, stats.p, sizeof(hst), hipMemcpyDeviceToHost) != hipSuccess) {
+        g_err = "replay: readback failed";
+        return -1;
+    }
+    if (flag) {   // some sample overflowed the queue: render these seeds eagerly (exact)
+        std::vector<const Camera *> cp;
+        for (auto &c : graph.cams) cp.push_back(&c);
+        if (!renderViews(nv, cp.data(), graph.opts.data(), graph.images.data(), true)) return -1;
+        return 1;
+    }
+    std::memset(kernel_ms, 0, sizeof(kernel_ms));
+    std::memset(kernel_launches, 0, sizeof(kernel_launches));
+    kernel_ms[0] = ms;
+    kernel_launches[0] = 1;
+    for (int i = 0; i < 6; ++i) last_stats[i] = hst[i];
+    last_stats[6] = 0;
+    return 0;
 }
 
 // Replace the density grid of medium `med` (same resolution): negative values are
@@ -1406,6 +1558,30 @@ int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_objec
         g_err = e.what();
         return -1;
     }
+}
+
+int dtrt_scene_capture_views(dtrt_object *scene, int nviews, dtrt_object *const *cameras, const dtrt_render_options *options,
+                             float *const *images_device) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || nviews <= 0 || !options || !images_device) { if (s) g_err = "capture_views: bad arguments"; return -1; }
+    std::vector<const Camera *> cams(nviews);
+    for (int v = 0; v < nviews; ++v) {
+        if (!images_device[v]) { g_err = "capture_views: null image"; return -1; }
+        cams[v] = (cameras && cameras[v]) ? as<Camera>(cameras[v], T_CAMERA) : &s->cam;
+        if (!cams[v]) return -1;
+    }
+    try {
+        return s->captureViews(nviews, cams.data(), options, images_device) ? 0 : -1;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !seeds) { if (s) g_err = "replay_views: bad arguments"; return -1; }
+    return s->replayViews(seeds);
 }
 
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {

@@ -12,6 +12,8 @@
 #include "tracer.h"
 #include "launch.h"
 
+#include <algorithm>
+
 namespace dt {
 
 __constant__ FrameRot c_frame_rot;
@@ -526,6 +528,14 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     flushWork(work, 0, T.nbox, T.ntri, nlk, T.nseg);
 }
 
+// Graph-safe clears and counters (captured graphs use kernels, not memset nodes).
+__global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_add_u32(unsigned *dst, const unsigned *src) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst += *src;
+}
+
 // dst = max(src, 0) (NaN -> 0), and the maximum into *maxbits (non-negative floats
 // order like their bit patterns). dst may alias src.
 __global__ void k_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits) {
@@ -895,6 +905,15 @@ template <int N> static void launchAdjoint(const DScene &S, const RenderParams &
 hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,
                           int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm) {
     DT_DISPATCH(launchAdjoint, S, P, s0, n, dimg, grad, pmed, st, wk, sm)
+}
+hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm) {
+    const unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill32, dim3(std::max(nb, 1u)), dim3(256), 0, sm, static_cast<uint32_t *>(p), n, v);
+    return hipGetLastError();
+}
+hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm) {
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, sm, dst, src);
+    return hipGetLastError();
 }
 hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm) {
     hipLaunchKernelGGL(k_load_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sm, dst, src, n, maxbits);

@@ -52,6 +52,9 @@ hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int
 // d loss / d node density of medium pmed's grid into grad, given d loss / d pixel.
 hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,
                           int pmed, unsigned *st, unsigned long long *work, hipStream_t sm);
+// p[0..n) = v (32-bit words); *dst += *src. Kernels, so that captured graphs hold no memset nodes.
+hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm);
+hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm);
 // dst = max(src, 0) over n floats; atomicMax of the result's bits into *maxbits.
 hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm);
 hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);

@@ -64,6 +64,9 @@ _lib.dtrt_test_scene_info.argtypes = [_P, C.POINTER(C.c_int64)]
 _lib.dtrt_test_set_queue_budget.argtypes = [_P, C.c_uint64]
 _lib.dtrt_test_set_voxel_lanes.argtypes = [_P, C.c_int, C.POINTER(C.c_int), C.c_int]
 _lib.dtrt_scene_set_density.argtypes = [_P, C.c_int, _P, C.c_int]
+_lib.dtrt_scene_capture_views.argtypes = [_P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
+                                          C.POINTER(_P)]
+_lib.dtrt_scene_replay_views.argtypes = [_P, C.POINTER(C.c_uint64)]
 _lib.dtrt_render_views.argtypes = [C.c_int, _P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
                                    C.POINTER(_P), C.c_int]
 _lib.dtrt_render_adjoint.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
@@ -104,6 +107,24 @@ def render_views(scene, cameras, options, images, integrator=0):
     on_dev = hasattr(images[0], "data_ptr")
     ptrs = (_P * n)(*[im.data_ptr() if on_dev else im.ctypes.data for im in images])
     _chk(_lib.dtrt_render_views(integrator, scene._h, n, cams, opts, ptrs, int(on_dev)), "render_views")
+
+
+def capture_views(scene, cameras, options, images):
+    """``dtrt_scene_capture_views``: render once, then capture the launch sequence as a HIP graph."""
+    n = len(options)
+    cams = (_P * n)(*[None if c is None else c._h for c in (cameras or [None] * n)])
+    opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
+    ptrs = (_P * n)(*[im.data_ptr() for im in images])
+    _chk(_lib.dtrt_scene_capture_views(scene._h, n, cams, opts, ptrs), "capture_views")
+
+
+def replay_views(scene, seeds):
+    """``dtrt_scene_replay_views``: 0 = graph replay, 1 = eager fallback (overflow)."""
+    s = (C.c_uint64 * len(seeds))(*[int(x) for x in seeds])
+    rc = _lib.dtrt_scene_replay_views(scene._h, s)
+    if rc < 0:
+        raise RuntimeError("replay_views: " + _lib.dtrt_last_error().decode())
+    return rc
 
 
 def set_density(scene, medium, density):

@@ -138,6 +138,18 @@ int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_opt
 int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
                       const dtrt_render_options *options, float *const *images, int images_on_device);
 
+/* Render views once with dtrt_render_views (device images), then capture that launch
+ * sequence (its chunk schedule fixed) as a HIP graph owned by the scene, replacing any
+ * earlier one. dtrt_scene_replay_views re-renders the same views into the same images
+ * with new seeds (one per view) as a single graph launch: no host round trips between
+ * kernels. A replay returns 0; if a sample of the fixed schedule overflowed the child
+ * queue, the views are rendered eagerly instead (exact) and it returns 1; -1 on error
+ * (also when a later render reallocated the scene's buffers: capture again). Images
+ * are overwritten (pixels the reference drops read 0). No counterpart in the reference. */
+int dtrt_scene_capture_views(dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                             const dtrt_render_options *options, float *const *images_device);
+int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds);
+
 /* Replace the density grid of heterogeneous medium `medium` (index into the scene's
  * media) with `density` float32[rz][ry][rx] of the same resolution, host memory
  * (on_device = 0) or device memory (1). Negative values are clamped to 0, as the

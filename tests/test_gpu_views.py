@@ -50,3 +50,38 @@ def test_views_reject_mixed_settings(gpu):
     outs = [torch.zeros((2, 16, 16, 3), device="cuda") for _ in range(2)]
     with pytest.raises(RuntimeError, match="share"):
         gpu.render_views(sc, None, [gpu.RenderOptions(0, 4, 5, 0, 4, True), gpu.RenderOptions(0, 4, 6, 0, 4, True)], outs)
+
+
+def test_graph_replay_equals_eager(gpu):
+    """Replays of the captured views with new seeds, with eager renders in between,
+    each equal to an eager render with those seeds."""
+    import torch
+    d = scenes.config0(nder=1, res=32, spp=16)
+    sc = scenes.instantiate(gpu, d)
+    cams = [None, _camera(gpu, d, (1.5, 1.2, 3.0), (48, 40))]
+    shapes = [(2, 32, 32, 3), (2, 40, 48, 3)]
+    opts = [gpu.RenderOptions(1, 16, 5, 512, 4, True), gpu.RenderOptions(2, 24, 5, 256, 4, True)]
+    imgs = [torch.zeros(sh, device="cuda") for sh in shapes]
+    gpu.capture_views(sc, cams, opts, imgs)
+    for k, seeds in enumerate([(11, 12), (21, 22), (11, 12)]):
+        assert gpu.replay_views(sc, seeds) == 0
+        got = [im.cpu().numpy() for im in imgs]
+        ref = [torch.zeros(sh, device="cuda") for sh in shapes]
+        eo = [gpu.RenderOptions(sd, o.spp, o.max_bounces, o.sppe, o.sppse, True) for sd, o in zip(seeds, opts)]
+        gpu.render_views(sc, cams, eo, ref)
+        for v in range(2):
+            r = ref[v].cpu().numpy()
+            assert np.abs(r[0]).sum() > 0
+            for lane in range(2):
+                scale = np.abs(r[lane]).max() + 1e-12
+                assert np.allclose(got[v][lane], r[lane], rtol=1e-4, atol=1e-5 * scale), (k, v, lane)
+
+
+def test_graph_capture_refuses_overflowing_schedule(gpu):
+    import torch
+    d = scenes.config0(nder=1, res=32, spp=16)
+    sc = scenes.instantiate(gpu, d)
+    gpu.set_queue_budget(sc, 64 * 128)
+    with pytest.raises(RuntimeError, match="overflowed"):
+        gpu.capture_views(sc, None, [gpu.RenderOptions(1, 16, 5, 0, 4, True)], [torch.zeros((2, 32, 32, 3), device="cuda")])
+    gpu.set_queue_budget(sc, 0)
