@@ -273,6 +273,16 @@ def bind(lib_path, prefix, name="dtrt"):
             if rc != 0:
                 raise DtrtError("render: " + fns["last_error"]().decode())
 
+    class VolPathTracer(Integrator):
+        """volpath.cpp:106-141: writes the primal image float32[H][W][3]."""
+        kind = 1
+
+        def render(self, scene, options, out):
+            st = options._struct()
+            rc = fns["render"](self.kind, scene._h, C.byref(st), _addr(out))
+            if rc != 0:
+                raise DtrtError("render: " + fns["last_error"]().decode())
+
     class _Module:
         pass
 
@@ -283,7 +293,7 @@ def bind(lib_path, prefix, name="dtrt"):
                      Emitter=Emitter, AreaLight=AreaLight, AreaLightEx=AreaLightEx, Medium=Medium,
                      Homogeneous=Homogeneous, Heterogeneous=Heterogeneous, HeterogeneousGrid=HeterogeneousGrid,
                      Scene=Scene, RenderOptions=RenderOptions, Integrator=Integrator,
-                     VolPathTracerAD=VolPathTracerAD).items():
+                     VolPathTracerAD=VolPathTracerAD, VolPathTracer=VolPathTracer).items():
         setattr(m, k, v)
     m.lib = lib
     m.fns = fns

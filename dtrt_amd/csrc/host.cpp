@@ -593,6 +593,7 @@ struct Scene : Obj {
     bool setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
                     std::vector<uint64_t> &pe_total);
     bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
+    bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device);
     bool setDensity(int med, const float *density, bool on_device);
     bool setVoxelLanes(int med, const int *vox, int n);
     bool uploadMedia() {
@@ -1225,6 +1226,56 @@ bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *di
     return true;
 }
 
+// VolPathTracer::render (volpath.cpp:106-141): float32[H][W][3] (crop size if
+// cropped), each pixel the mean of spp jittered primal samples; every pixel is written.
+bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device) {
+    if (o.spp <= 0 || o.max_bounces < 0 || o.max_bounces > 200) return fail("render: bad options");
+    RenderParams P{};
+    P.max_bounces = o.max_bounces;
+    dtrt_render_options ov = o;
+    ov.sppe = 0;   // no primary-edge pass
+    const Camera *c = &cam;
+    std::vector<DView> hv;
+    std::vector<uint64_t> pe_total;
+    if (!setupViews(1, &c, &ov, hv, pe_total)) return false;
+    P.npix = hv[0].npix;
+    const size_t n3 = (size_t)P.npix * 3;
+    if (!ensure(acc, n3 * 4) || !ensure(work, 16 * 8)) return false;
+    float *dout = out;
+    if (!out_is_device) {
+        if (!ensure(outbuf, n3 * 4)) return false;
+        dout = static_cast<float *>(outbuf.p);
+    }
+    HIPCHK(hipMemsetAsync(acc.p, 0, n3 * 4, stream));
+    HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
+    if (ev.empty()) {
+        ev.resize(2);
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    unsigned long long *wk = static_cast<unsigned long long *>(work.p);
+    float *A = static_cast<float *>(acc.p);
+    const uint64_t total = (uint64_t)P.npix * o.spp, chunk = uint64_t(1) << 26;
+    std::memset(kernel_ms, 0, sizeof(kernel_ms));
+    std::memset(kernel_launches, 0, sizeof(kernel_launches));
+    HIPCHK(hipEventRecord(ev[0], stream));
+    for (uint64_t s0 = 0; s0 < total; s0 += chunk)
+        HIPCHK(launch_primal_paths(S, P, s0, std::min(chunk, total - s0), A, wk, stream));
+    HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(launch_scale(A, dout, n3, 1.0f / (float)o.spp, stream));
+    if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, n3 * 4, hipMemcpyDeviceToHost, stream));
+    unsigned long long hw[12];
+    HIPCHK(hipMemcpyAsync(hw, wk, sizeof(hw), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    kernel_ms[0] = ms;
+    kernel_launches[0] = (int64_t)((total + chunk - 1) / chunk);
+    for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
+    for (int i = 0; i < 10; ++i) last_stats[i] = 0;
+    last_stats[7] = (int64_t)total;
+    return true;
+}
+
 template <class T> T *as(dtrt_object *o, int tag) {
     Obj *b = reinterpret_cast<Obj *>(o);
     if (!b || b->tag != tag) { g_err = "wrong or null object handle"; return nullptr; }
@@ -1585,10 +1636,14 @@ int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {
 }
 
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
-    if (integrator != 0) { g_err = "render: only VolPathTracerAD (integrator 0) is provided"; return -2; }
+    if (integrator != 0 && integrator != 1) {
+        g_err = "render: integrators are VolPathTracerAD (0) and VolPathTracer (1)";
+        return -2;
+    }
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s || !o || !img) { if (s) g_err = "render: null argument"; return -1; }
     try {
+        if (integrator == 1) return s->renderPrimal(*o, img, dev) ? 0 : -1;
         return s->render(*o, img, dev) ? 0 : -1;
     } catch (const std::exception &e) {
         g_err = e.what();

@@ -114,6 +114,20 @@ template <int N> DI void cameraRayAD(const DCamera &c, float x, float y, Dv<N> &
     }
 }
 
+// camera.cpp:40-50 (primal, jittered pixel): origin and direction through the
+// camera-to-world matrix, as VolPathTracer's camera rays are built.
+DI void cameraRay(const DCamera &c, float x, float y, f3 &o, f3 &d) {
+    x /= c.width;
+    y /= c.height;
+    f3 n3{(x - 0.5f) * 2.0f, (y - 0.5f) * (-2.0f) / c.aspect, 1.0f};
+    f3 dc = normalize(f3{c.ndc2c[0] * n3.x + c.ndc2c[1] * n3.y + c.ndc2c[2] * n3.z,
+                         c.ndc2c[3] * n3.x + c.ndc2c[4] * n3.y + c.ndc2c[5] * n3.z,
+                         c.ndc2c[6] * n3.x + c.ndc2c[7] * n3.y + c.ndc2c[8] * n3.z});
+    o = f3{c.c2w[3], c.c2w[7], c.c2w[11]};
+    d = f3{c.c2w[0] * dc.x + c.c2w[1] * dc.y + c.c2w[2] * dc.z, c.c2w[4] * dc.x + c.c2w[5] * dc.y + c.c2w[6] * dc.z,
+           c.c2w[8] * dc.x + c.c2w[9] * dc.y + c.c2w[10] * dc.z};
+}
+
 // shape.cpp:302-335
 DI int isSilhouette(const DScene &S, int4 e, f3 p, bool hasNormals) {
     if (e.z < 0 || e.w < 0) return 2;
@@ -486,6 +500,59 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
         }
     }
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
+// VolPathTracer (volpath.cpp:106-141): one thread per primal sample s0 .. s0+n-1 of
+// view 0 (sample k of pixel p is p*spp + k), a jittered camera ray and the primal
+// estimator (Tracer::liStep: volpath.cpp:10-103 is volpathAD.cpp:50-142 with the
+// first ray leaving the camera). Radiance sums go to acc[pixel][3].
+__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DScene S, RenderParams P, uint64_t s0,
+                                                                              uint64_t nsamp, float *acc,
+                                                                              unsigned long long *work) {
+    const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    const bool active = li < nsamp;
+    const DView &vw = S.views[0];
+    const DCamera &cam = vw.cam;
+    const uint64_t ls = s0 + li;
+    const int pixel = (int)(ls / (uint64_t)vw.spp);
+    Tracer<1> T(S);
+    Rng r;
+    r.init(vw.seed, ls);
+    f3 L{0, 0, 0};
+    if (active) {
+        const int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
+        const int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
+        const float jx = r.next(), jy = r.next();
+        f3 ro, rd;
+        cameraRay(cam, ix + jx, iy + jy, ro, rd);
+        Hit h;
+        T.rayIntersect(ro, rd, false, h);
+        LiPath p;
+        T.liStartAt(p, ro, rd, P.max_bounces, cam.med_id, true, h);
+        while (T.liStep(p, r)) {}
+        L = p.ret;
+    }
+    const int gp0 = __shfl(pixel, 0);
+    if (__all(!active || pixel == gp0)) {   // the wave covers one pixel
+        float v[3] = {L.x, L.y, L.z};
+        for (int c = 0; c < 3; ++c)
+            for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off);
+        if ((threadIdx.x & 63) == 0 && active)
+            for (int c = 0; c < 3; ++c)
+                if (v[c] != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + c, v[c]);
+    } else if (active) {
+        if (L.x != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3, L.x);
+        if (L.y != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + 1, L.y);
+        if (L.z != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + 2, L.z);
+    }
+    flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
+// out[i] = acc[i] * scale over n floats (the pixel mean of VolPathTracer)
+__global__ void k_scale(const float *acc, float *out, uint64_t n, float scale) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = acc[i] * scale;
 }
 
 // Path-replay adjoint of the density derivative of the interior terms: for each AD
@@ -928,6 +995,19 @@ hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float 
 }
 hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm) {
     hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, face, t);
+    return hipGetLastError();
+}
+hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc,
+                              unsigned long long *work, hipStream_t sm) {
+    const uint64_t nb = (n + BLOCK - 1) / BLOCK;
+    if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_primal_paths, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+    return hipGetLastError();
+}
+hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm) {
+    const uint64_t nb = (n + BLOCK - 1) / BLOCK;
+    if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)nb), dim3(BLOCK), 0, sm, acc, out, n, scale);
     return hipGetLastError();
 }
 hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm) {

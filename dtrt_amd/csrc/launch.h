@@ -57,6 +57,11 @@ hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm);
 hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm);
 // dst = max(src, 0) over n floats; atomicMax of the result's bits into *maxbits.
 hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm);
+// VolPathTracer: primal samples s0 .. s0+n-1 of view 0, radiance sums into acc[pixel][3].
+hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc,
+                              unsigned long long *work, hipStream_t sm);
+// out[i] = acc[i] * scale, i < n.
+hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm);
 hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);
 hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm);
 hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out, hipStream_t sm);

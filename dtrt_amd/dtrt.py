@@ -40,6 +40,7 @@ Scene = _m.Scene
 RenderOptions = _m.RenderOptions
 Integrator = _m.Integrator
 VolPathTracerAD = _m.VolPathTracerAD
+VolPathTracer = _m.VolPathTracer
 angleEps = _m.angleEps
 abi_version = _m.abi_version
 set_nder = _m.set_nder

@@ -375,6 +375,16 @@ def render(mod, d: SceneDesc, seed=0, spp=None, mode=-1, sppe=None, sppse=None, 
     return out
 
 
+def render_primal(mod, d: SceneDesc, seed=0, spp=None, max_bounces=None, scene=None):
+    """VolPathTracer render of ``d``; returns the primal image float32[H, W, 3]."""
+    scene = scene if scene is not None else instantiate(mod, d)
+    opts = mod.RenderOptions(seed, d.spp if spp is None else spp, d.max_bounces if max_bounces is None else max_bounces,
+                             0, 0, True)
+    out = np.zeros(d.out_shape()[1:], dtype=np.float32)
+    mod.VolPathTracer().render(scene, opts, out)
+    return out
+
+
 # ------------------------------------------------------------- configs
 def config0(nder=1, res=256, spp=64, crop=None):
     """Ground quad + icosphere (subdiv 3, r 0.5) + area-light quad; no media; sphere x-translation."""

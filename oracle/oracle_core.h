@@ -1318,11 +1318,15 @@ template <int N> struct Tracer {
     static double mis(double a, double b) { a *= a; b *= b; return a / (a + b); }   // volpathAD.cpp:43-47
 
     // ===================================================== integrator
-    // volpathAD.cpp:50-142 — primal volumetric path tracer
-    Spec Li(Pcg &s, V3 ro, V3 rd, int max_depth, int med, bool incEmission) const {
+    // volpathAD.cpp:50-142 — primal volumetric path tracer. With `volpath` set it is
+    // VolPathTracer::radiance (volpath.cpp:10-103) instead: the same loop, but the first
+    // ray leaves the camera (tnear 0) and paths end on exactly-zero throughput, phase
+    // value or BSDF weight rather than below Epsilon.
+    Spec Li(Pcg &s, V3 ro, V3 rd, int max_depth, int med, bool incEmission, bool volpath = false) const {
         Its its;
         Spec ret, thr = Spec::ones();
-        rayIntersect(ro, rd, true, its);
+        const double zeps = volpath ? 1e-12 : EPS;   // Eigen isZero() vs isZero(Epsilon)
+        rayIntersect(ro, rd, !volpath, its);
         for (int depth = 0; depth <= max_depth; ++depth) {
             int max_inter = max_depth - depth - 1;
             double rnd1[2] = {s.next1D(), s.next1D()};
@@ -1336,21 +1340,21 @@ template <int N> struct Tracer {
             }
             if (inside) {
                 if (depth >= max_depth) break;
-                if (thr.isZero(EPS)) break;
+                if (thr.isZero(zeps)) break;
                 const OPhase &ph = *sc.phases[sc.media[med]->phase_id];
                 V3 wo;
                 double pdf_nee;
                 Spec value = sampleAttenuatedEmitterDirectPt(ro, rnd2, s, med, max_inter, wo, pdf_nee);
-                if (!value.isZero(EPS)) {
+                if (!value.isZero(zeps)) {
                     double pv = phaseEval(ph, -rd, wo);
-                    if (pv > EPS) ret += thr * value * (mis(pdf_nee, phaseEval(ph, -rd, wo)) * pv);
+                    if (volpath ? pv != 0.0 : pv > EPS) ret += thr * value * (mis(pdf_nee, phaseEval(ph, -rd, wo)) * pv);
                 }
                 double pv = phaseSample(ph, -rd, rnd3[0], rnd3[1], wo);
-                if (pv < EPS) break;
+                if (volpath ? pv == 0.0 : pv < EPS) break;
                 thr *= pv;
                 rd = wo;
                 Spec att = rayIntersectAndLookForEmitter(ro, rd, false, s, med, max_inter, its, pdf_nee);
-                if (!att.isZero(EPS)) {
+                if (!att.isZero(zeps)) {
                     double ppdf = phaseEval(ph, -rd, wo);
                     ret += thr * att * mis(ppdf, pdf_nee);
                 }
@@ -1365,7 +1369,7 @@ template <int N> struct Tracer {
                 double pdf_nee;
                 if (!b.isNull()) {
                     Spec value = sampleAttenuatedEmitterDirectIts(its, rnd2, s, med, max_inter, wo, pdf_nee, false);
-                    if (!value.isZero(EPS)) {
+                    if (!value.isZero(zeps)) {
                         Spec bv = bsdfEval(b, its, wo);
                         double bp = bsdfPdf(b, its, wo);
                         ret += thr * value * bv * mis(pdf_nee, bp);
@@ -1373,7 +1377,7 @@ template <int N> struct Tracer {
                 }
                 double bpdf, beta;
                 Spec bw = bsdfSample(b, its, rnd3, wo, bpdf, beta);
-                if (bw.isZero(EPS)) break;
+                if (bw.isZero(zeps)) break;
                 wo = its.sh.toWorld(wo);
                 ro = its.p;
                 rd = wo;
@@ -1740,6 +1744,28 @@ template <int N> struct Tracer {
         double t0 = norm(tmp - pe.v0p) / norm(pe.v1p - pe.v0p);
         double t1 = computeIntersectionInTri(V3(), pe.v0p, pe.v1p, pe.v0c, pe.v1c, t0);
         return norm(pe.v0c + t1 * (pe.v1c - pe.v0c));
+    }
+
+    // volpath.cpp:106-141 — VolPathTracer::render: one stream per pixel, a jittered
+    // camera ray per sample, the mean written to float32[H][W][3] (crop size if cropped).
+    void renderPrimal(uint64_t seed, int spp, int max_bounces, float *out) const {
+        const OCamera &cam = *sc.cam;
+        const bool cropped = cam.cropped();
+        const int npix = cropped ? cam.cw * cam.ch : cam.width * cam.height;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int ip = 0; ip < npix; ++ip) {
+            int ix = cropped ? cam.ox + ip % cam.cw : ip % cam.width;
+            int iy = cropped ? cam.oy + ip / cam.cw : ip / cam.width;
+            Pcg s(seed, ip);
+            Spec pix;
+            for (int k = 0; k < spp; ++k) {
+                double rx = s.next1D(), ry = s.next1D();
+                V3 o, d;
+                cam.samplePrimaryRay(ix + rx, iy + ry, o, d);
+                pix += Li(s, o, d, max_bounces, cam.med_id, true, true);
+            }
+            for (int c = 0; c < 3; ++c) out[(size_t)ip * 3 + c] = (float)(pix[c] / spp);
+        }
     }
 
     // integratorAD.cpp:19-166

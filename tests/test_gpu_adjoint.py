@@ -106,3 +106,24 @@ def test_autograd_function_backward_is_the_adjoint(gpu):
     direct = _adjoint(gpu, sc, d, 7, w.cpu().numpy(), spp=d.spp)
     assert np.allclose(rho.grad.cpu().numpy(), direct, rtol=1e-4, atol=1e-6 * np.abs(direct).max())
     assert np.abs(direct).max() > 0
+
+
+def test_pyramid_weighted_adjoint_equals_adlossfunc_on_voxel_lanes(gpu):
+    """The adjoint of the pyramid weight image reports, per grid node, what
+    ADLossFunc.backward (auto_grad.py:46-74; restated in the oracle) computes from the
+    forward-mode tangent images of that node, for a 3-level clamped pyramid."""
+    import torch
+    from dtrt_amd.adjoint import pyramid_weights
+    from oracle import ad_loss
+    d = _density_scene(nder=16)
+    sc = scenes.instantiate(gpu, d)
+    vox = _support_voxels(d.media[0].density, 16, seed=4)
+    gpu.set_voxel_lanes(sc, 0, vox)
+    img = scenes.render(gpu, d, seed=12, scene=sc, mode=INTERIOR, sppe=0, sppse=0).astype(np.float64)
+    g = 1.6 * np.random.default_rng(2).normal(size=img.shape[1:])
+    expect = ad_loss.backward(img[1:], g, pyramid_level=3, pyramid_scale=4.0, clamping=1)
+    W = pyramid_weights(torch.tensor(g, dtype=torch.float32, device="cuda"), 3, 4.0, 1).cpu().numpy()
+    got = _adjoint(gpu, sc, d, 12, W).ravel()[vox]
+    assert np.count_nonzero(np.abs(expect) > 1e-4) >= 8
+    scale = np.abs(expect).max()
+    assert np.allclose(got, expect, rtol=2e-3, atol=2e-4 * scale), np.c_[got, expect]

@@ -59,3 +59,31 @@ def test_medium_albedo_tangent_equals_crn_finite_difference(oracle):
     ad, fd = _fd_crn(oracle, lambda e: F.cfg1_albedo_lane(res=12, spp=4, albedo=0.8 + e, grid_res=12), 1e-3)
     assert np.abs(fd).max() > 1e-2
     assert np.allclose(ad, fd, rtol=2e-3, atol=2e-3)
+
+
+# ---------------------------------------------------------------- VolPathTracer (volpath.cpp)
+@pytest.mark.parametrize("hetero", [False, True])
+def test_primal_furnace(oracle, hetero):
+    d = F.furnace(nder=1, heterogeneous=hetero, res=12, spp=32)
+    v = scenes.render_primal(oracle, d, seed=3)[..., 0].astype(np.float64)
+    assert abs(v.mean() - 1.0) < 4 * v.std() / math.sqrt(v.size) + 1e-3
+
+
+def test_primal_direct_lighting_matches_lambert_polygon(oracle):
+    d, lv, rho, Le = F.direct_lighting(res=16, spp=256)
+    img = scenes.render_primal(oracle, d, seed=1)[..., 0]
+    ref = F.direct_expected(d, lv, rho, Le)
+    assert abs(img.mean() / ref.mean() - 1) < 5e-3
+    assert np.abs(img - ref).max() < 0.25 * ref.max()
+
+
+def test_primal_equals_ad_value_channel(oracle):
+    """VolPathTracer and the value channel of VolPathTracerAD's interior terms estimate
+    the same radiance (volpath.cpp:10-103 vs volpathAD.cpp:145-620 on the value lane)."""
+    d = scenes.config1(nder=1, res=32, spp=16, grid_res=16, crop=(10, 10, 12, 12))
+    sc = scenes.instantiate(oracle, d)
+    a = np.stack([scenes.render_primal(oracle, d, seed=s, scene=sc)[None] for s in range(48)]).astype(np.float64)
+    b = np.stack([scenes.render(oracle, d, seed=1000 + s, scene=sc, mode=1 | 2 | 8, sppe=0, sppse=0)[:1]
+                  for s in range(48)]).astype(np.float64)
+    from tests.helpers import assert_3sigma, block_z
+    assert_3sigma(block_z(a, b, 3), what="primal vs AD value")
