@@ -283,6 +283,10 @@ def bind(lib_path, prefix, name="dtrt"):
             if rc != 0:
                 raise DtrtError("render: " + fns["last_error"]().decode())
 
+    class VolPathTracerSimple(VolPathTracer):
+        """volpath_simple.cpp:89-124: next-event estimation only; float32[H][W][3]."""
+        kind = 2
+
     class _Module:
         pass
 
@@ -293,7 +297,8 @@ def bind(lib_path, prefix, name="dtrt"):
                      Emitter=Emitter, AreaLight=AreaLight, AreaLightEx=AreaLightEx, Medium=Medium,
                      Homogeneous=Homogeneous, Heterogeneous=Heterogeneous, HeterogeneousGrid=HeterogeneousGrid,
                      Scene=Scene, RenderOptions=RenderOptions, Integrator=Integrator,
-                     VolPathTracerAD=VolPathTracerAD, VolPathTracer=VolPathTracer).items():
+                     VolPathTracerAD=VolPathTracerAD, VolPathTracer=VolPathTracer,
+                     VolPathTracerSimple=VolPathTracerSimple).items():
         setattr(m, k, v)
     m.lib = lib
     m.fns = fns

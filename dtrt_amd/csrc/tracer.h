@@ -338,6 +338,64 @@ template <int N> struct Tracer {
         return p.depth <= p.max_depth;
     }
 
+    // volpath_simple.cpp:10-86 — VolPathTracerSimple's bounce: next-event estimation
+    // without MIS; emission found by a traced ray counts only before the first
+    // non-null scattering event. Same random-number order as liStep.
+    DI bool liSimpleStep(LiPath &p, Rng &r) {
+        if (p.depth > p.max_depth) return false;
+        const int max_inter = p.max_depth - p.depth - 1;
+        float r1a = r.next(), r1b = r.next();
+        float r2[4] = {r.next(), r.next(), r.next(), r.next()};
+        float r3[3] = {r.next(), r.next(), r.next()};
+        bool inside = false;
+        if (p.med >= 0) {
+            f3 ps;
+            inside = sampleDistance(S.media[p.med], p.ro, p.rd, p.its.t, r1a, r1b, r, ps, p.thr);
+            if (inside) p.ro = ps;
+        }
+        if (inside) {
+            if (p.depth >= p.max_depth) return false;
+            if (allZero(p.thr, 0.f)) return false;
+            const DPhase &ph = S.phases[S.media[p.med].phase_id];
+            f3 wo;
+            float pdf_nee;
+            f3 value = neePoint(p.ro, r2, r, p.med, max_inter, wo, pdf_nee);
+            if (!allZero(value, 0.f)) {
+                float pv = phaseEval(ph, -p.rd, wo);
+                if (pv != 0.f) p.ret = p.ret + p.thr * value * pv;
+            }
+            p.rd = phaseSample(ph, -p.rd, r3[0], r3[1]);   // weight eval/pdf = 1
+            rayIntersect(p.ro, p.rd, false, p.its);
+            p.incEmission = false;
+        } else {
+            if (p.its.face < 0) return false;
+            const DShape &sh = shapeOf(S, p.its.face);
+            if (sh.light_id >= 0 && p.incEmission)
+                p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], p.its.gn, p.its.sn, -p.rd);
+            if (p.depth >= p.max_depth) return false;
+            const DBsdf &b = bsdfOf(S, sh);
+            Frame shf(p.its.sn);
+            f3 wo;
+            if (b.type != BSDF_NULL) {
+                float pdf_nee;
+                f3 value = neeSurface(p.its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
+                if (!allZero(value, 0.f)) p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo);
+                p.incEmission = false;
+            }
+            float bpdf = 0.f;
+            f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+            if (allZero(bw, 0.f)) return false;
+            p.thr = p.thr * bw;
+            wo = shf.toWorld(wo);
+            p.ro = p.its.p;
+            p.rd = wo;
+            if (isTransition(sh)) p.med = targetMedium(sh, p.its.gn, wo);
+            rayIntersect(p.ro, p.rd, true, p.its);
+        }
+        p.depth++;
+        return p.depth <= p.max_depth;
+    }
+
     DI f3 Li(Rng &r, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {   // volpathAD.cpp:50-142
         LiPath p;
         liStart(p, ro, rd, max_depth, med, incEmission);

@@ -41,6 +41,7 @@ RenderOptions = _m.RenderOptions
 Integrator = _m.Integrator
 VolPathTracerAD = _m.VolPathTracerAD
 VolPathTracer = _m.VolPathTracer
+VolPathTracerSimple = _m.VolPathTracerSimple
 angleEps = _m.angleEps
 abi_version = _m.abi_version
 set_nder = _m.set_nder

@@ -321,9 +321,13 @@ def load_mitsuba(path, nder=1):
 
 def integrator_for(mod, name):
     """Renderer object for an integrator type name of the XML (``volpath``,
-    ``volpathAD``); the reference's other integrators are not provided."""
+    ``volpath_simple``, ``volpathAD``); the reference's other integrators are not
+    provided."""
     if name == "volpathAD":
         return mod.VolPathTracerAD()
     if name == "volpath":
         return mod.VolPathTracer()
-    raise NotImplementedError("integrator %s is not provided (VolPathTracer and VolPathTracerAD are)" % name)
+    if name == "volpath_simple":
+        return mod.VolPathTracerSimple()
+    raise NotImplementedError("integrator %s is not provided (VolPathTracer, VolPathTracerSimple and "
+                              "VolPathTracerAD are)" % name)

@@ -375,13 +375,14 @@ def render(mod, d: SceneDesc, seed=0, spp=None, mode=-1, sppe=None, sppse=None, 
     return out
 
 
-def render_primal(mod, d: SceneDesc, seed=0, spp=None, max_bounces=None, scene=None):
-    """VolPathTracer render of ``d``; returns the primal image float32[H, W, 3]."""
+def render_primal(mod, d: SceneDesc, seed=0, spp=None, max_bounces=None, scene=None, simple=False):
+    """VolPathTracer (``simple``: VolPathTracerSimple) render of ``d``; returns the
+    primal image float32[H, W, 3]."""
     scene = scene if scene is not None else instantiate(mod, d)
     opts = mod.RenderOptions(seed, d.spp if spp is None else spp, d.max_bounces if max_bounces is None else max_bounces,
                              0, 0, True)
     out = np.zeros(d.out_shape()[1:], dtype=np.float32)
-    mod.VolPathTracer().render(scene, opts, out)
+    (mod.VolPathTracerSimple() if simple else mod.VolPathTracer()).render(scene, opts, out)
     return out
 
 

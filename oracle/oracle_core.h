@@ -1395,6 +1395,68 @@ template <int N> struct Tracer {
         return ret;
     }
 
+    // volpath_simple.cpp:10-86 — VolPathTracerSimple::radiance: next-event estimation
+    // without MIS; traced rays add emission only before the first non-null scattering.
+    Spec LiSimple(Pcg &s, V3 ro, V3 rd, int max_depth, int med) const {
+        Its its;
+        Spec ret, thr = Spec::ones();
+        bool incEmission = true;
+        rayIntersect(ro, rd, false, its);
+        for (int depth = 0; depth <= max_depth; ++depth) {
+            int max_inter = max_depth - depth - 1;
+            double rnd1[2] = {s.next1D(), s.next1D()};
+            double rnd2[4] = {s.next1D(), s.next1D(), s.next1D(), s.next1D()};
+            double rnd3[3] = {s.next1D(), s.next1D(), s.next1D()};
+            bool inside = false;
+            if (med >= 0) {
+                V3 ps;
+                inside = sampleDistance(*sc.media[med], ro, rd, its.t, rnd1, s, ps, thr);
+                if (inside) ro = ps;
+            }
+            if (inside) {
+                if (depth >= max_depth) break;
+                if (thr.isZero()) break;
+                const OPhase &ph = *sc.phases[sc.media[med]->phase_id];
+                V3 wo;
+                double pdf_nee;
+                Spec value = sampleAttenuatedEmitterDirectPt(ro, rnd2, s, med, max_inter, wo, pdf_nee);
+                if (!value.isZero()) {
+                    double pv = phaseEval(ph, -rd, wo);
+                    if (pv != 0.0) ret += thr * value * pv;
+                }
+                double pv = phaseSample(ph, -rd, rnd3[0], rnd3[1], wo);
+                if (pv == 0.0) break;
+                thr *= pv;
+                rd = wo;
+                rayIntersect(ro, rd, false, its);
+                incEmission = false;
+            } else {
+                if (its.shape < 0) break;
+                const OShape &shp = shape(its.shape);
+                if (shp.isEmitter() && incEmission) ret += thr * emitterEvalIts(*sc.emitters[shp.light_id], its, -rd);
+                if (depth >= max_depth) break;
+                const OBsdf &b = bsdfOf(its.shape);
+                V3 wo;
+                if (!b.isNull()) {
+                    double pdf_nee;
+                    Spec value = sampleAttenuatedEmitterDirectIts(its, rnd2, s, med, max_inter, wo, pdf_nee, false);
+                    if (!value.isZero()) ret += thr * value * bsdfEval(b, its, wo);
+                    incEmission = false;
+                }
+                double bpdf, beta;
+                Spec bw = bsdfSample(b, its, rnd3, wo, bpdf, beta);
+                if (bw.isZero()) break;
+                thr *= bw;
+                wo = its.sh.toWorld(wo);
+                ro = its.p;
+                rd = wo;
+                if (isTransition(its.shape)) med = targetMedium(its.shape, its.geo.n, wo);
+                rayIntersect(ro, rd, true, its);
+            }
+        }
+        return ret;
+    }
+
     // volpathAD.cpp:145-620 — forward-mode differential estimator
     DS<N> LiAD(const ItsAD<N> &its, Pcg &s, const DV<N> &ray_o, const DV<N> &ray_d, int med, int max_depth, int mode,
                int nEdge, double ddistCoeff) const {
@@ -1746,9 +1808,10 @@ template <int N> struct Tracer {
         return norm(pe.v0c + t1 * (pe.v1c - pe.v0c));
     }
 
-    // volpath.cpp:106-141 — VolPathTracer::render: one stream per pixel, a jittered
-    // camera ray per sample, the mean written to float32[H][W][3] (crop size if cropped).
-    void renderPrimal(uint64_t seed, int spp, int max_bounces, float *out) const {
+    // volpath.cpp:106-141 — VolPathTracer::render (simple: volpath_simple.cpp:89-124,
+    // VolPathTracerSimple::render): one stream per pixel, a jittered camera ray per
+    // sample, the mean written to float32[H][W][3] (crop size if cropped).
+    void renderPrimal(uint64_t seed, int spp, int max_bounces, float *out, bool simple = false) const {
         const OCamera &cam = *sc.cam;
         const bool cropped = cam.cropped();
         const int npix = cropped ? cam.cw * cam.ch : cam.width * cam.height;
@@ -1762,7 +1825,7 @@ template <int N> struct Tracer {
                 double rx = s.next1D(), ry = s.next1D();
                 V3 o, d;
                 cam.samplePrimaryRay(ix + rx, iy + ry, o, d);
-                pix += Li(s, o, d, max_bounces, cam.med_id, true, true);
+                pix += simple ? LiSimple(s, o, d, max_bounces, cam.med_id) : Li(s, o, d, max_bounces, cam.med_id, true, true);
             }
             for (int c = 0; c < 3; ++c) out[(size_t)ip * 3 + c] = (float)(pix[c] / spp);
         }

@@ -1,5 +1,5 @@
-"""VolPathTracer (volpath.cpp:106-141) on the GPU: closed forms and 3-sigma agreement
-with the oracle's restatement."""
+"""VolPathTracer (volpath.cpp:106-141) and VolPathTracerSimple (volpath_simple.cpp:
+89-124) on the GPU: closed forms and 3-sigma agreement with the oracle's restatement."""
 import math
 
 import numpy as np
@@ -12,37 +12,43 @@ from tests.helpers import assert_3sigma, block_z
 pytestmark = pytest.mark.gpu
 
 
-def _stack(mod, d, seeds, sc=None):
+def _stack(mod, d, seeds, sc=None, simple=False):
     sc = sc if sc is not None else scenes.instantiate(mod, d)
-    return np.stack([scenes.render_primal(mod, d, seed=s, scene=sc)[None] for s in seeds]).astype(np.float64)
+    return np.stack([scenes.render_primal(mod, d, seed=s, scene=sc, simple=simple)[None]
+                     for s in seeds]).astype(np.float64)
 
 
+@pytest.mark.parametrize("simple", [False, True])
 @pytest.mark.parametrize("hetero", [False, True])
-def test_primal_furnace(gpu, hetero):
+def test_primal_furnace(gpu, hetero, simple):
     d = F.furnace(nder=1, heterogeneous=hetero, res=32, spp=64)
-    v = scenes.render_primal(gpu, d, seed=3)[..., 0].astype(np.float64)
+    v = scenes.render_primal(gpu, d, seed=3, simple=simple)[..., 0].astype(np.float64)
     assert abs(v.mean() - 1.0) < 4 * v.std() / math.sqrt(v.size) + 1e-3
 
 
-def test_primal_direct_lighting_matches_lambert_polygon(gpu):
+@pytest.mark.parametrize("simple", [False, True])
+def test_primal_direct_lighting_matches_lambert_polygon(gpu, simple):
     d, lv, rho, Le = F.direct_lighting(res=32, spp=1024)
-    img = scenes.render_primal(gpu, d, seed=1)[..., 0]
+    img = scenes.render_primal(gpu, d, seed=1, simple=simple)[..., 0]
     ref = F.direct_expected(d, lv, rho, Le)
     assert abs(img.mean() / ref.mean() - 1) < 3e-3
     assert np.abs(img - ref).max() < 0.1 * ref.max()
 
 
+@pytest.mark.parametrize("simple", [False, True])
 @pytest.mark.parametrize("cfg", ["cfg0", "cfg1", "cfg2"])
-def test_primal_parity(gpu, oracle, cfg):
+def test_primal_parity(gpu, oracle, cfg, simple):
+    """VolPathTracer (simple: VolPathTracerSimple) against the oracle's restatement."""
     if cfg == "cfg0":
         d = scenes.config0(nder=1, res=24, spp=32)
     elif cfg == "cfg1":
         d = scenes.config1(nder=1, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
     else:
         d = scenes.config2(nder=1, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
-    a = _stack(gpu, d, range(16))
-    b = _stack(oracle, d, range(100, 116))
-    assert_3sigma(block_z(a, b, 3), what="VolPathTracer " + cfg)
+    k = 32 if cfg == "cfg2" else 16   # the near-specular dielectric gives heavy-tailed samples
+    a = _stack(gpu, d, range(k), simple=simple)
+    b = _stack(oracle, d, range(100, 100 + k), simple=simple)
+    assert_3sigma(block_z(a, b, 3), what=("VolPathTracerSimple " if simple else "VolPathTracer ") + cfg)
 
 
 def test_primal_every_pixel_written_and_device_output(gpu):

@@ -87,3 +87,28 @@ def test_primal_equals_ad_value_channel(oracle):
                   for s in range(48)]).astype(np.float64)
     from tests.helpers import assert_3sigma, block_z
     assert_3sigma(block_z(a, b, 3), what="primal vs AD value")
+
+
+@pytest.mark.parametrize("hetero", [False, True])
+def test_primal_simple_furnace(oracle, hetero):
+    """VolPathTracerSimple (volpath_simple.cpp): NEE alone also gives L = 1 in a furnace."""
+    d = F.furnace(nder=1, heterogeneous=hetero, res=12, spp=32)
+    v = scenes.render_primal(oracle, d, seed=3, simple=True)[..., 0].astype(np.float64)
+    assert abs(v.mean() - 1.0) < 4 * v.std() / math.sqrt(v.size) + 1e-3
+
+
+def test_primal_simple_direct_lighting_matches_lambert_polygon(oracle):
+    d, lv, rho, Le = F.direct_lighting(res=16, spp=256)
+    img = scenes.render_primal(oracle, d, seed=1, simple=True)[..., 0]
+    ref = F.direct_expected(d, lv, rho, Le)
+    assert abs(img.mean() / ref.mean() - 1) < 5e-3
+
+
+def test_primal_simple_equals_mis_estimator(oracle):
+    """NEE-only and NEE+MIS primal estimators have the same expectation."""
+    d = scenes.config0(nder=1, res=24, spp=8)
+    sc = scenes.instantiate(oracle, d)
+    a = np.stack([scenes.render_primal(oracle, d, seed=s, scene=sc, simple=True)[None] for s in range(32)])
+    b = np.stack([scenes.render_primal(oracle, d, seed=500 + s, scene=sc)[None] for s in range(32)])
+    from tests.helpers import assert_3sigma, block_z
+    assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 3), what="simple vs MIS")
