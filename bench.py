@@ -194,6 +194,73 @@ def views_leg(nviews=32):
                             "replays_that_fell_back": int(fallbacks)}}
 
 
+def primal_leg(res=512, spp=128, reps=2):
+    """VolPathTracer and VolPathTracerSimple (primal target renders) on the same
+    heterogeneous-medium scene and size as the main workload."""
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    d = scenes.config1(nder=3, res=res, spp=spp)
+    sc = scenes.instantiate(G, d)
+    out = torch.zeros((d.height, d.width, 3), device="cuda")
+    rec = {"workload": "primal render of the heterogeneous-medium scene, %d^2, %d spp, depth %d" % (res, spp, d.max_bounces)}
+    for kind, name in ((1, "VolPathTracer"), (2, "VolPathTracerSimple")):
+        opts = G.RenderOptions(7, spp, d.max_bounces, 0, 0, True)
+        G.render_device(kind, sc, opts, out.data_ptr())   # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(reps):
+            opts.seed = 8 + k
+            G.render_device(kind, sc, opts, out.data_ptr())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        rec[name] = {"s_per_render": dt, "samples_per_s": res * res * spp / dt}
+    return rec
+
+
+def glass_leg(iters=4):
+    """One optimisation step of the reference's glass_1 example (examples/glass_1/
+    optimize.py: 256^2, spp 1024, sppe 32768, sppse 10, 4 bounces, 4 variables, 10-level
+    pyramid loss, Adam): AD render + loss + backward + parameter update, averaged."""
+    import numpy as np
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import loaders
+    from dtrt_amd.harness import ADLossFunc, SceneManager, SceneTransform
+    d, integ = loaders.load_mitsuba(os.path.join(ROOT, "tests", "golden", "scenes", "glass_1", "scene.xml"), nder=4)
+    opts = G.RenderOptions(0, 1024, 4, 32768, 10, True)
+    sm = SceneManager(d, [[SceneTransform("SHAPE_TRANSLATE", [1.0, 0, 0], 0)],
+                          [SceneTransform("SHAPE_TRANSLATE", [0, 1.0, 0], 0)],
+                          [SceneTransform("SHAPE_TRANSLATE", [0, 0, 1.0], 0)],
+                          [SceneTransform("BSDF_VARY", "roughdielectric", 0, 0.1, 0.0)]])
+    sm.set_arguments([-5.0, -3.0, -4.0, -2.5])
+    tgt = torch.zeros(d.out_shape(), device="cuda")
+    G.render_device(loaders.integrator_for(G, integ).kind, sm.scene(G), opts, tgt.data_ptr())
+    target = tgt.view(-1)[: d.height * d.width * 3].view(d.height, d.width, 3).clone()
+    sm.reset()
+    var = torch.zeros(4, requires_grad=True)
+    optim = torch.optim.Adam([var], lr=2e-1)
+    oor = torch.zeros(4)
+    times, losses = [], []
+    for t in range(iters + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        optim.zero_grad()
+        opts.seed = t + 1
+        img = ADLossFunc.apply(sm, G.VolPathTracerAD(), opts, var, oor, torch.tensor([10000.0] * 4), 10)
+        loss = (img - target).pow(2).sum()
+        loss.backward()
+        optim.step()
+        oor = torch.as_tensor(sm.set_arguments(var.detach().numpy()))
+        torch.cuda.synchronize()
+        if t > 0:   # the first step also builds the scene's device buffers
+            times.append(time.perf_counter() - t0)
+        losses.append(float(loss))
+    return {"workload": "examples/glass_1 optimisation step at the script's settings (256^2, spp 1024, sppe 32768, "
+                        "sppse 10, nder 4)", "s_per_step": float(np.mean(times)), "steps_timed": iters,
+            "loss_first_steps": losses, "reference_loss_first_steps": [820.2951, 721.77826, 631.98737, 548.87231, 474.86902]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +271,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward gradient timing")
     ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view timing")
+    ap.add_argument("--no-primal", action="store_true", help="skip the primal-integrator timing")
+    ap.add_argument("--no-examples", action="store_true", help="skip the glass_1 optimisation-step timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -297,6 +366,10 @@ def main():
         rec["views"] = views_leg()
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_leg()
+    if not args.no_primal:
+        rec["primal"] = primal_leg(args.res, args.spp)
+    if not args.no_examples:
+        rec["glass_1"] = glass_leg()
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(desc)
         rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
