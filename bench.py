@@ -255,7 +255,7 @@ def glass_leg(iters=4):
         torch.cuda.synchronize()
         if t > 0:   # the first step also builds the scene's device buffers
             times.append(time.perf_counter() - t0)
-        losses.append(float(loss))
+        losses.append(loss.item())
     return {"workload": "examples/glass_1 optimisation step at the script's settings (256^2, spp 1024, sppe 32768, "
                         "sppse 10, nder 4)", "s_per_step": float(np.mean(times)), "steps_timed": iters,
             "loss_first_steps": losses, "reference_loss_first_steps": [820.2951, 721.77826, 631.98737, 548.87231, 474.86902]}

@@ -287,6 +287,14 @@ def bind(lib_path, prefix, name="dtrt"):
         """volpath_simple.cpp:89-124: next-event estimation only; float32[H][W][3]."""
         kind = 2
 
+    class PathTracer(VolPathTracer):
+        """path.cpp:64-96: surfaces only, NEE + BSDF sampling with MIS; float32[H][W][3]."""
+        kind = 3
+
+    class DirectIntegrator(VolPathTracer):
+        """direct.cpp:51-85: direct illumination (one bounce) with MIS; float32[H][W][3]."""
+        kind = 4
+
     class _Module:
         pass
 
@@ -298,7 +306,8 @@ def bind(lib_path, prefix, name="dtrt"):
                      Homogeneous=Homogeneous, Heterogeneous=Heterogeneous, HeterogeneousGrid=HeterogeneousGrid,
                      Scene=Scene, RenderOptions=RenderOptions, Integrator=Integrator,
                      VolPathTracerAD=VolPathTracerAD, VolPathTracer=VolPathTracer,
-                     VolPathTracerSimple=VolPathTracerSimple).items():
+                     VolPathTracerSimple=VolPathTracerSimple, PathTracer=PathTracer,
+                     DirectIntegrator=DirectIntegrator).items():
         setattr(m, k, v)
     m.lib = lib
     m.fns = fns

@@ -593,7 +593,7 @@ struct Scene : Obj {
     bool setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
                     std::vector<uint64_t> &pe_total);
     bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
-    bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, bool simple);
+    bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator);
     bool setDensity(int med, const float *density, bool on_device);
     bool setVoxelLanes(int med, const int *vox, int n);
     bool uploadMedia() {
@@ -1226,13 +1226,15 @@ bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *di
     return true;
 }
 
-// VolPathTracer::render (volpath.cpp:106-141) and VolPathTracerSimple::render
-// (volpath_simple.cpp:89-124): float32[H][W][3] (crop size if cropped), each pixel the
-// mean of spp jittered primal samples; every pixel is written.
-bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, bool simple) {
+// The primal integrators' render (volpath.cpp:106-141, volpath_simple.cpp:89-124,
+// path.cpp:64-96, direct.cpp:51-85): float32[H][W][3] (crop size if cropped), each
+// pixel the mean of spp jittered primal samples; every pixel is written. integrator:
+// 1 VolPathTracer, 2 VolPathTracerSimple, 3 PathTracer, 4 DirectIntegrator.
+bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator) {
     if (o.spp <= 0 || o.max_bounces < 0 || o.max_bounces > 200) return fail("render: bad options");
     RenderParams P{};
-    P.max_bounces = o.max_bounces;
+    P.max_bounces = integrator == 4 ? 1 : o.max_bounces;
+    const int kind = integrator == 2 ? PRIMAL_SIMPLE : integrator >= 3 ? PRIMAL_PATH : PRIMAL_VOLPATH;
     dtrt_render_options ov = o;
     ov.sppe = 0;   // no primary-edge pass
     const Camera *c = &cam;
@@ -1260,7 +1262,7 @@ bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_d
     std::memset(kernel_launches, 0, sizeof(kernel_launches));
     HIPCHK(hipEventRecord(ev[0], stream));
     for (uint64_t s0 = 0; s0 < total; s0 += chunk)
-        HIPCHK(launch_primal_paths(S, P, simple, s0, std::min(chunk, total - s0), A, wk, stream));
+        HIPCHK(launch_primal_paths(S, P, kind, s0, std::min(chunk, total - s0), A, wk, stream));
     HIPCHK(hipEventRecord(ev[1], stream));
     HIPCHK(launch_scale(A, dout, n3, 1.0f / (float)o.spp, stream));
     if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, n3 * 4, hipMemcpyDeviceToHost, stream));
@@ -1637,14 +1639,15 @@ int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {
 }
 
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
-    if (integrator < 0 || integrator > 2) {
-        g_err = "render: integrators are VolPathTracerAD (0), VolPathTracer (1) and VolPathTracerSimple (2)";
+    if (integrator < 0 || integrator > 4) {
+        g_err = "render: integrators are VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2), "
+                "PathTracer (3) and DirectIntegrator (4)";
         return -2;
     }
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s || !o || !img) { if (s) g_err = "render: null argument"; return -1; }
     try {
-        if (integrator > 0) return s->renderPrimal(*o, img, dev, integrator == 2) ? 0 : -1;
+        if (integrator > 0) return s->renderPrimal(*o, img, dev, integrator) ? 0 : -1;
         return s->render(*o, img, dev) ? 0 : -1;
     } catch (const std::exception &e) {
         g_err = e.what();

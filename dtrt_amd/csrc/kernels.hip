@@ -502,12 +502,16 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
 }
 
-// VolPathTracer (volpath.cpp:106-141) and VolPathTracerSimple (volpath_simple.cpp:
-// 89-124): one thread per primal sample s0 .. s0+n-1 of view 0 (sample k of pixel p
-// is p*spp + k), a jittered camera ray and the primal estimator: Tracer::liStep
-// (volpath.cpp:10-103 is volpathAD.cpp:50-142 with the first ray leaving the camera)
-// or, SIMPLE, Tracer::liSimpleStep. Radiance sums go to acc[pixel][3].
-template <bool SIMPLE>
+// The primal integrators: one thread per sample s0 .. s0+n-1 of view 0 (sample k of
+// pixel p is p*spp + k), a jittered camera ray and the estimator of KIND:
+//   PRIMAL_VOLPATH  VolPathTracer (volpath.cpp:106-141): Tracer::liStep
+//                   (volpath.cpp:10-103 is volpathAD.cpp:50-142 with the first ray
+//                   leaving the camera);
+//   PRIMAL_SIMPLE   VolPathTracerSimple (volpath_simple.cpp:89-124): liSimpleStep;
+//   PRIMAL_PATH     PathTracer (path.cpp:64-96) and DirectIntegrator (direct.cpp:
+//                   51-85, max depth 1): pathStep, camera ray from tnear ShadowEpsilon.
+// Radiance sums go to acc[pixel][3].
+template <int KIND>
 __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DScene S, RenderParams P, uint64_t s0,
                                                                               uint64_t nsamp, float *acc,
                                                                               unsigned long long *work) {
@@ -529,11 +533,13 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DS
         f3 ro, rd;
         cameraRay(cam, ix + jx, iy + jy, ro, rd);
         Hit h;
-        T.rayIntersect(ro, rd, false, h);
+        T.rayIntersect(ro, rd, KIND == PRIMAL_PATH, h);
         LiPath p;
-        T.liStartAt(p, ro, rd, P.max_bounces, cam.med_id, true, h);
-        if constexpr (SIMPLE) {
+        T.liStartAt(p, ro, rd, P.max_bounces, KIND == PRIMAL_PATH ? -1 : cam.med_id, true, h);
+        if constexpr (KIND == PRIMAL_SIMPLE) {
             while (T.liSimpleStep(p, r)) {}
+        } else if constexpr (KIND == PRIMAL_PATH) {
+            while (T.pathStep(p, r)) {}
         } else {
             while (T.liStep(p, r)) {}
         }
@@ -1003,14 +1009,18 @@ hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, fl
     hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, face, t);
     return hipGetLastError();
 }
-hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, bool simple, uint64_t s0, uint64_t n, float *acc,
+hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, float *acc,
                               unsigned long long *work, hipStream_t sm) {
     const uint64_t nb = (n + BLOCK - 1) / BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    if (simple)
-        hipLaunchKernelGGL(k_primal_paths<true>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+    if (kind == PRIMAL_SIMPLE)
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+    else if (kind == PRIMAL_PATH)
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+    else if (kind == PRIMAL_VOLPATH)
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
     else
-        hipLaunchKernelGGL(k_primal_paths<false>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm) {

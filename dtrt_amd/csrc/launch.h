@@ -57,9 +57,9 @@ hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm);
 hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm);
 // dst = max(src, 0) over n floats; atomicMax of the result's bits into *maxbits.
 hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm);
-// VolPathTracer (simple: VolPathTracerSimple): primal samples s0 .. s0+n-1 of view 0,
-// radiance sums into acc[pixel][3].
-hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, bool simple, uint64_t s0, uint64_t n, float *acc,
+// Primal integrators: samples s0 .. s0+n-1 of view 0, radiance sums into acc[pixel][3].
+enum PrimalKind { PRIMAL_VOLPATH = 1, PRIMAL_SIMPLE = 2, PRIMAL_PATH = 3 };
+hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, float *acc,
                               unsigned long long *work, hipStream_t sm);
 // out[i] = acc[i] * scale, i < n.
 hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm);

@@ -396,6 +396,66 @@ template <int N> struct Tracer {
         return p.depth <= p.max_depth;
     }
 
+    // scene.cpp:302-344 — sampleEmitterDirect: no media; the light point is visible
+    // when no surface lies within [ShadowEpsilon, (1 - ShadowEpsilon) * dist]
+    DI f3 neeDirect(const Hit &its, const Frame &shf, const float *r4, f3 &wo, float &pdf) {
+        if (S.nlights == 0) return f3{0, 0, 0};
+        f3 lp, ln;
+        int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - its.p;
+        pdf = S.light_pmf[l] / S.light_areas[l];
+        if (dot(dir, its.sn) > 0.f && dot(dir, ln) < 0.f && pdf != 0.f) {
+            float dist = length(dir), th;
+            dir = dir / dist;
+            if (trace(its.p, dir, SHADOW_EPS, (1.0f - SHADOW_EPS) * dist, th) < 0) {
+                pdf *= dist * dist / dot(ln, -dir);
+                wo = shf.toLocal(dir);
+                return emitterEvalNorm(S.emitters[l], ln, -dir) * (1.0f / pdf);
+            }
+        }
+        return f3{0, 0, 0};
+    }
+
+    // path.cpp:11-61 — one bounce of PathTracer::radiance: surfaces only (media are
+    // ignored), emission seen directly at the first hit, NEE and BSDF sampling with the
+    // power heuristic. p.its holds the current hit; the caller traces the camera ray
+    // with tnear = ShadowEpsilon, as path.cpp:14 does.
+    DI bool pathStep(LiPath &p, Rng &r) {
+        if (p.depth > p.max_depth || p.its.face < 0) return false;
+        const DShape &sh = shapeOf(S, p.its.face);
+        if (sh.light_id >= 0 && p.depth == 0)
+            p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], p.its.gn, p.its.sn, -p.rd);
+        if (p.depth >= p.max_depth) return false;
+        const DBsdf &b = bsdfOf(S, sh);
+        Frame shf(p.its.sn);
+        float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+        f3 wo;
+        float pdf_nee;
+        f3 value = neeDirect(p.its, shf, r4, wo, pdf_nee);
+        if (!allZero(value, 0.f))
+            p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, p.its.wi, wo));
+        float r3[3] = {r.next(), r.next(), r.next()};
+        float bpdf = 0.f;
+        f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+        if (allZero(bw, 0.f)) return false;
+        p.ro = p.its.p;
+        p.rd = shf.toWorld(wo);
+        if (!rayIntersect(p.ro, p.rd, true, p.its)) return false;
+        p.thr = p.thr * bw;
+        const DShape &hs = shapeOf(S, p.its.face);
+        if (hs.light_id >= 0) {
+            f3 lc = emitterEval(S.emitters[hs.light_id], p.its.gn, p.its.sn, -p.rd);
+            if (!allZero(lc, 0.f)) {
+                f3 dd = p.its.p - p.ro;
+                float G = p.its.wi.z / dot(dd, dd);
+                float pn = S.light_pmf[hs.light_id] / (S.light_areas[hs.light_id] * G);
+                p.ret = p.ret + p.thr * lc * misPower(bpdf, pn);
+            }
+        }
+        p.depth++;
+        return true;
+    }
+
     DI f3 Li(Rng &r, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {   // volpathAD.cpp:50-142
         LiPath p;
         liStart(p, ro, rd, max_depth, med, incEmission);

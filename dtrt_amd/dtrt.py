@@ -42,6 +42,8 @@ Integrator = _m.Integrator
 VolPathTracerAD = _m.VolPathTracerAD
 VolPathTracer = _m.VolPathTracer
 VolPathTracerSimple = _m.VolPathTracerSimple
+PathTracer = _m.PathTracer
+DirectIntegrator = _m.DirectIntegrator
 angleEps = _m.angleEps
 abi_version = _m.abi_version
 set_nder = _m.set_nder

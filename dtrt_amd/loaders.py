@@ -320,14 +320,17 @@ def load_mitsuba(path, nder=1):
 
 
 def integrator_for(mod, name):
-    """Renderer object for an integrator type name of the XML (``volpath``,
-    ``volpath_simple``, ``volpathAD``); the reference's other integrators are not
-    provided."""
+    """Renderer object for an integrator type name of the XML (``volpathAD``,
+    ``volpath``, ``volpath_simple``, ``path``, ``direct``); ``pathAD``, ``directAD`` and
+    ``ptracer`` are not provided."""
     if name == "volpathAD":
         return mod.VolPathTracerAD()
     if name == "volpath":
         return mod.VolPathTracer()
     if name == "volpath_simple":
         return mod.VolPathTracerSimple()
-    raise NotImplementedError("integrator %s is not provided (VolPathTracer, VolPathTracerSimple and "
-                              "VolPathTracerAD are)" % name)
+    if name == "path":
+        return mod.PathTracer()
+    if name == "direct":
+        return mod.DirectIntegrator()
+    raise NotImplementedError("integrator %s is not provided" % name)

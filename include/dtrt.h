@@ -122,10 +122,12 @@ int dtrt_scene_init_edges(dtrt_object *scene, const float *weights);
  * float32[(nder+1)][H][W][3] (crop size if cropped), zeroed by the caller; values
  * and tangents the reference would drop (negative pixel, non-finite tangent) are
  * left untouched, primary-edge contributions are added. Synchronous.
- * integrator 1 = VolPathTracer (src/dtrt.cpp:132-134, volpath.cpp:106-141) and
- * integrator 2 = VolPathTracerSimple (src/dtrt.cpp:136-138, volpath_simple.cpp:89-124):
- * the primal image float32[H][W][3], every pixel written with the mean of spp jittered
- * samples; sppe, sppse, mode and ddistCoeff are ignored. */
+ * The primal integrators write the image float32[H][W][3], every pixel the mean of spp
+ * jittered samples; sppe, sppse, mode and ddistCoeff are ignored:
+ *   1 = VolPathTracer       (src/dtrt.cpp:132-134, volpath.cpp:106-141)
+ *   2 = VolPathTracerSimple (src/dtrt.cpp:136-138, volpath_simple.cpp:89-124)
+ *   3 = PathTracer          (src/dtrt.cpp:123-125, path.cpp:64-96; surfaces only)
+ *   4 = DirectIntegrator    (src/dtrt.cpp:119-121, direct.cpp:51-85; surfaces only) */
 int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image);
 /* Same, with `image` in device memory of the current HIP device (no host copies). */
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);

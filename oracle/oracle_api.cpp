@@ -420,18 +420,19 @@ void oracle_counters(long long *out6) {
     out6[3] = g_counters.bad_edge_weight; out6[4] = g_counters.nonortho; out6[5] = g_counters.med_inconsistent;
 }
 
-// integrator kind 0: VolPathTracerAD (integratorAD.cpp:19-166); kind 1: VolPathTracer
-// (volpath.cpp:106-141) and kind 2: VolPathTracerSimple (volpath_simple.cpp:89-124),
-// float32[H][W][3]
+// integrator kind 0: VolPathTracerAD (integratorAD.cpp:19-166); kinds 1-4, float32[H][W][3]:
+// VolPathTracer (volpath.cpp:106-141), VolPathTracerSimple (volpath_simple.cpp:89-124),
+// PathTracer (path.cpp:64-96), DirectIntegrator (direct.cpp:51-85)
 int oracle_render(int integrator, void *scene, const oracle_render_options *o, float *out) {
-    if (integrator < 0 || integrator > 2) {
-        g_err = "integrators: VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2)";
+    if (integrator < 0 || integrator > 4) {
+        g_err = "integrators: VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2), PathTracer (3), "
+                "DirectIntegrator (4)";
         return -2;
     }
     OScene &s = static_cast<HScene *>(scene)->s;
     g_counters.reset();
     if (integrator > 0) {
-        Tracer<1>(s).renderPrimal(o->seed, o->spp, o->max_bounces, out, integrator == 2);
+        Tracer<1>(s).renderPrimal(o->seed, o->spp, o->max_bounces, out, integrator);
         return 0;
     }
 #define OR_CASE(K) case K: Tracer<K>(s).render(o->seed, o->spp, o->max_bounces, o->sppe, o->sppse, o->mode, o->ddist_coeff, out); break;

@@ -1262,6 +1262,26 @@ template <int N> struct Tracer {
         }
         return Spec();
     }
+    // scene.cpp:302-344 — sampleEmitterDirect: no media; visible when no surface lies
+    // within [ShadowEpsilon, (1 - ShadowEpsilon) * dist] along the shadow ray
+    Spec sampleEmitterDirect(const Its &its, const double *rnd4, V3 &wo, double &pdf) const {
+        int l, k;
+        V3 lp, ln;
+        sampleLight(rnd4, l, k, lp, ln);
+        V3 dir = lp - its.p;
+        pdf = sc.light_pmf[l] / sc.light_areas[l];
+        if (dot(dir, its.sh.n) > 0 && dot(dir, ln) < 0 && pdf != 0) {
+            double dist = norm(dir);
+            dir = dir / dist;
+            int hs, hk;
+            if (!sc.intersect(its.p, dir, SHADOW_EPS, (1.0 - SHADOW_EPS) * dist, hs, hk)) {
+                pdf *= dist * dist / dot(ln, -dir);
+                wo = its.sh.toLocal(dir);
+                return emitterEvalNorm(*sc.emitters[l], ln, -dir) / pdf;
+            }
+        }
+        return Spec();
+    }
     // scene.cpp:428-454
     DS<N> sampleAttenuatedEmitterDirectPtAD(const DV<N> &ps, const double *rnd4, Pcg &s, int medium, int max_inter,
                                             DV<N> &wo, double &pdf) const {
@@ -1452,6 +1472,47 @@ template <int N> struct Tracer {
                 rd = wo;
                 if (isTransition(its.shape)) med = targetMedium(its.shape, its.geo.n, wo);
                 rayIntersect(ro, rd, true, its);
+            }
+        }
+        return ret;
+    }
+
+    // path.cpp:11-61 — PathTracer::radiance: surfaces only (media are ignored), emission
+    // seen directly at the first hit, NEE + BSDF sampling with the power heuristic.
+    // DirectIntegrator::radiance (direct.cpp:7-48) is this with max_depth 1.
+    Spec LiPath(Pcg &s, V3 ro, V3 rd, int max_depth) const {
+        Its its;
+        Spec ret;
+        if (!rayIntersect(ro, rd, true, its)) return ret;
+        Spec thr = Spec::ones();
+        for (int depth = 0; depth <= max_depth && its.shape >= 0; ++depth) {
+            const OShape &shp = shape(its.shape);
+            if (shp.isEmitter() && depth == 0) ret += thr * emitterEvalIts(*sc.emitters[shp.light_id], its, -rd);
+            if (depth >= max_depth) break;
+            const OBsdf &b = bsdfOf(its.shape);
+            double rnd4[4] = {s.next1D(), s.next1D(), s.next1D(), s.next1D()};
+            V3 wo;
+            double pdf_nee;
+            Spec value = sampleEmitterDirect(its, rnd4, wo, pdf_nee);
+            if (!value.isZero()) ret += thr * value * bsdfEval(b, its, wo) * mis(pdf_nee, bsdfPdf(b, its, wo));
+            double rnd3[3] = {s.next1D(), s.next1D(), s.next1D()};
+            double bpdf, beta;
+            Spec bw = bsdfSample(b, its, rnd3, wo, bpdf, beta);
+            if (bw.isZero()) break;
+            wo = its.sh.toWorld(wo);
+            ro = its.p;
+            rd = wo;
+            if (!rayIntersect(ro, rd, true, its)) break;
+            thr *= bw;
+            const OShape &hs = shape(its.shape);
+            if (hs.isEmitter()) {
+                Spec lc = emitterEvalIts(*sc.emitters[hs.light_id], its, -rd);
+                if (!lc.isZero()) {
+                    V3 dd = its.p - ro;
+                    double G = its.wi.z / dot(dd, dd);
+                    double pn = sc.light_pmf[hs.light_id] / sc.light_areas[hs.light_id] / G;
+                    ret += thr * lc * mis(bpdf, pn);
+                }
             }
         }
         return ret;
@@ -1811,7 +1872,9 @@ template <int N> struct Tracer {
     // volpath.cpp:106-141 — VolPathTracer::render (simple: volpath_simple.cpp:89-124,
     // VolPathTracerSimple::render): one stream per pixel, a jittered camera ray per
     // sample, the mean written to float32[H][W][3] (crop size if cropped).
-    void renderPrimal(uint64_t seed, int spp, int max_bounces, float *out, bool simple = false) const {
+    // kind: 1 VolPathTracer, 2 VolPathTracerSimple, 3 PathTracer (path.cpp:64-96),
+    // 4 DirectIntegrator (direct.cpp:51-85, = PathTracer at depth 1)
+    void renderPrimal(uint64_t seed, int spp, int max_bounces, float *out, int kind = 1) const {
         const OCamera &cam = *sc.cam;
         const bool cropped = cam.cropped();
         const int npix = cropped ? cam.cw * cam.ch : cam.width * cam.height;
@@ -1825,7 +1888,9 @@ template <int N> struct Tracer {
                 double rx = s.next1D(), ry = s.next1D();
                 V3 o, d;
                 cam.samplePrimaryRay(ix + rx, iy + ry, o, d);
-                pix += simple ? LiSimple(s, o, d, max_bounces, cam.med_id) : Li(s, o, d, max_bounces, cam.med_id, true, true);
+                if (kind == 2) pix += LiSimple(s, o, d, max_bounces, cam.med_id);
+                else if (kind >= 3) pix += LiPath(s, o, d, kind == 4 ? 1 : max_bounces);
+                else pix += Li(s, o, d, max_bounces, cam.med_id, true, true);
             }
             for (int c = 0; c < 3; ++c) out[(size_t)ip * 3 + c] = (float)(pix[c] / spp);
         }

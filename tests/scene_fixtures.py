@@ -256,3 +256,15 @@ def rotating_grid(nder=2, res=48, spp=16, grid_res=16, crop=(12, 12, 24, 24)):
     m.d_scale = np.array([0.0, 1.0] + [0.0] * (nder - 2), np.float32)
     d.sppe, d.sppse, d.max_bounces = 256, 4, 6
     return d
+
+
+def diffuse_furnace(albedo=0.5, Le=1.0, res=16, spp=16, depth=5):
+    """Camera inside a closed inward-facing sphere that both emits Le (two-sided) and
+    reflects diffusely with the given albedo; no media. Every path tracer that counts
+    the emission seen at the first hit and estimates each further bounce without bias
+    gives L = Le * sum_{k=0..depth} albedo^k in every pixel."""
+    v, f = scenes.icosphere(3, 1.0, (0, 0, 0), inward=True)
+    return SceneDesc(nder=1, width=res, height=res, cam_to_world=scenes.look_at((0, 0, 0), (0, 0, -1), (0, 1, 0)),
+                     cam_to_ndc=scenes.cam_to_ndc(60.0), shapes=[ShapeDesc(v, f, bsdf_id=0, light_id=0)],
+                     bsdfs=[("diffuse", (albedo, albedo, albedo))], lights=[("area", 0, (Le, Le, Le), True)],
+                     spp=spp, max_bounces=depth, sppe=-1, sppse=0, name="diffuse furnace")

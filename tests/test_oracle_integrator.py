@@ -112,3 +112,23 @@ def test_primal_simple_equals_mis_estimator(oracle):
     b = np.stack([scenes.render_primal(oracle, d, seed=500 + s, scene=sc)[None] for s in range(32)])
     from tests.helpers import assert_3sigma, block_z
     assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 3), what="simple vs MIS")
+
+
+@pytest.mark.parametrize("kind,depth", [(3, 5), (4, 5), (3, 1)])
+def test_surface_integrators_diffuse_furnace(oracle, kind, depth):
+    """PathTracer (path.cpp) and DirectIntegrator (direct.cpp, one bounce) inside an
+    emitting diffuse sphere: L = Le * sum_{k<=depth} a^k."""
+    a = 0.5
+    d = F.diffuse_furnace(albedo=a, Le=2.0, res=12, spp=32, depth=depth)
+    v = scenes.render_primal(oracle, d, seed=2, kind=kind)[..., 0].astype(np.float64)
+    eff = 1 if kind == 4 else depth
+    want = 2.0 * sum(a ** k for k in range(eff + 1))
+    assert abs(v.mean() / want - 1) < 4 * v.std() / math.sqrt(v.size) / want + 1e-3
+
+
+@pytest.mark.parametrize("kind", [3, 4])
+def test_surface_integrators_direct_lighting_matches_lambert_polygon(oracle, kind):
+    d, lv, rho, Le = F.direct_lighting(res=16, spp=256)
+    img = scenes.render_primal(oracle, d, seed=1, kind=kind)[..., 0]
+    ref = F.direct_expected(d, lv, rho, Le)
+    assert abs(img.mean() / ref.mean() - 1) < 5e-3
