@@ -295,6 +295,10 @@ def bind(lib_path, prefix, name="dtrt"):
         """direct.cpp:51-85: direct illumination (one bounce) with MIS; float32[H][W][3]."""
         kind = 4
 
+    class ParticleTracer(VolPathTracer):
+        """ptracer.cpp:10-167: 100000 * spp light particles splatted to the camera; float32[H][W][3]."""
+        kind = 5
+
     class _Module:
         pass
 
@@ -307,7 +311,7 @@ def bind(lib_path, prefix, name="dtrt"):
                      Scene=Scene, RenderOptions=RenderOptions, Integrator=Integrator,
                      VolPathTracerAD=VolPathTracerAD, VolPathTracer=VolPathTracer,
                      VolPathTracerSimple=VolPathTracerSimple, PathTracer=PathTracer,
-                     DirectIntegrator=DirectIntegrator).items():
+                     DirectIntegrator=DirectIntegrator, ParticleTracer=ParticleTracer).items():
         setattr(m, k, v)
     m.lib = lib
     m.fns = fns

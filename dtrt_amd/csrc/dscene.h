@@ -616,7 +616,9 @@ template <int N> DI Ds<N> specAD(const float *v, const float (*d)[3]) {
 template <int N> DI Df<N> scalarAD(float v, const float *d) { Df<N> r(v); for (int j = 0; j < N; ++j) r.d[j] = d[j]; return r; }
 
 // diffuse.cpp:7-38, null.cpp:5-29, phong.cpp:19-95, roughdielectric.cpp:9-206
-DI f3 bsdfEval(const DBsdf &b, f3 wi, f3 wo) {
+// importance: adjoint transport (light tracing); drops the radiance scaling of
+// refraction, roughdielectric.cpp:50,156
+DI f3 bsdfEval(const DBsdf &b, f3 wi, f3 wo, bool importance = false) {
     switch (b.type) {
         case BSDF_DIFFUSE:
             if (wi.z < EPS || wo.z < EPS) return f3{0, 0, 0};
@@ -643,7 +645,7 @@ DI f3 bsdfEval(const DBsdf &b, f3 wi, f3 wo) {
             float eta = wi.z > 0.f ? b.eta : invEta;
             float sd = dot(wi, H) + eta * dot(wo, H);
             float value = ((1.0f - F) * Dv_ * G * eta * eta * dot(wi, H) * dot(wo, H)) / (wi.z * sd * sd);
-            float factor = wi.z > 0.f ? invEta : b.eta;
+            float factor = importance ? 1.0f : (wi.z > 0.f ? invEta : b.eta);
             float v = fabsf(value * factor * factor);
             return f3{v, v, v};
         }
@@ -678,7 +680,7 @@ DI float bsdfPdf(const DBsdf &b, f3 wi, f3 wo) {
         }
     }
 }
-DI f3 bsdfSample(const DBsdf &b, f3 wi, float r0, float r1, float r2, f3 &wo, float &pdf) {
+DI f3 bsdfSample(const DBsdf &b, f3 wi, float r0, float r1, float r2, f3 &wo, float &pdf, bool importance = false) {
     switch (b.type) {
         case BSDF_DIFFUSE:
             if (wi.z < EPS) return f3{0, 0, 0};
@@ -723,7 +725,7 @@ DI f3 bsdfSample(const DBsdf &b, f3 wi, float r0, float r1, float r2, f3 &wo, fl
                 wo = refract(wi, m, b.eta, ct);
                 float eta = ct < 0.f ? b.eta : invEta;
                 if (wi.z * wo.z >= 0.f) return f3{0, 0, 0};
-                float factor = ct < 0.f ? invEta : b.eta;
+                float factor = importance ? 1.0f : (ct < 0.f ? invEta : b.eta);
                 weight *= factor * factor;
                 float sd = dot(wi, m) + eta * dot(wo, m);
                 dwh = (eta * eta * dot(wo, m)) / (sd * sd);

@@ -495,6 +495,8 @@ void fillDCamera(const Camera &cam, int nder, DCamera &c) {
     c.cropped = cam.cropped(); c.med_id = cam.med;
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) c.c2w[4 * i + j] = (float)cam.c2w[i][j];
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c.ndc2c[3 * i + j] = (float)cam.ndc2c[i][j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) c.w2c[4 * i + j] = (float)cam.w2c[i][j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c.c2ndc[3 * i + j] = (float)cam.c2ndc[i][j];
     c.fov = (float)cam.c2ndc[0][0];
     c.aspect = (float)cam.w / (float)cam.h;
     c.clip_near = cam.clip_near;
@@ -1227,9 +1229,10 @@ bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *di
 }
 
 // The primal integrators' render (volpath.cpp:106-141, volpath_simple.cpp:89-124,
-// path.cpp:64-96, direct.cpp:51-85): float32[H][W][3] (crop size if cropped), each
-// pixel the mean of spp jittered primal samples; every pixel is written. integrator:
-// 1 VolPathTracer, 2 VolPathTracerSimple, 3 PathTracer, 4 DirectIntegrator.
+// path.cpp:64-96, direct.cpp:51-85, ptracer.cpp:10-55): float32[H][W][3] (crop size if
+// cropped), every pixel written. integrator: 1 VolPathTracer, 2 VolPathTracerSimple,
+// 3 PathTracer, 4 DirectIntegrator (each pixel the mean of spp jittered samples);
+// 5 ParticleTracer (100000 * spp light particles, splatted and divided by their count).
 bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator) {
     if (o.spp <= 0 || o.max_bounces < 0 || o.max_bounces > 200) return fail("render: bad options");
     RenderParams P{};
@@ -1257,14 +1260,18 @@ bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_d
     }
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
     float *A = static_cast<float *>(acc.p);
-    const uint64_t total = (uint64_t)P.npix * o.spp, chunk = uint64_t(1) << 26;
+    // ParticleTracer: spp blocks of 100000 particles (ptracer.cpp:11-12); others: spp per pixel
+    const uint64_t total = integrator == 5 ? (uint64_t)100000 * o.spp : (uint64_t)P.npix * o.spp;
+    const uint64_t chunk = uint64_t(1) << (integrator == 5 ? 22 : 26);
     std::memset(kernel_ms, 0, sizeof(kernel_ms));
     std::memset(kernel_launches, 0, sizeof(kernel_launches));
     HIPCHK(hipEventRecord(ev[0], stream));
-    for (uint64_t s0 = 0; s0 < total; s0 += chunk)
-        HIPCHK(launch_primal_paths(S, P, kind, s0, std::min(chunk, total - s0), A, wk, stream));
+    for (uint64_t s0 = 0; s0 < total; s0 += chunk) {
+        if (integrator == 5) HIPCHK(launch_particles(S, P, s0, std::min(chunk, total - s0), A, wk, stream));
+        else HIPCHK(launch_primal_paths(S, P, kind, s0, std::min(chunk, total - s0), A, wk, stream));
+    }
     HIPCHK(hipEventRecord(ev[1], stream));
-    HIPCHK(launch_scale(A, dout, n3, 1.0f / (float)o.spp, stream));
+    HIPCHK(launch_scale(A, dout, n3, (float)(1.0 / (double)(integrator == 5 ? total : o.spp)), stream));
     if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, n3 * 4, hipMemcpyDeviceToHost, stream));
     unsigned long long hw[12];
     HIPCHK(hipMemcpyAsync(hw, wk, sizeof(hw), hipMemcpyDeviceToHost, stream));
@@ -1639,9 +1646,9 @@ int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {
 }
 
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
-    if (integrator < 0 || integrator > 4) {
+    if (integrator < 0 || integrator > 5) {
         g_err = "render: integrators are VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2), "
-                "PathTracer (3) and DirectIntegrator (4)";
+                "PathTracer (3), DirectIntegrator (4) and ParticleTracer (5)";
         return -2;
     }
     Scene *s = as<Scene>(scene, T_SCENE);

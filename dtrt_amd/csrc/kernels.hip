@@ -561,6 +561,152 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DS
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
 }
 
+// ------------------------------------------------------------ ParticleTracer
+// camera.cpp:129-165 — importance of world point p for the pinhole camera: pixel
+// position (px, py) within the output window, direction to the camera; 0 when p is
+// behind the near plane or outside the window.
+DI float cameraSampleDirect(const DCamera &c, f3 p, float &px, float &py, f3 &dir) {
+    const f3 q{c.w2c[0] * p.x + c.w2c[1] * p.y + c.w2c[2] * p.z + c.w2c[3],
+               c.w2c[4] * p.x + c.w2c[5] * p.y + c.w2c[6] * p.z + c.w2c[7],
+               c.w2c[8] * p.x + c.w2c[9] * p.y + c.w2c[10] * p.z + c.w2c[11]};
+    if (q.z < c.clip_near) return 0.f;
+    const float f = c.c2ndc[0];
+    float inv_area = 0.25f * f * f * c.aspect;
+    int xmin = 0, xmax = c.width, ymin = 0, ymax = c.height;
+    if (c.cropped) {
+        inv_area *= (float)c.width / (float)c.cw * (float)c.height / (float)c.ch;
+        xmin = c.ox; xmax = c.ox + c.cw;
+        ymin = c.oy; ymax = c.oy + c.ch;
+    }
+    const f3 n{c.c2ndc[0] * q.x + c.c2ndc[1] * q.y + c.c2ndc[2] * q.z, c.c2ndc[3] * q.x + c.c2ndc[4] * q.y + c.c2ndc[5] * q.z,
+               c.c2ndc[6] * q.x + c.c2ndc[7] * q.y + c.c2ndc[8] * q.z};
+    const float sx = (n.x / n.z * 0.5f + 0.5f) * c.width, sy = (-(n.y / n.z) * 0.5f * c.aspect + 0.5f) * c.height;
+    if (!(sx >= xmin && sx <= xmax && sy >= ymin && sy <= ymax)) return 0.f;
+    px = sx - xmin;
+    py = sy - ymin;
+    const float dist = length(q), inv = 1.0f / dist, invCos = dist / q.z;
+    dir = (ld3(c.cpos) - p) * inv;
+    return inv * inv * invCos * invCos * invCos * (inv_area * (float)((xmax - xmin) * (ymax - ymin)));
+}
+
+// Adds v to pixel (px, py) of a w x h window. A point exactly on the window's closing
+// edge (px == w or py == h, which camera.cpp:151-152 accepts) has no pixel and is dropped.
+DI void splat(float *acc, int w, int h, float px, float py, f3 v) {
+    const int ix = (int)px, iy = (int)py;
+    if (ix < 0 || iy < 0 || ix >= w || iy >= h) return;
+    float *a = acc + ((size_t)iy * w + ix) * 3;
+    if (v.x != 0.f) unsafeAtomicAdd(a, v.x);
+    if (v.y != 0.f) unsafeAtomicAdd(a + 1, v.y);
+    if (v.z != 0.f) unsafeAtomicAdd(a + 2, v.z);
+}
+
+// ptracer.cpp:57-167 — one particle: a point on a light (scene.cpp:795-814), its
+// connection to the camera, then a random walk that connects every medium and
+// non-null surface vertex to the camera. The emitter's geometric frame is the sampled
+// face's normal (the reference leaves that frame unset there).
+DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r, float *acc) {
+    const DScene &S = T.S;
+    if (S.nlights == 0) return;
+    const int W = cam.cropped ? cam.cw : cam.width, H = cam.cropped ? cam.ch : cam.height;
+    const f3 cpos = ld3(cam.cpos);
+    float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+    f3 lp, ln;
+    const int l = T.sampleLight(r4, lp, ln);
+    const DEmitter &E = S.emitters[l];
+    const DShape &lsh = S.shapes[E.shape_id];
+    f3 power = ld3(E.I) * (S.light_areas[l] / S.light_pmf[l]);
+    {   // ptracer.cpp:118-126
+        float px, py;
+        f3 dir;
+        float v = cameraSampleDirect(cam, lp, px, py, dir);
+        if (v != 0.f) {
+            const int m = dot(dir, ln) > 0.f ? lsh.med_ext : lsh.med_int;
+            v *= T.evalTransmittance(lp, dir, true, m, length(lp - cpos), r, max_bounces);
+            if (v != 0.f) splat(acc, W, H, px, py, power * (v * fmaxf(0.f, dot(ln, dir))));
+        }
+    }
+    const float u0 = r.next(), u1 = r.next();
+    f3 d = Frame(ln).toWorld(squareToCosineHemisphere(u0, u1));
+    power = power * PI;
+    f3 o = lp;
+    int med = dot(d, ln) > 0.f ? lsh.med_ext : lsh.med_int;
+    f3 thr{1, 1, 1};
+    bool onSurf = true;
+    Hit its;
+    for (int depth = 0; !allZero(thr, 0.f) && depth < max_bounces; ++depth) {
+        T.rayIntersect(o, d, onSurf, its);
+        const int max_inter = max_bounces - depth - 1;
+        bool inside = false;
+        if (med >= 0) {
+            const float a = r.next(), b = r.next();
+            f3 ps;
+            inside = sampleDistance(S.media[med], o, d, its.t, a, b, r, ps, thr);
+            if (inside) o = ps;
+        }
+        if (inside) {
+            const DPhase &ph = S.phases[S.media[med].phase_id];
+            float px, py;
+            f3 wo;
+            float v = cameraSampleDirect(cam, o, px, py, wo);   // ptracer.cpp:152-165
+            if (v != 0.f) {
+                v *= T.evalTransmittance(o, wo, false, med, length(o - cpos), r, max_inter);
+                if (v != 0.f) splat(acc, W, H, px, py, thr * power * (v * phaseEval(ph, -d, wo)));
+            }
+            const float a = r.next(), b = r.next();
+            d = phaseSample(ph, -d, a, b);   // weight eval/pdf = 1
+            onSurf = false;
+        } else {
+            if (its.face < 0) break;
+            const DShape &sh = shapeOf(S, its.face);
+            const DBsdf &b = bsdfOf(S, sh);
+            const Frame shf(its.sn);
+            if (b.type != BSDF_NULL) {   // ptracer.cpp:128-150
+                float px, py;
+                f3 wo;
+                float v = cameraSampleDirect(cam, its.p, px, py, wo);
+                if (v != 0.f) {
+                    v *= T.evalTransmittance(its.p, wo, true, targetMedium(sh, its.gn, wo), length(its.p - cpos), r, max_inter);
+                    if (v != 0.f) {
+                        const f3 wi = shf.toWorld(its.wi), wol = shf.toLocal(wo);
+                        const float wiG = dot(wi, its.gn), woG = dot(wo, its.gn);
+                        if (wiG * its.wi.z > 0.f && woG * wol.z > 0.f) {
+                            const float corr = fabsf((its.wi.z * woG) / (wol.z * wiG));
+                            splat(acc, W, H, px, py, bsdfEval(b, its.wi, wol, true) * thr * power * (v * corr));
+                        }
+                    }
+                }
+            }
+            const float q0 = r.next(), q1 = r.next(), q2 = r.next();
+            f3 wol;
+            float bpdf = 0.f;
+            const f3 bw = bsdfSample(b, its.wi, q0, q1, q2, wol, bpdf, true);
+            if (allZero(bw, 0.f)) break;
+            const f3 wo = shf.toWorld(wol);
+            const float wiG = dot(-d, its.gn), woG = dot(wo, its.gn);
+            if (wiG * its.wi.z <= 0.f || woG * wol.z <= 0.f) break;   // shading-normal light leaks [Veach p. 158]
+            thr = thr * bw;
+            if (isTransition(sh)) med = woG > 0.f ? sh.med_ext : sh.med_int;
+            o = its.p;
+            d = wo;
+            onSurf = true;
+        }
+    }
+}
+
+// ParticleTracer::render (ptracer.cpp:10-55): particles s0 .. s0+n-1, each its own
+// stream keyed (seed, particle index), splatted into acc[pixel][3] of view 0's window.
+__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_particles(DScene S, RenderParams P, uint64_t s0,
+                                                                           uint64_t n, float *acc,
+                                                                           unsigned long long *work) {
+    const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    Tracer<1> T(S);
+    Rng r;
+    r.init(S.views[0].seed, s0 + li);
+    if (li < n) traceParticle(T, S.views[0].cam, P.max_bounces, r, acc);
+    flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
 // out[i] = acc[i] * scale over n floats (the pixel mean of VolPathTracer)
 __global__ void k_scale(const float *acc, float *out, uint64_t n, float scale) {
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -1021,6 +1167,13 @@ hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind,
         hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc,
+                            unsigned long long *work, hipStream_t sm) {
+    const uint64_t nb = (n + BLOCK - 1) / BLOCK;
+    if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
     return hipGetLastError();
 }
 hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm) {

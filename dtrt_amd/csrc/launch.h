@@ -61,6 +61,9 @@ hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigne
 enum PrimalKind { PRIMAL_VOLPATH = 1, PRIMAL_SIMPLE = 2, PRIMAL_PATH = 3 };
 hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, float *acc,
                               unsigned long long *work, hipStream_t sm);
+// ParticleTracer: particles s0 .. s0+n-1 splatted into acc[pixel][3] of view 0's window.
+hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc,
+                            unsigned long long *work, hipStream_t sm);
 // out[i] = acc[i] * scale, i < n.
 hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm);
 hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);

@@ -27,6 +27,8 @@ struct DCamera {
     float fov, aspect, clip_near, pad;
     float cpos[3], cs[3], ct[3], cn[3];
     float cpos_d[DTRT_NDER_MAX][3], cs_d[DTRT_NDER_MAX][3], ct_d[DTRT_NDER_MAX][3], cn_d[DTRT_NDER_MAX][3];
+    float w2c[12];          // rows of the 3x4 world-to-camera transform (light tracing)
+    float c2ndc[9];
 };
 
 struct DShape {

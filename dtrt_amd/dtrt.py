@@ -44,6 +44,7 @@ VolPathTracer = _m.VolPathTracer
 VolPathTracerSimple = _m.VolPathTracerSimple
 PathTracer = _m.PathTracer
 DirectIntegrator = _m.DirectIntegrator
+ParticleTracer = _m.ParticleTracer
 angleEps = _m.angleEps
 abi_version = _m.abi_version
 set_nder = _m.set_nder

@@ -321,8 +321,8 @@ def load_mitsuba(path, nder=1):
 
 def integrator_for(mod, name):
     """Renderer object for an integrator type name of the XML (``volpathAD``,
-    ``volpath``, ``volpath_simple``, ``path``, ``direct``); ``pathAD``, ``directAD`` and
-    ``ptracer`` are not provided."""
+    ``volpath``, ``volpath_simple``, ``path``, ``direct``, ``ptracer``); ``pathAD`` and
+    ``directAD`` are not provided."""
     if name == "volpathAD":
         return mod.VolPathTracerAD()
     if name == "volpath":
@@ -333,4 +333,6 @@ def integrator_for(mod, name):
         return mod.PathTracer()
     if name == "direct":
         return mod.DirectIntegrator()
+    if name == "ptracer":
+        return mod.ParticleTracer()
     raise NotImplementedError("integrator %s is not provided" % name)

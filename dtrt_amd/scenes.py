@@ -378,13 +378,14 @@ def render(mod, d: SceneDesc, seed=0, spp=None, mode=-1, sppe=None, sppse=None, 
 def render_primal(mod, d: SceneDesc, seed=0, spp=None, max_bounces=None, scene=None, simple=False, kind=None):
     """Primal render of ``d``; returns float32[H, W, 3]. ``kind``: 1 VolPathTracer
     (default), 2 VolPathTracerSimple (also ``simple=True``), 3 PathTracer,
-    4 DirectIntegrator."""
+    4 DirectIntegrator, 5 ParticleTracer (``spp`` blocks of 100000 particles)."""
     scene = scene if scene is not None else instantiate(mod, d)
     opts = mod.RenderOptions(seed, d.spp if spp is None else spp, d.max_bounces if max_bounces is None else max_bounces,
                              0, 0, True)
     out = np.zeros(d.out_shape()[1:], dtype=np.float32)
     kind = kind if kind is not None else (2 if simple else 1)
-    cls = {1: mod.VolPathTracer, 2: mod.VolPathTracerSimple, 3: mod.PathTracer, 4: mod.DirectIntegrator}[kind]
+    cls = {1: mod.VolPathTracer, 2: mod.VolPathTracerSimple, 3: mod.PathTracer, 4: mod.DirectIntegrator,
+           5: mod.ParticleTracer}[kind]
     cls().render(scene, opts, out)
     return out
 

@@ -422,15 +422,20 @@ void oracle_counters(long long *out6) {
 
 // integrator kind 0: VolPathTracerAD (integratorAD.cpp:19-166); kinds 1-4, float32[H][W][3]:
 // VolPathTracer (volpath.cpp:106-141), VolPathTracerSimple (volpath_simple.cpp:89-124),
-// PathTracer (path.cpp:64-96), DirectIntegrator (direct.cpp:51-85)
+// PathTracer (path.cpp:64-96), DirectIntegrator (direct.cpp:51-85); kind 5: ParticleTracer
+// (ptracer.cpp:10-55)
 int oracle_render(int integrator, void *scene, const oracle_render_options *o, float *out) {
-    if (integrator < 0 || integrator > 4) {
+    if (integrator < 0 || integrator > 5) {
         g_err = "integrators: VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2), PathTracer (3), "
-                "DirectIntegrator (4)";
+                "DirectIntegrator (4), ParticleTracer (5)";
         return -2;
     }
     OScene &s = static_cast<HScene *>(scene)->s;
     g_counters.reset();
+    if (integrator == 5) {
+        Tracer<1>(s).renderParticles(o->seed, o->spp, o->max_bounces, out);
+        return 0;
+    }
     if (integrator > 0) {
         Tracer<1>(s).renderPrimal(o->seed, o->spp, o->max_bounces, out, integrator);
         return 0;

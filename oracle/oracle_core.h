@@ -854,7 +854,8 @@ template <int N> static DS<N> specAD(const Spec &v, const std::vector<Spec> &d) 
 static V3 phongReflect(const V3 &wi) { return V3(-wi.x, -wi.y, wi.z); }
 
 // diffuse.cpp:7-38, null.cpp:5-29, phong.cpp:19-95, roughdielectric.cpp:9-206
-static Spec bsdfEval(const OBsdf &b, const Its &its, const V3 &wo) {
+// importance: adjoint transport (ptracer.cpp), roughdielectric.cpp:50,156
+static Spec bsdfEval(const OBsdf &b, const Its &its, const V3 &wo, bool importance = false) {
     const V3 &wi = its.wi;
     switch (b.type) {
         case 0:
@@ -886,7 +887,7 @@ static Spec bsdfEval(const OBsdf &b, const Its &its, const V3 &wo) {
             double eta = wi.z > 0.0 ? b.eta : invEta;
             double sqrtDenom = dot(wi, H) + eta * dot(wo, H);
             double value = ((1.0 - F) * Dv * G * eta * eta * dot(wi, H) * dot(wo, H)) / (wi.z * sqrtDenom * sqrtDenom);
-            double factor = wi.z > 0 ? invEta : b.eta;
+            double factor = importance ? 1.0 : (wi.z > 0 ? invEta : b.eta);
             return Spec::ones() * std::abs(value * factor * factor);
         }
     }
@@ -930,7 +931,8 @@ static double bsdfPdf(const OBsdf &b, const Its &its, const V3 &wo) {
     }
 }
 
-static Spec bsdfSample(const OBsdf &b, const Its &its, const double *rnd, V3 &wo, double &pdf, double &eta) {
+static Spec bsdfSample(const OBsdf &b, const Its &its, const double *rnd, V3 &wo, double &pdf, double &eta,
+                       bool importance = false) {
     const V3 &wi = its.wi;
     switch (b.type) {
         case 0:
@@ -982,7 +984,7 @@ static Spec bsdfSample(const OBsdf &b, const Its &its, const double *rnd, V3 &wo
                 wo = refract(wi, m, b.eta, cosThetaT);
                 eta = cosThetaT < 0 ? b.eta : invEta;
                 if (wi.z * wo.z >= 0) return Spec();
-                double factor = cosThetaT < 0 ? invEta : b.eta;
+                double factor = importance ? 1.0 : (cosThetaT < 0 ? invEta : b.eta);
                 weight *= factor * factor;
                 double sd = dot(wi, m) + eta * dot(wo, m);
                 dwh = (eta * eta * dot(wo, m)) / (sd * sd);
@@ -1516,6 +1518,126 @@ template <int N> struct Tracer {
             }
         }
         return ret;
+    }
+
+    // ptracer.cpp:57-167 — ParticleTracer::traceParticle into img[pixel] (window size
+    // w x h). The emitter's geometric frame is the sampled face normal (the reference
+    // leaves it unset); a splat on the window's closing edge has no pixel and is dropped.
+    void traceParticle(Pcg &s, int max_bounces, std::vector<Spec> &img) const {
+        const OCamera &cam = *sc.cam;
+        const int W = cam.cropped() ? cam.cw : cam.width, H = cam.cropped() ? cam.ch : cam.height;
+        auto splat = [&](double px, double py, const Spec &v) {
+            int ix = (int)px, iy = (int)py;
+            if (ix >= 0 && iy >= 0 && ix < W && iy < H) img[(size_t)iy * W + ix] += v;
+        };
+        if (sc.emitters.empty()) return;
+        double rnd4[4] = {s.next1D(), s.next1D(), s.next1D(), s.next1D()};
+        int l, k;
+        V3 lp, ln;
+        sampleLight(rnd4, l, k, lp, ln);   // scene.cpp:795-814
+        const OEmitter &E = *sc.emitters[l];
+        const OShape &lsh = shape(E.shape_id);
+        Spec power = E.I * (sc.light_areas[l] / sc.light_pmf[l]);
+        {
+            double px, py;
+            V3 dir;
+            double v = cam.sampleDirect(lp, px, py, dir);
+            if (v != 0.0) {
+                int m = dot(dir, ln) > 0 ? lsh.med_ext : lsh.med_int;
+                v *= evalTransmittance(lp, dir, true, m, norm(lp - cam.cpos), s, max_bounces);
+                if (v != 0.0) splat(px, py, power * (v * std::max(0.0, dot(ln, dir))));
+            }
+        }
+        double u0 = s.next1D(), u1 = s.next1D();
+        V3 d = Frame(ln).toWorld(squareToCosineHemisphere(u0, u1));
+        power = power * PI;
+        V3 o = lp;
+        int medium = dot(d, ln) > 0 ? lsh.med_ext : lsh.med_int;
+        Spec thr = Spec::ones();
+        bool onSurf = true;
+        Its its;
+        for (int depth = 0; !thr.isZero() && depth < max_bounces; ++depth) {
+            rayIntersect(o, d, onSurf, its);
+            const int max_inter = max_bounces - depth - 1;
+            bool inside = false;
+            if (medium >= 0) {
+                double rnd1[2] = {s.next1D(), s.next1D()};
+                V3 ps;
+                inside = sampleDistance(*sc.media[medium], o, d, its.t, rnd1, s, ps, thr);
+                if (inside) o = ps;
+            }
+            if (inside) {
+                const OPhase &ph = *sc.phases[sc.media[medium]->phase_id];
+                double px, py;
+                V3 wo;
+                double v = cam.sampleDirect(o, px, py, wo);
+                if (v != 0.0) {
+                    v *= evalTransmittance(o, wo, false, medium, norm(o - cam.cpos), s, max_inter);
+                    if (v != 0.0) splat(px, py, thr * power * (v * phaseEval(ph, -d, wo)));
+                }
+                double a = s.next1D(), b = s.next1D();
+                V3 wn;
+                double pv = phaseSample(ph, -d, a, b, wn);
+                if (pv == 0.0) break;
+                thr *= pv;
+                d = wn;
+                onSurf = false;
+            } else {
+                if (its.shape < 0) break;
+                const OBsdf &b = bsdfOf(its.shape);
+                const OShape &sh = shape(its.shape);
+                if (!b.isNull()) {
+                    double px, py;
+                    V3 wo;
+                    double v = cam.sampleDirect(its.p, px, py, wo);
+                    if (v != 0.0) {
+                        int m = dot(wo, its.geo.n) > 0 ? sh.med_ext : sh.med_int;
+                        v *= evalTransmittance(its.p, wo, true, m, norm(its.p - cam.cpos), s, max_inter);
+                        if (v != 0.0) {
+                            V3 wi = its.sh.toWorld(its.wi), wol = its.sh.toLocal(wo);
+                            double wiG = dot(wi, its.geo.n), woG = dot(wo, its.geo.n);
+                            if (wiG * its.wi.z > 0 && woG * wol.z > 0) {
+                                double corr = std::abs((its.wi.z * woG) / (wol.z * wiG));
+                                splat(px, py, bsdfEval(b, its, wol, true) * thr * power * (v * corr));
+                            }
+                        }
+                    }
+                }
+                double rnd3[3] = {s.next1D(), s.next1D(), s.next1D()};
+                V3 wol;
+                double bpdf, beta;
+                Spec bw = bsdfSample(b, its, rnd3, wol, bpdf, beta, true);
+                if (bw.isZero()) break;
+                V3 wo = its.sh.toWorld(wol);
+                double wiG = dot(-d, its.geo.n), woG = dot(wo, its.geo.n);
+                if (wiG * its.wi.z <= 0 || woG * wol.z <= 0) break;
+                thr *= bw;
+                if (isTransition(its.shape)) medium = woG > 0 ? sh.med_ext : sh.med_int;
+                o = its.p;
+                d = wo;
+                onSurf = true;
+            }
+        }
+    }
+    // ptracer.cpp:10-55 — 100000 * spp particles, one stream each, image / count
+    void renderParticles(uint64_t seed, int spp, int max_bounces, float *out) const {
+        const OCamera &cam = *sc.cam;
+        const int npix = cam.cropped() ? cam.cw * cam.ch : cam.width * cam.height;
+        const int64_t total = (int64_t)100000 * spp;
+        std::vector<Spec> img(npix);
+#pragma omp parallel
+        {
+            std::vector<Spec> loc(npix);
+#pragma omp for schedule(dynamic, 1024)
+            for (int64_t i = 0; i < total; ++i) {
+                Pcg s(seed, (uint64_t)i);
+                traceParticle(s, max_bounces, loc);
+            }
+#pragma omp critical
+            for (int p = 0; p < npix; ++p) img[p] += loc[p];
+        }
+        for (int p = 0; p < npix; ++p)
+            for (int c = 0; c < 3; ++c) out[(size_t)p * 3 + c] = (float)(img[p][c] / (double)total);
     }
 
     // volpathAD.cpp:145-620 — forward-mode differential estimator

@@ -133,6 +133,28 @@ struct OCamera {
         for (int i = 0; i < 3; ++i) d[i] = ndc2c[i][0] * ndc.x + ndc2c[i][1] * ndc.y + ndc2c[i][2] * ndc.z;
         return normalized(d);
     }
+    // camera.cpp:129-165 — importance of world point p; pixel position within the window
+    double sampleDirect(const V3 &p, double &px, double &py, V3 &dir) const {
+        V3 q;
+        for (int i = 0; i < 3; ++i) q[i] = w2c[i][0] * p.x + w2c[i][1] * p.y + w2c[i][2] * p.z + w2c[i][3];
+        if (q.z < clip_near) return 0.0;
+        double f = c2ndc[0][0], aspect = double(width) / double(height);
+        double inv_area = 0.25 * f * f * aspect;
+        int xmin = 0, xmax = width, ymin = 0, ymax = height;
+        if (cropped()) {
+            inv_area *= double(width) / cw * double(height) / ch;
+            xmin = ox; xmax = ox + cw; ymin = oy; ymax = oy + ch;
+        }
+        V3 n;
+        for (int i = 0; i < 3; ++i) n[i] = c2ndc[i][0] * q.x + c2ndc[i][1] * q.y + c2ndc[i][2] * q.z;
+        double sx = (n.x / n.z * 0.5 + 0.5) * width, sy = (-(n.y / n.z) * 0.5 * aspect + 0.5) * height;
+        if (!(sx >= xmin && sx <= xmax && sy >= ymin && sy <= ymax)) return 0.0;
+        px = sx - xmin;
+        py = sy - ymin;
+        double dist = norm(q), inv = 1.0 / dist, invCos = dist / q.z;
+        dir = (cpos - p) * inv;
+        return inv * inv * invCos * invCos * invCos * inv_area * double(xmax - xmin) * double(ymax - ymin);
+    }
     void samplePrimaryRay(double x, double y, V3 &org, V3 &dir) const {  // camera.cpp:52-60
         V3 d = camDir(x, y);
         double w = c2w[3][3];

@@ -100,3 +100,27 @@ def test_surface_integrators_parity(gpu, oracle, cfg, kind):
     a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, kind=kind)[None] for s in range(k)])
     b = np.stack([scenes.render_primal(oracle, d, seed=100 + s, scene=so, kind=kind)[None] for s in range(k)])
     assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 3), what="kind %d %s" % (kind, cfg))
+
+
+def test_particle_tracer_closed_forms(gpu):
+    d, lv, rho, Le = F.direct_lighting(res=32, spp=64)
+    img = scenes.render_primal(gpu, d, seed=1, kind=5, spp=100)[..., 0]
+    ref = F.direct_expected(d, lv, rho, Le)
+    assert abs(img.mean() / ref.mean() - 1) < 3e-3
+    assert np.abs(img - ref).max() < 0.1 * ref.max()
+    d = F.diffuse_furnace(albedo=0.5, Le=2.0, res=32, spp=32, depth=5)
+    v = scenes.render_primal(gpu, d, seed=2, kind=5, spp=20)[..., 0]
+    assert abs(v.mean() / (2.0 * sum(0.5 ** k for k in range(6))) - 1) < 3e-3
+
+
+@pytest.mark.parametrize("cfg", ["cfg0", "cfg1"])
+def test_particle_tracer_parity(gpu, oracle, cfg):
+    """ParticleTracer against the oracle's restatement of ptracer.cpp (cfg1: media)."""
+    if cfg == "cfg0":
+        d = scenes.config0(nder=1, res=24, spp=32)
+    else:
+        d = scenes.config1(nder=1, res=24, spp=32, grid_res=16)
+    sg, so = scenes.instantiate(gpu, d), scenes.instantiate(oracle, d)
+    a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, kind=5, spp=4)[None] for s in range(16)])
+    b = np.stack([scenes.render_primal(oracle, d, seed=100 + s, scene=so, kind=5, spp=4)[None] for s in range(16)])
+    assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 3), what="ParticleTracer " + cfg)

@@ -147,4 +147,4 @@ def test_example_scene_renders_through_the_oracle(oracle):
     assert np.isfinite(img).all() and img[0].mean() > 0
     assert loaders.integrator_for(oracle, "volpath").kind == 1
     with pytest.raises(NotImplementedError):
-        loaders.integrator_for(oracle, "ptracer")
+        loaders.integrator_for(oracle, "pathAD")

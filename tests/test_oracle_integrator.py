@@ -132,3 +132,16 @@ def test_surface_integrators_direct_lighting_matches_lambert_polygon(oracle, kin
     img = scenes.render_primal(oracle, d, seed=1, kind=kind)[..., 0]
     ref = F.direct_expected(d, lv, rho, Le)
     assert abs(img.mean() / ref.mean() - 1) < 5e-3
+
+
+def test_particle_tracer_closed_forms(oracle):
+    """ParticleTracer (ptracer.cpp): light paths splatted to the camera reproduce the
+    Lambert polygon (one bounce) and the emitting diffuse sphere (L = Le sum a^k)."""
+    d, lv, rho, Le = F.direct_lighting(res=16, spp=64)
+    img = scenes.render_primal(oracle, d, seed=1, kind=5, spp=20)[..., 0]
+    ref = F.direct_expected(d, lv, rho, Le)
+    assert abs(img.mean() / ref.mean() - 1) < 5e-3
+    assert np.abs(img - ref).max() < 0.1 * ref.max()
+    d = F.diffuse_furnace(albedo=0.5, Le=2.0, res=12, spp=32, depth=5)
+    v = scenes.render_primal(oracle, d, seed=2, kind=5, spp=10)[..., 0]
+    assert abs(v.mean() / (2.0 * sum(0.5 ** k for k in range(6))) - 1) < 3e-3
