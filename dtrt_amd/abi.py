@@ -299,6 +299,14 @@ def bind(lib_path, prefix, name="dtrt"):
         """ptracer.cpp:10-167: 100000 * spp light particles splatted to the camera; float32[H][W][3]."""
         kind = 5
 
+    class PathTracerAD(VolPathTracerAD):
+        """pathAD.cpp: surface-only forward-mode path tracer; same layout as VolPathTracerAD."""
+        kind = 6
+
+    class DirectAD(VolPathTracerAD):
+        """directAD.cpp: surface-only forward-mode direct illumination; same layout as VolPathTracerAD."""
+        kind = 7
+
     class _Module:
         pass
 
@@ -311,7 +319,8 @@ def bind(lib_path, prefix, name="dtrt"):
                      Scene=Scene, RenderOptions=RenderOptions, Integrator=Integrator,
                      VolPathTracerAD=VolPathTracerAD, VolPathTracer=VolPathTracer,
                      VolPathTracerSimple=VolPathTracerSimple, PathTracer=PathTracer,
-                     DirectIntegrator=DirectIntegrator, ParticleTracer=ParticleTracer).items():
+                     DirectIntegrator=DirectIntegrator, ParticleTracer=ParticleTracer, PathTracerAD=PathTracerAD,
+                     DirectAD=DirectAD).items():
         setattr(m, k, v)
     m.lib = lib
     m.fns = fns

@@ -789,6 +789,17 @@ template <int N> DI Ds<N> emitterEvalAD(const DEmitter &e, const HitAD<N> &h, co
     return Ds<N>();
 }
 
+// area.cpp:17-19, area2.cpp:29-35 — evaluation against an explicit (AD) normal
+template <int N> DI Ds<N> emitterEvalNormAD(const DEmitter &e, const Dv<N> &n, const Dv<N> &d) {
+    if (e.type == EMIT_AREA) return (e.two_sided || dot(n.val(), d.val()) > 0.f) ? Ds<N>(ld3(e.I)) : Ds<N>();
+    Df<N> tmp = ddot(n, d);
+    if (tmp.v > EPS) {
+        Df<N> k = scalarAD<N>(e.kappa, e.kappa_d);
+        return specAD<N>(e.I, e.I_d) * (dexp(k * (tmp - 1.0f)) * k * INV_TWOPI);
+    }
+    return Ds<N>();
+}
+
 // ---------------------------------------------------- distributions
 // pmf.h:103-167 — lower_bound on an unnormalised-then-normalised CDF, sample reuse
 DI int sampleReuse(const double *cdf, int n, float &u, float *pdf) {

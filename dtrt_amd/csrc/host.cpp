@@ -580,12 +580,13 @@ struct Scene : Obj {
     }
     bool build();
     bool initEdges(const float *weights);
-    bool render(const dtrt_render_options &o, float *out, bool out_is_device) {
+    bool render(const dtrt_render_options &o, float *out, bool out_is_device, int ad_kind = AD_VOLPATH) {
         const Camera *c = &cam;
         float *outs[1] = {out};
-        return renderViews(1, &c, &o, outs, out_is_device);
+        return renderViews(1, &c, &o, outs, out_is_device, ad_kind);
     }
-    bool renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs, bool on_device);
+    bool renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs, bool on_device,
+                     int ad_kind = AD_VOLPATH);
     bool primaryEdges(const Camera &c, PECache &out);
     bool captureViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *images);
     int replayViews(const uint64_t *seeds);
@@ -872,7 +873,7 @@ constexpr uint64_t FIRST_CHUNK = uint64_t(1) << 16;  // before the children-per-
 // pixels occupy their own range of the accumulators. Every view's image equals what
 // a render of that view alone gives (same streams), up to float summation order.
 bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs,
-                        bool out_is_device) {
+                        bool out_is_device, int ad_kind) {
     if (nv <= 0) return fail("render: no views");
     const dtrt_render_options &o0 = opts[0];
     for (int v = 1; v < nv; ++v)
@@ -880,6 +881,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
             opts[v].ddist_coeff != o0.ddist_coeff)
             return fail("render: views of one call must share max_bounces, mode, sppse and ddistCoeff");
     RenderParams P{};
+    P.ad_kind = ad_kind;
     P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
@@ -1646,16 +1648,17 @@ int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {
 }
 
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
-    if (integrator < 0 || integrator > 5) {
+    if (integrator < 0 || integrator > 7) {
         g_err = "render: integrators are VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2), "
-                "PathTracer (3), DirectIntegrator (4) and ParticleTracer (5)";
+                "PathTracer (3), DirectIntegrator (4), ParticleTracer (5), PathTracerAD (6) and DirectAD (7)";
         return -2;
     }
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s || !o || !img) { if (s) g_err = "render: null argument"; return -1; }
     try {
-        if (integrator > 0) return s->renderPrimal(*o, img, dev, integrator) ? 0 : -1;
-        return s->render(*o, img, dev) ? 0 : -1;
+        if (integrator >= 1 && integrator <= 5) return s->renderPrimal(*o, img, dev, integrator) ? 0 : -1;
+        const int ad = integrator == 6 ? AD_PATH : integrator == 7 ? AD_DIRECT : AD_VOLPATH;
+        return s->render(*o, img, dev, ad) ? 0 : -1;
     } catch (const std::exception &e) {
         g_err = e.what();
         return -1;

@@ -227,6 +227,206 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
     }
 }
 
+// Surface-only forward-mode integrators: PathTracerAD (pathAD.cpp:64-214, DIRECT false)
+// and DirectIntegratorAD (directAD.cpp:5-98, DIRECT true), for one AD sample of pixel
+// `pixel`. Media are ignored. pathAD's edge terms push their two primal sub-paths as
+// CHILD_PATH records (PathTracerAD::Li = path.cpp:11-61); directAD's need only the
+// emission seen along two rays and are evaluated here.
+template <int N, bool DIRECT>
+DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, Ds<N> &L, Pusher &u,
+                        unsigned *stats) {
+    const DScene &S = T.S;
+    const DCamera &cam = vw.cam;
+    const int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
+    const int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
+    float jx = 0.5f, jy = 0.5f;
+    if (vw.sppe >= 0) { jx = r.next(); jy = r.next(); }
+    Dv<N> ro, rd;
+    cameraRayAD<N>(cam, ix + jx, iy + jy, ro, rd);
+    HitAD<N> its;
+    // pathAD.cpp:222-227 traces the camera ray from 0, directAD.cpp:11 from ShadowEpsilon
+    if (!T.rayIntersectAD(ro, rd, DIRECT, its)) return;
+    const int mode = DIRECT ? 0x3F : P.mode;
+    int depth = DIRECT ? 1 : P.max_bounces;
+    bool incEmission = true;
+    Ds<N> beta(f3{1, 1, 1});
+    while (true) {
+        const DShape &sh = shapeOf(S, its.face);
+        if (sh.light_id >= 0 && (mode & 1) && incEmission)
+            L += beta * emitterEvalAD<N>(S.emitters[sh.light_id], its, -rd);
+        if (depth == 0) break;
+        const DBsdf &b = bsdfOf(S, sh);
+        const bool isnull = b.type == BSDF_NULL;
+        const Hit its0 = its.val();
+        FrameAD<N> shf(its.sn);
+        if (mode & 2) {   // light sampling
+            float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+            Dv<N> woL;
+            float pdf_nee;
+            Ds<N> value = T.neeDirectAD(its, shf, r4, woL, pdf_nee);
+            if (!value.isZero(EPS)) {
+                Ds<N> bv = bsdfEvalAD<N>(b, its.wi, woL);
+                const float bp = bsdfPdf(b, its0.wi, woL.val());
+                const float mis = DIRECT ? pdf_nee / (pdf_nee + bp) : misPower(pdf_nee, bp);
+                L += beta * (value * bv * mis);
+            }
+        }
+        // BSDF sampling (bsdf.cpp:11-21; null.cpp:20-25)
+        float r3[3] = {r.next(), r.next(), r.next()};
+        bool cont = false;
+        Dv<N> nd;
+        Ds<N> bw;
+        float bpdf = 1.f;
+        if (isnull) {
+            nd = rd;
+            bw = Ds<N>(f3{1, 1, 1});
+            cont = true;
+        } else {
+            f3 wo0;
+            f3 w0 = bsdfSample(b, its0.wi, r3[0], r3[1], r3[2], wo0, bpdf);
+            if (!allZero(w0, EPS) && bpdf > EPS) {
+                const f3 ww = shf.val().toWorld(wo0);
+                bw = bsdfEvalAD<N>(b, its.wi, shf.toLocal(Dv<N>(ww))) * (1.0f / bpdf);
+                if (!bw.isZero(EPS)) { nd = Dv<N>(ww); cont = true; }
+            }
+        }
+        HitAD<N> its1;
+        if (cont) {
+            cont = T.rayIntersectAD(its.p, nd, true, its1);
+            if (cont) {
+                const DShape &sh1 = shapeOf(S, its1.face);
+                if (sh1.light_id >= 0 && (mode & 2)) {
+                    Ds<N> lc = emitterEvalAD<N>(S.emitters[sh1.light_id], its1, -nd);
+                    if (!allZero(lc.val(), 0.f)) {
+                        const f3 dd = its1.p.val() - its0.p;
+                        const float G = its1.wi.z.v / dot(dd, dd);
+                        const float pn = S.light_pmf[sh1.light_id] / (S.light_areas[sh1.light_id] * G);
+                        const float mis = DIRECT ? bpdf / (pn + bpdf) : misPower(bpdf, pn);
+                        L += beta * (bw * lc * mis);
+                    }
+                }
+            }
+        }
+        // edge sampling at this vertex (pathAD.cpp:112-210; directAD.cpp:55-95)
+        if (P.sppse > 0 && (mode & 4) && S.edges_valid) {
+            const int ne = P.sppse;
+            const Dv<N> &x1 = its.p;
+            float der[N * 3];
+#pragma unroll
+            for (int k = 0; k < N * 3; ++k) der[k] = 0.f;
+            for (int i = 0; i < ne; ++i) {
+                float rnd = r.next(), pdfs, pdfe;
+                const int sid = sampleReuse(S.shape_edge_cdf, S.nshapes, rnd, &pdfs);
+                const DShape &es = S.shapes[sid];
+                if (es.ne <= 0) continue;
+                const int k = sampleReuse(S.edge_cdf + es.ebase + sid, es.ne, rnd, &pdfe);
+                const float t = rnd;
+                float edgePdf = pdfs * pdfe;
+                const int4 e = S.edges[es.ebase + k];
+                if (isSilhouette(S, e, x1.val(), es.has_normals) <= 0) continue;
+                const Dv<N> v0 = ldDv<N>(S.V + 3 * (size_t)e.x, S.Vd + (size_t)e.x * N * 3);
+                const Dv<N> v1 = ldDv<N>(S.V + 3 * (size_t)e.y, S.Vd + (size_t)e.y * N * 3);
+                Dv<N> w;
+                f3 nrm;
+                float dist, geo = 1.f;
+                if constexpr (DIRECT) {   // linear in t, pdf per unit length (scene.h:103-108)
+                    edgePdf /= length(v1.val() - v0.val());
+                    Dv<N> y1 = v0 * (1.0f - t) + v1 * t;
+                    w = y1 - x1;
+                    nrm = normalize(cross(v0.val() - x1.val(), v1.val() - x1.val()));
+                    Df<N> dAD = dnorm(w);
+                    w = w / dAD;
+                    dist = dAD.v;
+                    const float ct = -dot(w.val(), normalize(v1.val() - v0.val()));
+                    geo = sqrtf(fmaxf(0.f, 1.0f - ct * ct)) / dist;
+                } else {   // angle-uniform point on the edge (pathAD.cpp:136-164)
+                    const Dv<N> e0 = v0 - x1, e1 = v1 - x1;
+                    const f3 c01 = cross(e0.val(), e1.val());
+                    nrm = normalize(c01);
+                    const float l0 = length(e0.val()), l1 = length(e1.val());
+                    const float theta = atan2f(length(c01) / (l0 * l1), dot(e0.val(), e1.val()) / (l0 * l1));
+                    if (theta < EPS || theta > PI - EPS) continue;
+                    const float sinT = sinf(theta);
+                    float w0 = sinf(theta * (1.0f - t)) / sinT / l0, w1 = sinf(theta * t) / sinT / l1;
+                    const float ws = w0 + w1;
+                    w0 /= ws;
+                    w1 /= ws;
+                    if (w0 < -EPS || w1 < -EPS || fabsf(w0 + w1 - 1.0f) > 1e-5f) { atomicAdd(stats + STAT_BAD_EDGE_WEIGHT, 1u); continue; }
+                    w = e0 * w0 + e1 * w1;
+                    Df<N> dAD = dnorm(w);
+                    w = w / dAD;
+                    dist = dAD.v;
+                    edgePdf /= theta;
+                    float orth = 0.f, gmax = 0.f;
+#pragma unroll
+                    for (int j = 0; j < N; ++j) { orth = fmaxf(orth, fabsf(dot(w.grad(j), w.val()))); gmax = fmaxf(gmax, length(w.grad(j))); }
+                    if (orth > 1e-4f * (1.0f + gmax)) { atomicAdd(stats + STAT_NONORTHO, 1u); continue; }
+                    if (w.gradZero(EPS)) continue;
+                }
+                const f3 wv = w.val();
+                const f3 fv = bsdfEval(b, its0.wi, shf.val().toLocal(wv));
+                if (allZero(fv, EPS)) continue;
+                float th;   // scene.cpp:150-176: nothing between x1 and the edge point
+                if (T.trace(x1.val(), wv, SHADOW_EPS, (1.0f - SHADOW_EPS) * dist, th) >= 0) continue;
+                const f3 base = beta.val() * fv * (geo / (edgePdf * ne));
+                if constexpr (DIRECT) {
+                    f3 delta{0, 0, 0};
+                    for (int sgn = 0; sgn < 2; ++sgn) {
+                        const f3 w2 = normalize(sgn == 0 ? wv - nrm * ANGLE_EPS : wv + nrm * ANGLE_EPS);
+                        Hit h1;
+                        if (T.rayIntersect(x1.val(), w2, true, h1)) {
+                            const DShape &hs = shapeOf(S, h1.face);
+                            if (hs.light_id >= 0) {
+                                const f3 le = emitterEval(S.emitters[hs.light_id], h1.gn, h1.sn, -wv);
+                                delta = sgn == 0 ? delta + le : delta - le;
+                            }
+                        }
+                    }
+                    if (allZero(delta, EPS)) continue;
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        const float nd_ = dot(nrm, w.grad(j));
+                        der[3 * j] += nd_ * base.x * delta.x;
+                        der[3 * j + 1] += nd_ * base.y * delta.y;
+                        der[3 * j + 2] += nd_ * base.z * delta.z;
+                    }
+                } else {
+                    float W[N * 3];
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        const float nd_ = dot(nrm, w.grad(j));
+                        W[3 * j] = base.x * nd_; W[3 * j + 1] = base.y * nd_; W[3 * j + 2] = base.z * nd_;
+                    }
+                    ChildHdr h;
+                    const f3 org = x1.val();
+                    h.org[0] = org.x; h.org[1] = org.y; h.org[2] = org.z;
+                    h.meta = CHILD_PATH | ((depth - 1) << 4);
+                    h.face = -1;
+                    for (int sgn = 0; sgn < 2; ++sgn) {
+                        const f3 w2 = normalize(sgn == 0 ? wv - nrm * ANGLE_EPS : wv + nrm * ANGLE_EPS);
+                        h.dir[0] = w2.x; h.dir[1] = w2.y; h.dir[2] = w2.z;
+                        if (sgn == 1)
+#pragma unroll
+                            for (int k2 = 0; k2 < N * 3; ++k2) W[k2] = -W[k2];
+                        push(P, u, h, W, N * 3);
+                    }
+                }
+            }
+            if constexpr (DIRECT) {
+#pragma unroll
+                for (int j = 0; j < N; ++j)
+                    for (int c = 0; c < 3; ++c) L.c[c].d[j] += der[3 * j + c];
+            }
+        }
+        if (DIRECT || !cont) break;
+        beta = beta * bw;
+        rd = nd;
+        its = its1;
+        depth -= 1;
+        incEmission = false;
+    }
+}
+
 // Passes of the path-replay adjoint (k_adjoint); NONE in the forward kernel.
 enum class AdjPass { NONE, TOTAL, SCATTER };
 struct AdjCtx {
@@ -447,7 +647,7 @@ DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &
 // Normal pass: samples s0 .. s0+nsamp-1. Replay pass (replay != nullptr): entries
 // {sid lo, sid hi, children delivered, 0} of samples whose pushes overflowed.
 // Samples that overflow (again) are appended to ovf.
-template <int N>
+template <int N, int I>   // I: AdKind
 __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
                                                     const uint4 *replay, float *acc, char *q, unsigned *qcount,
                                                     unsigned *stats, uint4 *ovf, unsigned *ovf_count,
@@ -471,7 +671,10 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
     Pusher u{q, qcount, gpix, v, (ls + 1) << 20, 0, skip, Pusher::NO_FAIL};
     Rng r;
     r.init(vw.seed, ls);
-    if (active) adPathSample<N, AdjPass::NONE>(T, P, vw, r, pixel, L, u, stats, nullptr);
+    if (active) {
+        if constexpr (I == AD_VOLPATH) adPathSample<N, AdjPass::NONE>(T, P, vw, r, pixel, L, u, stats, nullptr);
+        else adSurfaceSample<N, I == AD_DIRECT>(T, P, vw, r, pixel, L, u, stats);
+    }
     const bool done = u.failed_at == Pusher::NO_FAIL;
     if (active && !done) ovf[atomicAdd(ovf_count, 1u)] = make_uint4((uint32_t)sid, (uint32_t)(sid >> 32), u.failed_at, 0u);
     // accumulate: one atomic per wave when the wave covers one pixel, else per lane
@@ -890,6 +1093,57 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene
     flushWork(work, 4, T.nbox, T.ntri, nlk, T.nseg);
 }
 
+// Persistent drain of CHILD_PATH records (the surface-only integrators' sub-paths,
+// path.cpp:11-61 from the record's origin and direction), as k_children.
+template <int N>
+__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children_path(DScene S, RenderParams P, const char *q,
+                                                                            const unsigned *qcount, unsigned *qnext,
+                                                                            float *acc, float *acc_pe,
+                                                                            unsigned long long *work) {
+    Tracer<N> T(S);
+    const unsigned n = min(*qcount, P.qcap);
+    const int lane = threadIdx.x & 63;
+    uint32_t nlk = 0;
+    bool alive = false, exhausted = false;
+    unsigned rec = 0;
+    LiPath path;
+    Rng r;
+    while (true) {
+        bool fin = false;
+        f3 Lv{0, 0, 0};
+        if (!exhausted) {
+            const uint64_t need = __ballot(!alive);
+            if (need) {
+                const int lead = __ffsll((unsigned long long)need) - 1;
+                unsigned base = 0;
+                if (lane == lead) base = atomicAdd(qnext, (unsigned)__popcll(need));
+                base = __shfl(base, lead);
+                if (base + __popcll(need) >= n) exhausted = true;
+                const unsigned my = base + (unsigned)__popcll(need & ((1ull << lane) - 1ull));
+                if (!alive && my < n) {
+                    rec = my;
+                    const ChildHdr h = *reinterpret_cast<const ChildHdr *>(q + (size_t)rec * P.rec_stride);
+                    r.init(S.views[h.view].seed ^ 0x6a09e667f3bcc909ull, ((uint64_t)h.key_hi << 32) | h.key_lo);
+                    const f3 org{h.org[0], h.org[1], h.org[2]}, dir{h.dir[0], h.dir[1], h.dir[2]};
+                    Hit h0;
+                    T.rayIntersect(org, dir, true, h0);
+                    T.liStartAt(path, org, dir, (h.meta >> 4) & 255, -1, true, h0);
+                    alive = true;
+                }
+            }
+        }
+        if (alive && !T.pathStep(path, r)) {
+            alive = false;
+            fin = true;
+            Lv = path.ret;
+            nlk += r.nlk;
+        }
+        scatterFinished<N>(P, q, acc, acc_pe, fin, rec, Lv);
+        if (exhausted && !__any(alive)) break;
+    }
+    flushWork(work, 4, T.nbox, T.ntri, nlk, T.nseg);
+}
+
 // edge_manager.cpp:112-134 + integratorAD.cpp:126-144 for one primary-edge sample
 template <int N>
 DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, const DView &vw, int view, Rng &r, uint64_t sid,
@@ -956,7 +1210,9 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, const DView &vw, 
     h.org[0] = org.x; h.org[1] = org.y; h.org[2] = org.z;
     h.pixel = pix;
     h.view = view;
-    h.meta = CHILD_LI | 8 | (P.max_bounces << 4) | ((c.med_id + 1) << 12);
+    // the integrator's pixelColor: volpathAD.cpp:623-629, pathAD.cpp:216-219, directAD.cpp:101-125
+    if (P.ad_kind == AD_VOLPATH) h.meta = CHILD_LI | 8 | (P.max_bounces << 4) | ((c.med_id + 1) << 12);
+    else h.meta = CHILD_PATH | 8 | ((P.ad_kind == AD_DIRECT ? 1 : P.max_bounces) << 4);
     h.face = -1;
     for (int s = 0; s < 2; ++s) {
         double sg = s == 0 ? -1.0 : 1.0;
@@ -1076,11 +1332,22 @@ template <int N> static void launchAd(const DScene &S, const RenderParams &P, ui
                                       float *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,
                                       unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_ad_paths<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf, ovf_count, wk);
+    if (P.ad_kind == AD_PATH)
+        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf,
+                           ovf_count, wk);
+    else if (P.ad_kind == AD_DIRECT)
+        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf,
+                           ovf_count, wk);
+    else
+        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st,
+                           ovf, ovf_count, wk);
 }
 template <int N> static void launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
                                             float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
-    hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
+    if (P.ad_kind == AD_VOLPATH)
+        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
+    else
+        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
 }
 template <int N> static void launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,
                                            uint64_t total, char *q, unsigned *qc, unsigned *st, unsigned long long *wk,

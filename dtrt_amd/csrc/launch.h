@@ -8,7 +8,11 @@ namespace dt {
 
 // Per-render parameters (RenderOptions, integrator.h:10-25, plus the image layout).
 // Settings shared by all views of a render (per-view ones are in DView).
+// The forward-mode integrator of a render (IntegratorAD subclasses).
+enum AdKind { AD_VOLPATH = 0, AD_PATH = 1, AD_DIRECT = 2 };   // volpathAD.cpp, pathAD.cpp, directAD.cpp
+
 struct RenderParams {
+    int ad_kind;             // AdKind
     int max_bounces, sppse, mode;
     float ddist;
     int npix;                // pixels of all views (accumulator stride)

@@ -126,7 +126,7 @@ struct DView {
 
 // Child primal sub-path records written by the AD-path kernel and the
 // primary-edge kernel, consumed by the child kernel.
-enum ChildKind { CHILD_LI = 0, CHILD_LINS_NULL = 1, CHILD_LINS_SURF = 2 };
+enum ChildKind { CHILD_LI = 0, CHILD_LINS_NULL = 1, CHILD_LINS_SURF = 2, CHILD_PATH = 3 };   // PATH: path.cpp radiance
 
 struct ChildHdr {
     float org[3];

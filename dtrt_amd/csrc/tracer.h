@@ -253,6 +253,32 @@ template <int N> struct Tracer {
         return Ds<N>();
     }
 
+    // scene.cpp:346-397 — sampleEmitterDirectAD: no media; the shadow ray must first hit
+    // the sampled light within ShadowEpsilon of the sampled distance; the value's
+    // tangents come from the emitter only (pdf and geometry term are primal)
+    DI Ds<N> neeDirectAD(const HitAD<N> &its, const FrameAD<N> &shf, const float *r4, Dv<N> &wo, float &pdf) {
+        if (S.nlights == 0) return Ds<N>();
+        f3 lp, ln;
+        int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - its.p.val();
+        pdf = S.light_pmf[l] / S.light_areas[l];
+        if (dot(dir, its.sn.val()) > 0.f && dot(dir, ln) < 0.f && pdf != 0.f) {
+            float dist = length(dir), th;
+            dir = dir / dist;
+            int f = trace(its.p.val(), dir, SHADOW_EPS, FINF, th);
+            if (f >= 0 && __ldg(S.face_shape + f) == S.emitters[l].shape_id) {
+                HitAD<N> nh;
+                refineHitAD<N>(S, f, its.p, Dv<N>(dir), nh);
+                if (fabsf(nh.t.v - dist) < SHADOW_EPS) {
+                    pdf *= dist * dist / dot(ln, -dir);
+                    wo = shf.toLocal(Dv<N>(dir));
+                    return emitterEvalNormAD<N>(S.emitters[l], nh.sn, Dv<N>(-dir)) * (1.0f / pdf);
+                }
+            }
+        }
+        return Ds<N>();
+    }
+
     // volpathAD.cpp:50-142 — primal path tracer used by every edge / boundary sub-path
     // The primal estimator as a resumable state machine: liStart traces the first
     // segment, each liStep runs one iteration of the bounce loop of

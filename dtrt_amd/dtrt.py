@@ -45,6 +45,8 @@ VolPathTracerSimple = _m.VolPathTracerSimple
 PathTracer = _m.PathTracer
 DirectIntegrator = _m.DirectIntegrator
 ParticleTracer = _m.ParticleTracer
+PathTracerAD = _m.PathTracerAD
+DirectAD = _m.DirectAD
 angleEps = _m.angleEps
 abi_version = _m.abi_version
 set_nder = _m.set_nder

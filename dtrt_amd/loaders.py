@@ -320,9 +320,8 @@ def load_mitsuba(path, nder=1):
 
 
 def integrator_for(mod, name):
-    """Renderer object for an integrator type name of the XML (``volpathAD``,
-    ``volpath``, ``volpath_simple``, ``path``, ``direct``, ``ptracer``); ``pathAD`` and
-    ``directAD`` are not provided."""
+    """Renderer object for an integrator type name of the XML: every integrator the
+    reference's load_mitsuba.py:318-335 accepts."""
     if name == "volpathAD":
         return mod.VolPathTracerAD()
     if name == "volpath":
@@ -335,4 +334,8 @@ def integrator_for(mod, name):
         return mod.DirectIntegrator()
     if name == "ptracer":
         return mod.ParticleTracer()
+    if name == "pathAD":
+        return mod.PathTracerAD()
+    if name == "directAD":
+        return mod.DirectAD()
     raise NotImplementedError("integrator %s is not provided" % name)

@@ -364,14 +364,16 @@ def instantiate(mod, d: SceneDesc):
     return scene
 
 
-def render(mod, d: SceneDesc, seed=0, spp=None, mode=-1, sppe=None, sppse=None, max_bounces=None, scene=None, out=None):
-    """Render ``d`` with ``mod`` (product or oracle binding); returns float32[(nder+1), H, W, 3]."""
+def render(mod, d: SceneDesc, seed=0, spp=None, mode=-1, sppe=None, sppse=None, max_bounces=None, scene=None, out=None,
+           integrator=None):
+    """Render ``d`` with ``mod`` (product or oracle binding) and a forward-mode
+    integrator (default VolPathTracerAD); returns float32[(nder+1), H, W, 3]."""
     scene = scene if scene is not None else instantiate(mod, d)
     opts = mod.RenderOptions(seed, d.spp if spp is None else spp, d.max_bounces if max_bounces is None else max_bounces,
                              d.sppe if sppe is None else sppe, d.sppse if sppse is None else sppse, True, mode, d.ddist)
     if out is None:
         out = np.zeros(d.out_shape(), dtype=np.float32)
-    mod.VolPathTracerAD().render(scene, opts, out)
+    (integrator or mod.VolPathTracerAD()).render(scene, opts, out)
     return out
 
 

@@ -129,7 +129,11 @@ int dtrt_scene_init_edges(dtrt_object *scene, const float *weights);
  *   3 = PathTracer          (src/dtrt.cpp:123-125, path.cpp:64-96; surfaces only)
  *   4 = DirectIntegrator    (src/dtrt.cpp:119-121, direct.cpp:51-85; surfaces only)
  *   5 = ParticleTracer      (src/dtrt.cpp:127-129, ptracer.cpp:10-167): 100000 * spp
- *       light particles splatted into the image and divided by their count */
+ *       light particles splatted into the image and divided by their count
+ * The surface-only forward-mode integrators fill the VolPathTracerAD layout (media
+ * ignored; primary-edge pass included):
+ *   6 = PathTracerAD        (src/dtrt.cpp:144-146, pathAD.cpp)
+ *   7 = DirectAD            (src/dtrt.cpp:140-142, directAD.cpp) */
 int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image);
 /* Same, with `image` in device memory of the current HIP device (no host copies). */
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);

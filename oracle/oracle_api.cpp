@@ -423,11 +423,12 @@ void oracle_counters(long long *out6) {
 // integrator kind 0: VolPathTracerAD (integratorAD.cpp:19-166); kinds 1-4, float32[H][W][3]:
 // VolPathTracer (volpath.cpp:106-141), VolPathTracerSimple (volpath_simple.cpp:89-124),
 // PathTracer (path.cpp:64-96), DirectIntegrator (direct.cpp:51-85); kind 5: ParticleTracer
-// (ptracer.cpp:10-55)
+// (ptracer.cpp:10-55); kinds 6, 7: PathTracerAD (pathAD.cpp), DirectAD (directAD.cpp)
+// through integratorAD.cpp:19-166
 int oracle_render(int integrator, void *scene, const oracle_render_options *o, float *out) {
-    if (integrator < 0 || integrator > 5) {
+    if (integrator < 0 || integrator > 7) {
         g_err = "integrators: VolPathTracerAD (0), VolPathTracer (1), VolPathTracerSimple (2), PathTracer (3), "
-                "DirectIntegrator (4), ParticleTracer (5)";
+                "DirectIntegrator (4), ParticleTracer (5), PathTracerAD (6), DirectAD (7)";
         return -2;
     }
     OScene &s = static_cast<HScene *>(scene)->s;
@@ -436,11 +437,12 @@ int oracle_render(int integrator, void *scene, const oracle_render_options *o, f
         Tracer<1>(s).renderParticles(o->seed, o->spp, o->max_bounces, out);
         return 0;
     }
-    if (integrator > 0) {
+    if (integrator >= 1 && integrator <= 4) {
         Tracer<1>(s).renderPrimal(o->seed, o->spp, o->max_bounces, out, integrator);
         return 0;
     }
-#define OR_CASE(K) case K: Tracer<K>(s).render(o->seed, o->spp, o->max_bounces, o->sppe, o->sppse, o->mode, o->ddist_coeff, out); break;
+#define OR_CASE(K) case K: { Tracer<K> tr(s); tr.adKind = integrator == 6 ? 1 : integrator == 7 ? 2 : 0; \
+        tr.render(o->seed, o->spp, o->max_bounces, o->sppe, o->sppse, o->mode, o->ddist_coeff, out); } break;
     switch (s.nder) {
         OR_CASE(1) OR_CASE(2) OR_CASE(3) OR_CASE(4) OR_CASE(8) OR_CASE(16)
         default: g_err = "unsupported nder"; return -1;

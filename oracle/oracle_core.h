@@ -1068,6 +1068,16 @@ static Spec emitterEvalNorm(const OEmitter &e, const V3 &nrm, const V3 &d) {   /
     if (tmp > EPS) return e.I * (e.kappa * INV_TWOPI * std::exp(e.kappa * (tmp - 1.0)));
     return Spec();
 }
+template <int N> static DS<N> emitterEvalNormAD(const OEmitter &e, const DV<N> &n, const DV<N> &d) {   // area.cpp:17-19, area2.cpp:29-35
+    if (e.type == 0) return (e.two_sided || dot(n.val(), d.val()) > 0) ? DS<N>(e.I) : DS<N>();
+    D<N> tmp = ddot(n, d);
+    if (tmp.v > EPS) {
+        D<N> k(e.kappa);
+        for (int j = 0; j < N; ++j) k.g[j] = e.kappa_d[j];
+        return specAD<N>(e.I, e.I_d) * (dexp(k * (tmp - 1.0)) * k * INV_TWOPI);
+    }
+    return DS<N>();
+}
 template <int N> static DS<N> emitterEvalAD(const OEmitter &e, const ItsAD<N> &its, const DV<N> &d) {   // area.cpp:9-11, area2.cpp:13-19
     if (e.type == 0) return (e.two_sided || dot(its.geo.n.val(), d.val()) > 0) ? DS<N>(e.I) : DS<N>();
     D<N> tmp = ddot(its.sh.n, d);
@@ -1954,14 +1964,169 @@ template <int N> struct Tracer {
         return ret;
     }
 
-    // volpathAD.cpp:623-629
+    // scene.cpp:346-397 — sampleEmitterDirectAD (no media)
+    DS<N> sampleEmitterDirectAD(const ItsAD<N> &its, const double *rnd4, DV<N> &wo, double &pdf) const {
+        int l, k;
+        V3 lp, ln;
+        sampleLight(rnd4, l, k, lp, ln);
+        V3 dir = lp - its.p.val();
+        pdf = sc.light_pmf[l] / sc.light_areas[l];
+        if (dot(dir, its.sh.n.val()) > 0 && dot(dir, ln) < 0 && pdf != 0) {
+            double dist = norm(dir);
+            dir = dir / dist;
+            ItsAD<N> nits;
+            if (rayIntersectAD(its.p, DV<N>(dir), true, nits) && nits.shape == sc.emitters[l]->shape_id &&
+                std::abs(nits.t.v - dist) < SHADOW_EPS) {
+                pdf *= dist * dist / dot(ln, -dir);
+                wo = its.sh.toLocal(DV<N>(dir));
+                return emitterEvalNormAD<N>(*sc.emitters[l], nits.sh.n, DV<N>(-dir)) / pdf;
+            }
+        }
+        return DS<N>();
+    }
+
+    // pathAD.cpp:64-214 (DIRECT: directAD.cpp:5-98) — surface-only forward-mode
+    // estimators; media are ignored. `its` is the hit of the incoming ray direction rd.
+    DS<N> LiSurfaceAD(const ItsAD<N> &its, Pcg &s, const DV<N> &rd, int max_depth, int mode, bool incEmission, int nEdge,
+                      bool direct) const {
+        DS<N> ret;
+        if (its.shape < 0) return ret;
+        const OShape &shp = shape(its.shape);
+        if (shp.isEmitter() && (mode & 1) && incEmission) ret += emitterEvalAD<N>(*sc.emitters[shp.light_id], its, -rd);
+        if (max_depth == 0) return ret;
+        const OBsdf &b = bsdfOf(its.shape);
+        Its its0 = its.toIts();
+        if (mode & 2) {
+            double rnd4[4] = {s.next1D(), s.next1D(), s.next1D(), s.next1D()};
+            DV<N> wo;
+            double pdf_nee;
+            DS<N> value = sampleEmitterDirectAD(its, rnd4, wo, pdf_nee);
+            if (!value.isZero(EPS)) {
+                DS<N> bv = bsdfEvalAD<N>(b, its, wo);
+                double bp = bsdfPdf(b, its0, wo.val());
+                ret += value * bv * (direct ? pdf_nee / (pdf_nee + bp) : mis(pdf_nee, bp));
+            }
+        }
+        {
+            double rnd3[3] = {s.next1D(), s.next1D(), s.next1D()};
+            DV<N> wo;
+            double bpdf, beta;
+            DS<N> value = bsdfSampleAD<N>(b, its, rnd3, wo, bpdf, beta);
+            if (!value.isZero(EPS)) {
+                DV<N> nd = b.isNull() ? rd : its.sh.toWorld(wo);
+                ItsAD<N> its1;
+                if (rayIntersectAD(its.p, nd, true, its1)) {
+                    const OShape &s1 = shape(its1.shape);
+                    if (s1.isEmitter() && (mode & 2)) {
+                        DS<N> lc = emitterEvalAD<N>(*sc.emitters[s1.light_id], its1, -nd);
+                        if (!lc.val().isZero()) {
+                            V3 dd = its1.p.val() - its.p.val();
+                            double G = its1.wi.val().z / dot(dd, dd);
+                            double pn = sc.light_pmf[s1.light_id] / sc.light_areas[s1.light_id] / G;
+                            ret += value * lc * (direct ? bpdf / (pn + bpdf) : mis(bpdf, pn));
+                        }
+                    }
+                    if (!direct) ret += value * LiSurfaceAD(its1, s, nd, max_depth - 1, mode, false, nEdge, false);
+                }
+            }
+        }
+        if (nEdge > 0 && (mode & 4) && sc.edges_distrb.sum > 0) {
+            double der[N][3] = {};
+            const DV<N> &x1 = its.p;
+            for (int i = 0; i < nEdge; ++i) {
+                double rnd = s.next1D();
+                int sid;
+                double edgePdf;
+                const Edge *e = sampleEdge(rnd, sid, edgePdf);
+                double t = rnd;
+                const OShape &esh = shape(sid);
+                if (esh.isSilhouette(*e, x1.val()) <= 0) continue;
+                DV<N> v0 = esh.vertexAD<N>(e->v0), v1 = esh.vertexAD<N>(e->v1);
+                DV<N> w1;
+                V3 nrm;
+                double dist, geo = 1.0;
+                if (direct) {   // scene.h:103-108: pdf per unit length, linear in t
+                    edgePdf /= e->length;
+                    DV<N> y1 = v0 * (1.0 - t) + v1 * t;
+                    w1 = y1 - x1;
+                    nrm = normalized(cross(v0.val() - x1.val(), v1.val() - x1.val()));
+                    D<N> dAD = dnorm(w1);
+                    w1 = w1 / dAD;
+                    dist = dAD.v;
+                    double ct = -dot(w1.val(), normalized(v1.val() - v0.val()));
+                    geo = std::sqrt(1.0 - ct * ct) / dist;
+                } else {
+                    DV<N> e0 = v0 - x1, e1 = v1 - x1;
+                    nrm = normalized(cross(e0.val(), e1.val()));
+                    V3 ne0 = normalized(e0.val()), ne1 = normalized(e1.val());
+                    double theta = std::acos(dot(ne0, ne1)), sinTheta = std::sin(theta);
+                    if (theta < EPS || theta > PI - EPS) continue;
+                    double w0 = std::sin(theta * (1.0 - t)) / sinTheta, ww1 = std::sin(theta * t) / sinTheta;
+                    w0 /= norm(e0.val());
+                    ww1 /= norm(e1.val());
+                    double tmp = w0 + ww1;
+                    w0 /= tmp;
+                    ww1 /= tmp;
+                    if (w0 < -EPS || ww1 < -EPS || std::abs(w0 + ww1 - 1.0) > EPS) { g_counters.bad_edge_weight++; continue; }
+                    w1 = e0 * w0 + e1 * ww1;
+                    D<N> dAD = dnorm(w1);
+                    w1 = w1 / dAD;
+                    dist = dAD.v;
+                    edgePdf /= theta;
+                    double tmpo = 0;
+                    for (int j = 0; j < N; ++j) tmpo = std::max(tmpo, std::abs(dot(w1.grad(j), w1.val())));
+                    if (tmpo > EPS) { g_counters.nonortho++; continue; }
+                    if (w1.gradZero(EPS)) continue;
+                }
+                Spec bsdfVal = bsdfEval(b, its0, its0.sh.toLocal(w1.val()));
+                if (bsdfVal.isZero(EPS)) continue;
+                int hs, hk;   // scene.cpp:150-176
+                if (sc.intersect(x1.val(), w1.val(), SHADOW_EPS, (1.0 - SHADOW_EPS) * dist, hs, hk)) continue;
+                Spec delta;
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    V3 w2 = normalized(sgn == 0 ? w1.val() - ANGLE_EPS * nrm : w1.val() + ANGLE_EPS * nrm);
+                    Spec v;
+                    if (direct) {
+                        Its i1;
+                        if (rayIntersect(x1.val(), w2, true, i1) && shape(i1.shape).isEmitter())
+                            v = emitterEvalIts(*sc.emitters[shape(i1.shape).light_id], i1, -w1.val());
+                    } else {
+                        v = LiPath(s, x1.val(), w2, max_depth - 1);
+                    }
+                    if (sgn == 0) delta += v; else delta -= v;
+                }
+                if (delta.isZero(EPS)) continue;
+                for (int j = 0; j < N; ++j) {
+                    double ndot = dot(nrm, w1.grad(j));
+                    for (int c = 0; c < 3; ++c) der[j][c] += ndot * bsdfVal[c] * delta[c] * geo / edgePdf;
+                }
+            }
+            for (int j = 0; j < N; ++j)
+                for (int c = 0; c < 3; ++c) ret.c[c].g[j] += der[j][c] / nEdge;
+        }
+        return ret;
+    }
+
+    int adKind = 0;   // 0 VolPathTracerAD, 1 PathTracerAD, 2 DirectAD
+
+    // volpathAD.cpp:623-629; pathAD.cpp:216-219 / directAD (the path tracer's radiance,
+    // depth 1 for directAD)
     Spec pixelColor(Pcg &s, int max_bounces, double x, double y) const {
         V3 o, d;
         sc.cam->samplePrimaryRay(x, y, o, d);
+        if (adKind) return LiPath(s, o, d, adKind == 2 ? 1 : max_bounces);
         return Li(s, o, d, max_bounces, sc.cam->med_id, true);
     }
     // volpathAD.cpp:632-643
     DS<N> pixelColorAD(Pcg &s, int max_bounces, int mode, int nEdge, double ddist, double x, double y) const {
+        if (adKind) {   // pathAD.cpp:222-227 (camera ray from 0); directAD.cpp:11 (from ShadowEpsilon)
+            DV<N> o, d;
+            sc.cam->samplePrimaryRayAD<N>(x, y, o, d);
+            ItsAD<N> its;
+            if (!rayIntersectAD(o, d, adKind == 2, its)) return DS<N>();
+            return LiSurfaceAD(its, s, d, adKind == 2 ? 1 : max_bounces, adKind == 2 ? 0x3F : (mode >= 0 ? mode : 0x3F),
+                               true, nEdge, adKind == 2);
+        }
         DV<N> o, d;
         sc.cam->samplePrimaryRayAD<N>(x, y, o, d);
         ItsAD<N> its;

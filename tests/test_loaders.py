@@ -146,5 +146,8 @@ def test_example_scene_renders_through_the_oracle(oracle):
     img = scenes.render(oracle, d, seed=0, spp=4, sppe=0, sppse=0)
     assert np.isfinite(img).all() and img[0].mean() > 0
     assert loaders.integrator_for(oracle, "volpath").kind == 1
+    kinds = [loaders.integrator_for(oracle, n).kind for n in
+             ("volpathAD", "volpath", "volpath_simple", "path", "direct", "ptracer", "pathAD", "directAD")]
+    assert kinds == [0, 1, 2, 3, 4, 5, 6, 7]
     with pytest.raises(NotImplementedError):
-        loaders.integrator_for(oracle, "pathAD")
+        loaders.integrator_for(oracle, "bdpt")

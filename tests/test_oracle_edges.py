@@ -53,3 +53,50 @@ def test_lins_nee_gated_by_light_bsdf(oracle):
     d.bsdfs = [("null",), ("null",)]
     transmissive = np.mean([scenes.render(oracle, d, seed=s, mode=16)[1, ..., 0].sum() for s in range(3)])
     assert abs(opaque) < 0.05 * abs(transmissive)
+
+
+# ------------------------------------------- surface-only AD integrators (pathAD, directAD)
+def _surface_ad(mod, name):
+    return mod.PathTracerAD() if name == "path" else mod.DirectAD()
+
+
+def test_surface_ad_primary_edges_translating_emitter(oracle):
+    """integratorAD.cpp's primary-edge pass through PathTracerAD / DirectAD's pixelColor."""
+    for name in ("path", "direct"):
+        d, half = F.translating_emitter()
+        g = scenes.render(oracle, d, seed=0, integrator=_surface_ad(oracle, name))[1, ..., 0]
+        w = d.width // 2
+        assert abs(g[:, :w].sum() / -half - 1) < 5e-3, name
+        assert abs(g[:, w:].sum() / half - 1) < 5e-3, name
+
+
+def test_surface_ad_edges_match_lambert_derivative(oracle):
+    """pathAD.cpp:112-210 (angle-uniform points, primal sub-paths) and directAD.cpp:55-95
+    (points linear in t, emission only) against the analytic derivative of the
+    irradiance of a translating quad light."""
+    for name in ("path", "direct"):   # unstratified edge samples: more spp than the volpathAD test
+        d, dE = F.translating_light(res=10, spp=2048)
+        img = scenes.render(oracle, d, seed=0, integrator=_surface_ad(oracle, name))
+        g = img[1, ..., 0]
+        assert np.corrcoef(g.ravel(), dE.ravel())[0, 1] > 0.998, name
+        assert np.abs(g - dE).max() < 0.06 * np.abs(dE).max(), name
+
+
+def test_surface_ad_direct_lighting_value(oracle):
+    d, lv, rho, Le = F.direct_lighting(res=16, spp=256)
+    ref = F.direct_expected(d, lv, rho, Le)
+    for name in ("path", "direct"):
+        v = scenes.render(oracle, d, seed=1, integrator=_surface_ad(oracle, name))[0, ..., 0]
+        assert abs(v.mean() / ref.mean() - 1) < 5e-3, name
+
+
+def test_path_ad_agrees_with_volpath_ad_on_surfaces(oracle):
+    """Without media or null surfaces PathTracerAD and VolPathTracerAD estimate the same
+    image and tangents (all EMode terms, both edge passes)."""
+    from tests.helpers import assert_3sigma, block_z
+    d = scenes.config0(nder=1, res=16, spp=8)
+    sc = scenes.instantiate(oracle, d)
+    a = np.stack([scenes.render(oracle, d, seed=s, scene=sc, sppe=256) for s in range(24)]).astype(np.float64)
+    b = np.stack([scenes.render(oracle, d, seed=100 + s, scene=sc, sppe=256, integrator=oracle.PathTracerAD())
+                  for s in range(24)]).astype(np.float64)
+    assert_3sigma(block_z(a, b, 4), what="pathAD vs volpathAD")
