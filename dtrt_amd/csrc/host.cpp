@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -516,7 +517,7 @@ struct Scene : Obj {
         if (stream) (void)hipStreamSynchronize(stream);
         graph.reset();
         for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
-        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf})
+        for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf, &tunebuf})
             if (b->p) (void)hipFree(b->p);
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -561,6 +562,12 @@ struct Scene : Obj {
     uint64_t work_counts[12] = {0};
     int edge_ebase_total = 0;
     size_t queue_budget = size_t(6) << 30;   // bytes of queued child records
+    // launch settings chosen by autotune(): persistent child blocks per CU, the share of
+    // the queue an AD chunk aims to fill, and the largest AD chunk
+    int tune_child_blocks = 32;
+    double tune_fill = 0.75;
+    uint64_t tune_max_chunk = uint64_t(1) << 22;
+    DevBuf tunebuf;
 
     template <class T> const T *upload(const std::vector<T> &v) {
         if (v.empty()) return nullptr;
@@ -597,6 +604,7 @@ struct Scene : Obj {
                     std::vector<uint64_t> &pe_total);
     bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
     bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator);
+    bool autotune(const dtrt_render_options &o, int ad_kind, double *best_ms, int64_t *settings3);
     bool setDensity(int med, const float *density, bool on_device);
     bool setVoxelLanes(int med, const int *vox, int n);
     bool uploadMedia() {
@@ -904,7 +912,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     // most 2 records, so its chunks never overflow.
     const uint64_t cap = std::max<uint64_t>(128, std::min<uint64_t>(queue_budget / P.rec_stride, 0xFFFFFFF0ull));
     const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
-    const uint64_t max_chunk = std::min<uint64_t>(MAX_CHUNK, std::max<uint64_t>(total, 64));
+    const uint64_t max_chunk = std::min<uint64_t>(tune_max_chunk, std::max<uint64_t>(total, 64));
     P.qcap = (unsigned)cap;
     if (!ensure(acc, img * 4) || !ensure(acc_pe, (size_t)nder * npix * 3 * 4) || !ensure(qcount, 64) ||
         !ensure(stats, 64 * 4) || !ensure(work, 16 * 8) || !ensure(queue, (size_t)cap * P.rec_stride) ||
@@ -935,7 +943,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const unsigned childGrid = (unsigned)std::max(1, ncu) * 32;
+    const unsigned childGrid = (unsigned)std::max(1, ncu) * (unsigned)tune_child_blocks;
     std::memset(kernel_ms, 0, sizeof(kernel_ms));
     std::memset(kernel_launches, 0, sizeof(kernel_launches));
     auto timed = [&](int slot, auto &&fn) -> bool {
@@ -988,7 +996,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
         s0 += n;
         // next chunk: fill ~3/4 of the queue at the observed rate, in whole waves
         const double rate = (double)delivered / (double)n;
-        uint64_t next = rate > 0 ? (uint64_t)(0.75 * (double)cap / rate) : max_chunk;
+        uint64_t next = rate > 0 ? (uint64_t)(tune_fill * (double)cap / rate) : max_chunk;
         chunk = std::max<uint64_t>(64, std::min(next, max_chunk) / 64 * 64);
     }
     for (int v = 0; v < nv; ++v)
@@ -1054,7 +1062,7 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const unsigned childGrid = (unsigned)std::max(1, ncu) * 32;
+    const unsigned childGrid = (unsigned)std::max(1, ncu) * (unsigned)tune_child_blocks;
     unsigned *qc = static_cast<unsigned *>(qcount.p);
     unsigned *st = static_cast<unsigned *>(stats.p);
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
@@ -1285,6 +1293,58 @@ bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_d
     for (int i = 0; i < 12; ++i) work_counts[i] = hw[i];
     for (int i = 0; i < 10; ++i) last_stats[i] = 0;
     last_stats[7] = (int64_t)total;
+    return true;
+}
+
+// Coordinate descent over the launch settings, each candidate timed on one full render
+// of `o` (after one warm-up): persistent child blocks per CU, then the queue fill target
+// of the AD chunks, then the largest AD chunk. The fastest values stay in the scene.
+// settings3 = {child blocks per CU, fill target in 1/100, max AD chunk}.
+bool Scene::autotune(const dtrt_render_options &o, int ad_kind, double *best_ms, int64_t *settings3) {
+    const size_t npix = cam.cropped() ? (size_t)cam.cw * cam.ch : (size_t)cam.w * cam.h;
+    if (!ensure(tunebuf, (size_t)(nder + 1) * npix * 3 * 4)) return false;
+    float *out = static_cast<float *>(tunebuf.p);
+    auto run = [&](double &ms) -> bool {
+        auto t0 = std::chrono::steady_clock::now();
+        if (!render(o, out, true, ad_kind)) return false;
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return true;
+    };
+    double best = 0;
+    if (!run(best) || !run(best)) return false;   // warm-up, then the current settings
+    const int blocks[] = {16, 24, 32, 48};
+    const double fills[] = {0.5, 0.75, 0.9};
+    const uint64_t chunks[] = {uint64_t(1) << 21, uint64_t(1) << 22, uint64_t(1) << 23};
+    for (int b : blocks) {
+        if (b == tune_child_blocks) continue;
+        const int keep = tune_child_blocks;
+        tune_child_blocks = b;
+        double ms;
+        if (!run(ms)) return false;
+        if (ms < best) best = ms; else tune_child_blocks = keep;
+    }
+    for (double f : fills) {
+        if (f == tune_fill) continue;
+        const double keep = tune_fill;
+        tune_fill = f;
+        double ms;
+        if (!run(ms)) return false;
+        if (ms < best) best = ms; else tune_fill = keep;
+    }
+    for (uint64_t c : chunks) {
+        if (c == tune_max_chunk) continue;
+        const uint64_t keep = tune_max_chunk;
+        tune_max_chunk = c;
+        double ms;
+        if (!run(ms)) return false;
+        if (ms < best) best = ms; else tune_max_chunk = keep;
+    }
+    if (best_ms) *best_ms = best;
+    if (settings3) {
+        settings3[0] = tune_child_blocks;
+        settings3[1] = (int64_t)std::lround(tune_fill * 100.0);
+        settings3[2] = (int64_t)tune_max_chunk;
+    }
     return true;
 }
 
@@ -1675,6 +1735,20 @@ int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
     if (!s || !out) return -1;
     for (int i = 0; i < 10; ++i) out[i] = s->last_stats[i];
     return 0;
+}
+
+int dtrt_scene_autotune(int integrator, dtrt_object *scene, const dtrt_render_options *options, double *best_ms,
+                        int64_t *settings3) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !options) { if (s) g_err = "autotune: bad arguments"; return -1; }
+    if (integrator != 0 && integrator != 6 && integrator != 7) { g_err = "autotune: forward-mode integrators only"; return -2; }
+    try {
+        return s->autotune(*options, integrator == 6 ? AD_PATH : integrator == 7 ? AD_DIRECT : AD_VOLPATH, best_ms,
+                           settings3) ? 0 : -1;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
 }
 
 int dtrt_scene_set_density(dtrt_object *scene, int medium, const float *density, int on_device) {

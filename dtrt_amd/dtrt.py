@@ -101,6 +101,20 @@ def kernel_times(scene):
     return np.array(ms[:]), np.array(ln[:]), np.array(wk[:], dtype=np.uint64)
 
 
+_lib.dtrt_scene_autotune.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+
+
+def autotune(scene, options, integrator=0):
+    """``dtrt_scene_autotune``: picks and keeps the fastest launch settings for renders
+    like ``options``; returns (best ms, {"child_blocks_per_cu", "fill_percent", "max_chunk"})."""
+    st = options._struct()
+    ms = C.c_double()
+    s3 = (C.c_int64 * 3)()
+    _chk(_lib.dtrt_scene_autotune(integrator, scene._h, C.byref(st), C.byref(ms), s3), "autotune")
+    return ms.value, {"child_blocks_per_cu": int(s3[0]), "fill_percent": int(s3[1]), "max_chunk": int(s3[2])}
+
+
 def render_views(scene, cameras, options, images, integrator=0):
     """``dtrt_render_views``: one pass over several views.
 

@@ -138,6 +138,16 @@ int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *o
 /* Same, with `image` in device memory of the current HIP device (no host copies). */
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);
 
+/* Autotuner (no counterpart in the reference): renders `options` with the scene's
+ * camera several times, by coordinate descent over the launch settings (persistent
+ * child blocks per CU, the queue share an AD chunk aims at, the largest AD chunk), and
+ * keeps the fastest in the scene for later renders, view batches and graph captures.
+ * integrator: 0, 6 or 7. best_ms (may be NULL): the fastest render; settings3 (may be
+ * NULL): {blocks per CU, fill target in percent, max AD chunk}. Images and stats of
+ * the last call are overwritten. */
+int dtrt_scene_autotune(int integrator, dtrt_object *scene, const dtrt_render_options *options, double *best_ms,
+                        int64_t *settings3);
+
 /* Several views of one scene in one pass. View v renders cameras[v] (NULL entry or
  * NULL array: the scene's camera) with options[v] into images[v], laid out as for
  * dtrt_render for that camera's size/crop. Views may differ in camera, resolution,

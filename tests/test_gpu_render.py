@@ -179,3 +179,20 @@ def test_rotating_grid_parity(gpu, oracle):
     d = F.rotating_grid()
     z = _parity(gpu, oracle, d, seeds=16, block=4)
     assert_3sigma(z, what="rotating grid")
+
+
+def test_autotune_keeps_results_and_picks_listed_settings(gpu):
+    """dtrt_scene_autotune only changes launch settings: the same seed renders the same
+    image (up to float summation order) before and after tuning."""
+    d = scenes.config1(nder=3, res=64, spp=32, grid_res=16)
+    sc = scenes.instantiate(gpu, d)
+    before = scenes.render(gpu, d, seed=9, scene=sc).astype(np.float64)
+    opts = gpu.RenderOptions(3, d.spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
+    ms, st = gpu.autotune(sc, opts)
+    assert ms > 0
+    assert st["child_blocks_per_cu"] in (16, 24, 32, 48) and st["fill_percent"] in (50, 75, 90)
+    assert st["max_chunk"] in (1 << 21, 1 << 22, 1 << 23)
+    after = scenes.render(gpu, d, seed=9, scene=sc).astype(np.float64)
+    scale = np.abs(before).reshape(before.shape[0], -1).max(axis=1)
+    for lane in range(before.shape[0]):
+        assert np.allclose(after[lane], before[lane], rtol=1e-3, atol=1e-4 * scale[lane]), lane
