@@ -261,6 +261,51 @@ def glass_leg(iters=4):
             "loss_first_steps": losses, "reference_loss_first_steps": [820.2951, 721.77826, 631.98737, 548.87231, 474.86902]}
 
 
+def inverse_leg(spp_divisor=1):
+    """One batched inverse-rendering step on the config[4] workload: 32 orbit views
+    (res 128^2..768^2, spp 16..256 drawn with PCG32 seed 5, each divided by
+    spp_divisor) of a 1,003,520-face displaced sphere in a 256^3 fBm medium. Forward:
+    one dtrt_render_views pass with 8 shape-deformation tangent lanes (all terms).
+    Backward: one dtrt_render_views_adjoint pass for all 16.8M node densities, with
+    the loss sum(I^2) over every view (d loss / d I = 2 I)."""
+    import numpy as np
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    t0 = time.perf_counter()
+    d = scenes.config4()
+    views = scenes.orbit_views(32, seed=5, spp_choices=(16, 32, 64, 128, 256), radius=3.2)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sc = scenes.instantiate(G, d)
+    cams = [G.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
+            for c2w, w, h, _ in views]
+    spps = [max(1, spp // spp_divisor) for _, _, _, spp in views]
+    opts = [G.RenderOptions(500 + i, spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist) for i, spp in enumerate(spps)]
+    outs = [torch.zeros((d.nder + 1, h, w, 3), device="cuda") for _, w, h, _ in views]
+    samples = sum(w * h * spp for (_, w, h, _), spp in zip(views, spps))
+    G.render_views(sc, cams[:1], opts[:1], outs[:1])   # builds device buffers, primary edges of view 0
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G.render_views(sc, cams, opts, outs)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    dimgs = [2.0 * o[0] for o in outs]
+    grad = torch.zeros(d.media[0].density.shape, dtype=torch.float32, device="cuda")
+    aopts = [G.RenderOptions(900 + i, spp, d.max_bounces, d.sppe, 0, True, 1 | 2 | 8, 0.0) for i, spp in enumerate(spps)]
+    t0 = time.perf_counter()
+    G.render_views_adjoint(sc, cams, aopts, 0, dimgs, grad)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    return {"workload": "config[4]: 32 orbit views (res 128^2..768^2, spp 16..256 / %d, PCG32 seed 5) of a 1,003,520-face "
+                        "displaced sphere in a 256^3 fBm medium; forward nder 8 (shape lanes), all terms; adjoint over "
+                        "16,777,216 node densities, loss sum(I^2)" % spp_divisor,
+            "spp_divisor": spp_divisor, "samples": samples, "scene_generation_s": t_gen, "setup_s": t_setup,
+            "forward_views_s": t_fwd, "forward_samples_per_s": samples / t_fwd, "adjoint_views_s": t_adj,
+            "step_s": t_fwd + t_adj, "grad_nonzero_nodes": int((grad != 0).sum().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +319,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
     ap.add_argument("--no-primal", action="store_true", help="skip the primal-integrator timing")
     ap.add_argument("--no-examples", action="store_true", help="skip the glass_1 optimisation-step timing")
+    ap.add_argument("--inverse-spp-divisor", type=int, default=8,
+                    help="divide the config[4] views' spp by this in the batched inverse-step leg (0: skip the leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -377,6 +424,8 @@ def main():
         rec["primal"] = primal_leg(args.res, args.spp)
     if not args.no_examples:
         rec["glass_1"] = glass_leg()
+    if args.inverse_spp_divisor > 0:
+        rec["inverse_step"] = inverse_leg(args.inverse_spp_divisor)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(desc)
         rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]

@@ -602,7 +602,12 @@ struct Scene : Obj {
     }
     bool setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
                     std::vector<uint64_t> &pe_total);
-    bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad);
+    bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad) {
+        const Camera *c = &cam;
+        return renderAdjointViews(1, &c, &o, med, dimg, grad);
+    }
+    bool renderAdjointViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, int med, const float *dimg,
+                            float *grad);
     bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator);
     bool autotune(const dtrt_render_options &o, int ad_kind, double *best_ms, int64_t *settings3);
     bool setDensity(int med, const float *density, bool on_device);
@@ -1198,20 +1203,28 @@ bool Scene::setVoxelLanes(int med, const int *vox, int n) {
 // Path-replay adjoint of the interior terms (emission, surface, volume) w.r.t. the
 // node densities of medium `med`; dimg = d loss / d image value [npix][3], grad
 // accumulates [z][y][x]. Both are device pointers.
-bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad) {
-    if (!gridMedium(med) || !dimg || !grad) return false;
+// Views adjoint: dimg holds the views' loss gradients back to back ([h][w][3] each, in
+// view order); every view's samples replay the streams of its forward render.
+bool Scene::renderAdjointViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, int med,
+                               const float *dimg, float *grad) {
+    if (!gridMedium(med) || !dimg || !grad || nv <= 0) return false;
+    const dtrt_render_options &o = opts[0];
     RenderParams P{};
     P.max_bounces = o.max_bounces; P.sppse = 0;
     P.mode = (o.mode >= 0 ? o.mode : 0x3F) & (1 | 2 | 8);
     P.ddist = 0.f;
-    if (o.spp <= 0 || o.max_bounces < 0 || o.max_bounces > 200) return fail("adjoint: bad options");
-    dtrt_render_options ov = o;
-    ov.sppe = o.sppe < 0 ? -1 : 0;   // pixel jitter as in the forward render; no primary-edge pass
-    const Camera *c = &cam;
+    std::vector<dtrt_render_options> ov(opts, opts + nv);
+    for (auto &x : ov) {
+        if (x.spp <= 0 || x.max_bounces != o.max_bounces || x.mode != o.mode || x.max_bounces < 0 || x.max_bounces > 200)
+            return fail("adjoint: bad options (views must share max_bounces and mode)");
+        x.sppe = x.sppe < 0 ? -1 : 0;   // pixel jitter as in the forward render; no primary-edge pass
+    }
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
-    if (!setupViews(1, &c, &ov, hv, pe_total)) return false;
-    P.npix = hv[0].npix;
+    if (!setupViews(nv, cams, ov.data(), hv, pe_total)) return false;
+    uint64_t total = 0;
+    P.npix = 0;
+    for (int v = 0; v < nv; ++v) { P.npix += hv[v].npix; total += (uint64_t)hv[v].npix * hv[v].spp; }
     if (!ensure(stats, 64 * 4) || !ensure(work, 16 * 8)) return false;
     HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
     if (ev.empty()) {
@@ -1220,7 +1233,7 @@ bool Scene::renderAdjoint(const dtrt_render_options &o, int med, const float *di
     }
     unsigned *st = static_cast<unsigned *>(stats.p) + 32;   // warnings of the replays are not reported
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
-    const uint64_t total = (uint64_t)P.npix * o.spp, chunk = uint64_t(1) << 22;
+    const uint64_t chunk = uint64_t(1) << 22;
     std::memset(kernel_ms, 0, sizeof(kernel_ms));
     std::memset(kernel_launches, 0, sizeof(kernel_launches));
     HIPCHK(hipEventRecord(ev[0], stream));
@@ -1755,6 +1768,25 @@ int dtrt_scene_set_density(dtrt_object *scene, int medium, const float *density,
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s) return -1;
     return s->setDensity(medium, density, on_device != 0) ? 0 : -1;
+}
+
+int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                              const dtrt_render_options *options, int medium, const float *d_images_device,
+                              float *d_density_device) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !options || nviews <= 0) { if (s) g_err = "render_views_adjoint: bad arguments"; return -1; }
+    if (integrator != 0) { g_err = "render_views_adjoint: only VolPathTracerAD (0) is implemented"; return -1; }
+    std::vector<const Camera *> cams(nviews);
+    for (int v = 0; v < nviews; ++v) {
+        cams[v] = (cameras && cameras[v]) ? as<Camera>(cameras[v], T_CAMERA) : &s->cam;
+        if (!cams[v]) return -1;
+    }
+    try {
+        return s->renderAdjointViews(nviews, cams.data(), options, medium, d_images_device, d_density_device) ? 0 : -1;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
 }
 
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,

@@ -929,16 +929,19 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
     Tracer<N> T(S);
     const uint64_t sid = s0 + li;
-    const DView &vw = S.views[0];
-    const int pixel = (int)(sid / (uint64_t)vw.spp);
-    Pusher u{nullptr, nullptr, pixel, 0, 0, 0, 0, Pusher::NO_FAIL};
+    const int v = viewOf(S, sid);
+    const DView &vw = S.views[v];
+    const uint64_t ls = sid - vw.s0;   // sample index within the view: keys its random stream
+    const int pixel = (int)(ls / (uint64_t)vw.spp);
+    const size_t gp = (size_t)vw.pix0 + pixel;   // pixel of the concatenated loss-gradient images
+    Pusher u{nullptr, nullptr, (int)gp, v, 0, 0, 0, Pusher::NO_FAIL};
     uint32_t nlk = 0;
     if (li < nsamp) {
         const float inv = 1.0f / vw.spp;
-        const f3 adj{dimg[3 * (size_t)pixel] * inv, dimg[3 * (size_t)pixel + 1] * inv, dimg[3 * (size_t)pixel + 2] * inv};
+        const f3 adj{dimg[3 * gp] * inv, dimg[3 * gp + 1] * inv, dimg[3 * gp + 2] * inv};
         if (adj.x != 0.f || adj.y != 0.f || adj.z != 0.f) {
             Rng r;
-            r.init(vw.seed, sid);
+            r.init(vw.seed, ls);
             Ds<N> L;
             adPathSample<N, AdjPass::TOTAL>(T, P, vw, r, pixel, L, u, stats, nullptr);
             nlk += r.nlk;
@@ -946,7 +949,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
             if (Lt.x != 0.f || Lt.y != 0.f || Lt.z != 0.f) {
                 AdjCtx ac{grad, S.media + pmed, adj, Lt};
                 Rng r2;
-                r2.init(vw.seed, sid);
+                r2.init(vw.seed, ls);
                 Ds<N> L2;
                 adPathSample<N, AdjPass::SCATTER>(T, P, vw, r2, pixel, L2, u, stats, &ac);
                 nlk += r2.nlk;

@@ -77,6 +77,8 @@ _lib.dtrt_scene_replay_views.argtypes = [_P, C.POINTER(C.c_uint64)]
 _lib.dtrt_render_views.argtypes = [C.c_int, _P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
                                    C.POINTER(_P), C.c_int]
 _lib.dtrt_render_adjoint.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
+_lib.dtrt_render_views_adjoint.argtypes = [C.c_int, _P, C.c_int, C.POINTER(_P),
+                                           C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
 _lib.dtrt_test_shape_edges.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]
 _lib.dtrt_test_trace.argtypes = [_P, _P, C.c_int, _P, _P]
 _lib.dtrt_test_li.argtypes = [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, _P]
@@ -113,6 +115,19 @@ def autotune(scene, options, integrator=0):
     s3 = (C.c_int64 * 3)()
     _chk(_lib.dtrt_scene_autotune(integrator, scene._h, C.byref(st), C.byref(ms), s3), "autotune")
     return ms.value, {"child_blocks_per_cu": int(s3[0]), "fill_percent": int(s3[1]), "max_chunk": int(s3[2])}
+
+
+def render_views_adjoint(scene, cameras, options, medium, d_images, d_density, integrator=0):
+    """``dtrt_render_views_adjoint``: d_images is a list of device tensors [h][w][3] (one
+    per view); the density gradient of all views is accumulated into d_density."""
+    import torch
+    n = len(options)
+    cams = (_P * n)(*[None if c is None else c._h for c in (cameras or [None] * n)])
+    opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
+    flat = torch.cat([g.reshape(-1) for g in d_images]).float().contiguous()
+    _chk(_lib.dtrt_render_views_adjoint(integrator, scene._h, n, cams, opts, int(medium), C.c_void_p(flat.data_ptr()),
+                                        C.c_void_p(d_density.data_ptr())), "render_views_adjoint")
+    torch.cuda.synchronize()
 
 
 def render_views(scene, cameras, options, images, integrator=0):

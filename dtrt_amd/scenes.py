@@ -479,6 +479,16 @@ def config2(nder=8, res=512, spp=256, freq=71, grid_res=128, crop=None):
                      name="cfg2")
 
 
+def config4(nder=8, freq=224, grid_res=256):
+    """The batched inverse-rendering scene: config2's construction at scale, a displaced
+    geodesic sphere of frequency 224 (1,003,520 faces, rough dielectric) inside a 256^3
+    fBm medium (64 MiB), 8 deformation lanes. Views come from orbit_views(32, seed 5,
+    spp 16..256, radius 3.2)."""
+    d = config2(nder=nder, res=512, spp=64, freq=freq, grid_res=grid_res)
+    d.name = "cfg4"
+    return d
+
+
 def _smooth_field(dirs, seed, terms=6):
     """Seeded smooth scalar field on the unit sphere in [-1, 1]: a sum of random-direction cosines."""
     u = Pcg32(seed, 0).uniform(terms * 4).reshape(terms, 4)

@@ -190,6 +190,13 @@ int dtrt_scene_set_density(dtrt_object *scene, int medium, const float *density,
  * sample is evaluated once and replayed from its random stream. Device pointers;
  * synchronous. No counterpart in the reference, whose forward mode needs one
  * tangent lane per parameter. */
+/* The same over several views (no counterpart in the reference): cameras/options as
+ * for dtrt_render_views (views share max_bounces and mode); d_images_device holds
+ * every view's d loss / d image value back to back ([h][w][3] per view, in view
+ * order). The gradient of all views is accumulated into d_density_device. */
+int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                              const dtrt_render_options *options, int medium, const float *d_images_device,
+                              float *d_density_device);
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
                         const float *d_image_device, float *d_density_device);
 

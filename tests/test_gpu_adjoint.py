@@ -127,3 +127,25 @@ def test_pyramid_weighted_adjoint_equals_adlossfunc_on_voxel_lanes(gpu):
     assert np.count_nonzero(np.abs(expect) > 1e-4) >= 8
     scale = np.abs(expect).max()
     assert np.allclose(got, expect, rtol=2e-3, atol=2e-4 * scale), np.c_[got, expect]
+
+
+def test_views_adjoint_equals_sum_of_single_view_adjoints(gpu):
+    """dtrt_render_views_adjoint over three uneven views equals the sum of the three
+    single-view adjoints with the same seeds and loss gradients."""
+    import torch
+    d = _density_scene(nder=1, res=64, crop=None, spp=4, grid_res=16)
+    sc = scenes.instantiate(gpu, d)
+    views = scenes.orbit_views(3, seed=3, res_choices=(32, 48, 64), spp_choices=(4, 8), radius=3.0)
+    cams = [gpu.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
+            for c2w, w, h, _ in views]
+    opts = [gpu.RenderOptions(40 + i, spp, d.max_bounces, 0, 0, True, INTERIOR) for i, (_, _, _, spp) in enumerate(views)]
+    rng = np.random.default_rng(5)
+    dimgs = [torch.tensor(rng.normal(size=(h, w, 3)), dtype=torch.float32, device="cuda") for _, w, h, _ in views]
+    grad = torch.zeros(d.media[0].density.shape, dtype=torch.float32, device="cuda")
+    gpu.render_views_adjoint(sc, cams, opts, 0, dimgs, grad)
+    want = torch.zeros_like(grad)
+    for i in range(3):
+        gpu.render_views_adjoint(sc, [cams[i]], [opts[i]], 0, [dimgs[i]], want)
+    g, w = grad.cpu().numpy(), want.cpu().numpy()
+    assert np.abs(w).max() > 0
+    assert np.allclose(g, w, rtol=1e-3, atol=1e-5 * np.abs(w).max())
