@@ -261,8 +261,39 @@ def glass_leg(iters=4):
             "loss_first_steps": losses, "reference_loss_first_steps": [820.2951, 721.77826, 631.98737, 548.87231, 474.86902]}
 
 
+def configs_leg():
+    """The other forward-mode scenes at their full sizes, one timed render each after a
+    warm-up: the surface scene (ground + icosphere, no media, 256^2, 64 spp, depth 5,
+    nder 1) and the dielectric scene (100,820-face displaced rough-dielectric sphere in a
+    128^3 fBm medium, 512^2, 256 spp, depth 12, nder 8)."""
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    rec = {}
+    for name, d in (("surface_icosphere_256sq_64spp_nder1", scenes.config0()),
+                    ("dielectric_sphere_in_medium_512sq_256spp_nder8", scenes.config2())):
+        sc = scenes.instantiate(G, d)
+        out = torch.zeros(d.out_shape(), device="cuda")
+        opts = G.RenderOptions(1, d.spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
+        G.render_device(0, sc, opts, out.data_ptr())
+        torch.cuda.synchronize()
+        opts.seed = 2
+        out.zero_()
+        t = time.perf_counter()
+        G.render_device(0, sc, opts, out.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        n = d.width * d.height * d.spp
+        st = G.render_stats(sc)
+        rec[name] = {"width": d.width, "height": d.height, "spp": d.spp, "max_bounces": d.max_bounces, "nder": d.nder,
+                     "faces": int(sum(len(s.indices) for s in d.shapes)), "s_per_render": dt,
+                     "ad_samples_per_s": n / dt, "children": int(st["children"]), "pe_samples": int(st["pe_samples"])}
+        del sc
+    return rec
+
+
 def inverse_leg(spp_divisor=1):
-    """One batched inverse-rendering step on the config[4] workload: 32 orbit views
+    """One batched inverse-rendering step: 32 orbit views
     (res 128^2..768^2, spp 16..256 drawn with PCG32 seed 5, each divided by
     spp_divisor) of a 1,003,520-face displaced sphere in a 256^3 fBm medium. Forward:
     one dtrt_render_views pass with 8 shape-deformation tangent lanes (all terms).
@@ -298,7 +329,8 @@ def inverse_leg(spp_divisor=1):
     G.render_views_adjoint(sc, cams, aopts, 0, dimgs, grad)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
-    return {"workload": "config[4]: 32 orbit views (res 128^2..768^2, spp 16..256 / %d, PCG32 seed 5) of a 1,003,520-face "
+    return {"workload": "batched inverse-rendering step: 32 orbit views (res 128^2..768^2, spp 16..256 / %d, PCG32 seed 5) "
+                        "of a 1,003,520-face "
                         "displaced sphere in a 256^3 fBm medium; forward nder 8 (shape lanes), all terms; adjoint over "
                         "16,777,216 node densities, loss sum(I^2)" % spp_divisor,
             "spp_divisor": spp_divisor, "samples": samples, "scene_generation_s": t_gen, "setup_s": t_setup,
@@ -319,8 +351,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
     ap.add_argument("--no-primal", action="store_true", help="skip the primal-integrator timing")
     ap.add_argument("--no-examples", action="store_true", help="skip the glass_1 optimisation-step timing")
+    ap.add_argument("--no-configs", action="store_true", help="skip the surface / dielectric scene timings")
     ap.add_argument("--inverse-spp-divisor", type=int, default=8,
-                    help="divide the config[4] views' spp by this in the batched inverse-step leg (0: skip the leg)")
+                    help="divide the batched inverse step's per-view spp by this (0: skip the leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -424,6 +457,8 @@ def main():
         rec["primal"] = primal_leg(args.res, args.spp)
     if not args.no_examples:
         rec["glass_1"] = glass_leg()
+    if not args.no_configs:
+        rec["other_scenes"] = configs_leg()
     if args.inverse_spp_divisor > 0:
         rec["inverse_step"] = inverse_leg(args.inverse_spp_divisor)
     if not args.no_cpu_baseline:
