@@ -413,14 +413,17 @@ def main():
     dom = int(np.argmax(ms))
     name = KERNELS[dom]
     achieved = alg[name] / (ms[dom] * 1e-3) / 1e9
-    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE profile of this workload
-    # (copy-calibrated); null when the workload differs from the profiled one
+    # HBM bytes per launch: the FETCH_SIZE / WRITE_SIZE profile of this workload gives the
+    # kernel's (copy-calibrated) HBM bytes per algorithmic byte over its timed launches;
+    # times this run's algorithmic bytes per launch. Null when the workload differs.
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "ad_render_heterogeneous_512x512_128spp_nder3_hbm_traffic.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        if (t["width"], t["height"], t["spp"], t["nder"]) == (desc.width, desc.height, desc.spp, desc.nder):
-            traffic = t.get("%s<%d>" % (name, desc.nder))
+        entry = t.get("%s<%d>" % (name, desc.nder))
+        ratio = entry.get("hbm_per_algorithmic_byte") if isinstance(entry, dict) else None
+        if (t["width"], t["height"], t["spp"], t["nder"]) == (desc.width, desc.height, desc.spp, desc.nder) and ratio:
+            traffic = ratio * alg[name] / max(1, launches[dom])
     rec = {
         "metric": "AD path samples/s at 1 GPU; adjoint vs forward grad time at equal variance",
         "value": value,
@@ -440,8 +443,10 @@ def main():
                    "sppe": desc.sppe, "primary_edge_samples_per_step": int(st["pe_samples"])},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_ms": ms[dom] / max(1, launches[dom]),
+                     "avg_launch_ms": ms[dom] / max(1, launches[dom]), "launches_timed": int(launches[dom]),
                      "algorithmic_bytes_per_launch": alg[name] / max(1, launches[dom])},
+        "launches_per_step": {k: int(launches[i]) // args.steps for i, k in enumerate(KERNELS)},
+        "algorithmic_bytes_per_step": {k: float(alg[k]) / args.steps for k in KERNELS if k in alg},
         "kernel_ms_per_step": {k: ms[i] / args.steps for i, k in enumerate(KERNELS)},
         "work_per_step": {k: [int(x) // args.steps for x in work[b:b + 4]]
                           for k, b in (("ad_paths", 0), ("children", 4), ("primary_edges", 8))},
