@@ -49,17 +49,18 @@ def test_bvh_closest_hit_parity(gpu, oracle, which):
         brute = False
     sg = scenes.instantiate(gpu, d)
     so = scenes.instantiate(oracle, d)
-    fg, tg = gpu.trace(sg, rays.astype(np.float32))
-    sh, tr, to = oracle.intersect(so, rays, brute=brute)
+    r32 = rays.astype(np.float32)
+    fg, tg = gpu.trace(sg, r32)
+    # the oracle intersects the same (fp32-rounded) rays in double: what remains is the
+    # device's fp32 traversal and refinement, not the rounding of the input rays
+    sh, tr, to = oracle.intersect(so, r32.astype(np.float64), brute=brute)
     fo = _global_face(d, sh, tr)
     assert (fg == fo).mean() >= 0.9999
     both = (fg == fo) & (fo >= 0)
     assert both.sum() > 1000
     rel = np.abs(tg[both] - to[both]) / np.maximum(to[both], 1e-3)
-    # fp32 Moller-Trumbore: relative t error ~ eps32 * condition (grazing hits on
-    # the 40K-face sphere reach a few 1e-5)
     assert np.median(rel) <= 1e-6
-    assert np.quantile(rel, 0.9999) <= 1e-4
+    assert rel.max() <= 1e-5
 
 
 def test_grid_lookup_parity(gpu, oracle):
