@@ -23,9 +23,24 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-munsafe-fp-at
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include")]
 
 
-def _compile(src):
-    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+# kernels.hip is compiled once per dual width N (its AD, child, primary-edge,
+# adjoint and finalize kernels; each object holds its own frame-rotation constant)
+# and once without DT_N (the width-independent kernels and the dispatchers).
+WIDTHS = (16, 8, 4, 3, 2, 1)   # widest first: the longest compiles start first
+
+
+def _jobs():
+    kern = os.path.join(SRC, "kernels.hip")
+    jobs = [(kern, "kernels_n%d" % n, ["-DDT_N=%d" % n, "-DDT_FRAME_ROT=c_frame_rot_n%d" % n]) for n in WIDTHS]
+    jobs.append((kern, "kernels", []))
+    jobs.append((os.path.join(SRC, "host.cpp"), "host", []))
+    return jobs
+
+
+def _compile(job):
+    src, name, defs = job
+    obj = os.path.join(OBJ, name + ".o")
+    cmd = [HIPCC] + FLAGS + defs + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):   # host code: HIP runtime API only, no device pass
         cmd[1:1] = ["-x", "hip", "--offload-host-only"]
     subprocess.run(cmd, check=True)
@@ -35,9 +50,9 @@ def _compile(src):
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(SRC, f) for f in ("kernels.hip", "host.cpp")]
-    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
-        objs = list(ex.map(_compile, srcs))
+    jobs = _jobs()
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs, check=True)
     os.replace(tmp, LIB)

@@ -190,7 +190,11 @@ struct Rng {
 // utils.cpp:54-67 — orthonormal basis built after a fixed rotation of the
 // normal (0.1 rad about normalize(0.1,0.2,0.3)); R computed by Rodrigues on the host.
 struct FrameRot { float m[9]; };
-extern __constant__ FrameRot c_frame_rot;
+// Each kernel object defines its own copy under its own name.
+#ifndef DT_FRAME_ROT
+#define DT_FRAME_ROT c_frame_rot
+#endif
+extern __constant__ FrameRot DT_FRAME_ROT;
 
 DI f3 rotMul(const float *m, f3 v, bool tr) {
     return tr ? f3{m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z, m[2] * v.x + m[5] * v.y + m[8] * v.z}
@@ -213,7 +217,7 @@ struct Frame {
     f3 s, t, n;
     DI Frame() {}
     DI explicit Frame(f3 nn) : n(nn) {
-        const float *R = c_frame_rot.m;
+        const float *R = DT_FRAME_ROT.m;
         f3 q = rotMul(R, nn, false);
         if (fabsf(q.x) > fabsf(q.y)) {
             float il = 1.0f / sqrtf(q.x * q.x + q.z * q.z);
@@ -232,7 +236,7 @@ template <int N> struct FrameAD {   // frameAD.h:9-82 via coordinateSystemAD (ut
     Dv<N> s, t, n;
     DI FrameAD() {}
     DI explicit FrameAD(const Dv<N> &nn) : n(nn) {
-        const float *R = c_frame_rot.m;
+        const float *R = DT_FRAME_ROT.m;
         Dv<N> q = rotMul(R, nn, false);
         if (fabsf(q.x.v) > fabsf(q.y.v)) {
             Df<N> il = 1.0f / dsqrt(q.x * q.x + q.z * q.z);

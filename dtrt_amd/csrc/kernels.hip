@@ -14,9 +14,14 @@
 
 #include <algorithm>
 
+// The file is compiled once per dual width (-DDT_N=1,2,3,4,8,16: that width's AD,
+// child, primary-edge, adjoint and finalize kernels and their launchers) and once
+// without DT_N (the width-independent kernels and the dispatchers over S.nder), so
+// the seven code objects build in parallel. Each object holds its own copy of the
+// frame rotation constant, named by DT_FRAME_ROT; setFrameRotation fills all seven.
 namespace dt {
 
-__constant__ FrameRot c_frame_rot;
+__constant__ FrameRot DT_FRAME_ROT;
 
 constexpr int BLOCK = TRACE_BLOCK;
 // Minimum resident waves per SIMD requested from the register allocator (HIP's second
@@ -896,6 +901,7 @@ DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r,
     }
 }
 
+#ifndef DT_N
 // ParticleTracer::render (ptracer.cpp:10-55): particles s0 .. s0+n-1, each its own
 // stream keyed (seed, particle index), splatted into acc[pixel][3] of view 0's window.
 __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_particles(DScene S, RenderParams P, uint64_t s0,
@@ -915,6 +921,7 @@ __global__ void k_scale(const float *acc, float *out, uint64_t n, float scale) {
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) out[i] = acc[i] * scale;
 }
+#endif  // DT_N
 
 // Path-replay adjoint of the density derivative of the interior terms: for each AD
 // sample a TOTAL pass computes its radiance, then a SCATTER pass replays the same
@@ -959,6 +966,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     flushWork(work, 0, T.nbox, T.ntri, nlk, T.nseg);
 }
 
+#ifndef DT_N
 // Graph-safe clears and counters (captured graphs use kernels, not memset nodes).
 __global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -980,6 +988,7 @@ __global__ void k_load_density(float *dst, const float *src, uint64_t n, unsigne
     for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, off));
     if ((threadIdx.x & 63) == 0) atomicMax(maxbits, b);
 }
+#endif  // DT_N
 
 // Adds the finished sub-paths of this wave: lanes with fin set add W * Lv to their
 // record's pixel. Lanes that share the first finishing lane's pixel and image are
@@ -1274,6 +1283,7 @@ __global__ void k_finalize(RenderParams P, int npix, int pix0, int spp, int has_
 }
 
 // ---------------------------------------------------------------- test hooks
+#ifndef DT_N
 __global__ void k_trace(DScene S, const float *rays, int n, int *out_face, float *out_t) {
     Tracer<1> T(S);
     int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -1311,6 +1321,11 @@ __global__ void k_medium(DScene S, int med, int kind, const float *seg, int n, u
         out[i] = sampleDistance(m, o, d, s[7], a, b, r, ps, thr) ? 1.f : 0.f;
     }
 }
+__global__ void k_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t *out) {
+    u4 o = philox4x32_10(u4{c0, c1, c2, c3}, k0, k1);
+    out[0] = o.a; out[1] = o.b; out[2] = o.c; out[3] = o.d;
+}
+#endif  // DT_N
 template <int N> __global__ void k_grid(DScene S, int med, const float *pts, int n, float *out) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -1321,17 +1336,33 @@ template <int N> __global__ void k_grid(DScene S, int med, const float *pts, int
     out[(size_t)i * (1 + N)] = v.v;
     for (int j = 0; j < N; ++j) out[(size_t)i * (1 + N) + 1 + j] = v.d[j];
 }
-__global__ void k_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t *out) {
-    u4 o = philox4x32_10(u4{c0, c1, c2, c3}, k0, k1);
-    out[0] = o.a; out[1] = o.b; out[2] = o.c; out[3] = o.d;
-}
 
 // ------------------------------------------------------------------ launchers
 #define DT_FOR_N(X) X(1) X(2) X(3) X(4) X(8) X(16)
 
-hipError_t setFrameRotation(const float *m9) { return hipMemcpyToSymbol(HIP_SYMBOL(c_frame_rot), m9, sizeof(FrameRot)); }
+// Launchers of one dual width: defined and instantiated in that width's object,
+// called through the dispatchers of the width-independent one.
+template <int N> struct WidthOps {
+    static hipError_t setFrameRotation(const float *m9);
+    static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
+                         char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *wk,
+                         hipStream_t sm);
+    static void launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc,
+                               float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm);
+    static void launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total,
+                              char *q, unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm);
+    static void launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
+                               const float *acc_pe, float *out, unsigned *st, hipStream_t sm);
+    static void launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);
+    static void launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,
+                              float *grad, int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm);
+};
 
-template <int N> static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,
+#ifdef DT_N
+template <int N> hipError_t WidthOps<N>::setFrameRotation(const float *m9) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(DT_FRAME_ROT), m9, sizeof(FrameRot));
+}
+template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,
                                       float *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,
                                       unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
@@ -1345,38 +1376,56 @@ template <int N> static void launchAd(const DScene &S, const RenderParams &P, ui
         hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st,
                            ovf, ovf_count, wk);
 }
-template <int N> static void launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
+template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
                                             float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
     if (P.ad_kind == AD_VOLPATH)
         hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
     else
         hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
 }
-template <int N> static void launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,
+template <int N> void WidthOps<N>::launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,
                                            uint64_t total, char *q, unsigned *qc, unsigned *st, unsigned long long *wk,
                                            hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, view, s0, n, total, q, qc, st, wk);
 }
-template <int N> static void launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
+template <int N> void WidthOps<N>::launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
                                             const float *acc_pe, float *out, unsigned *st, hipStream_t sm) {
     hipLaunchKernelGGL(k_finalize<N>, dim3((npix + 255) / 256), dim3(256), 0, sm, P, npix, pix0, spp, has_pe, acc, acc_pe,
                        out, st);
 }
-template <int N> static void launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
+template <int N> void WidthOps<N>::launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
     hipLaunchKernelGGL(k_grid<N>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, med, pts, n, out);
 }
+template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n,
+                                                 const float *dimg, float *grad, int pmed, unsigned *st,
+                                                 unsigned long long *wk, hipStream_t sm) {
+    unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, dimg, grad, pmed, st, wk);
+}
+template struct WidthOps<DT_N>;
 
-#define DT_DISPATCH(FN, ...)                          \
-    switch (S.nder) {                                 \
-        case 1: FN<1>(__VA_ARGS__); break;            \
-        case 2: FN<2>(__VA_ARGS__); break;            \
-        case 3: FN<3>(__VA_ARGS__); break;            \
-        case 4: FN<4>(__VA_ARGS__); break;            \
-        case 8: FN<8>(__VA_ARGS__); break;            \
-        case 16: FN<16>(__VA_ARGS__); break;          \
-        default: return hipErrorInvalidValue;         \
-    }                                                 \
+#else  // the width-independent object: dispatchers and the remaining launchers
+
+hipError_t setFrameRotation(const float *m9) {
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(DT_FRAME_ROT), m9, sizeof(FrameRot));
+#define DT_SET_ROT(n) \
+    if (e == hipSuccess) e = WidthOps<n>::setFrameRotation(m9);
+    DT_FOR_N(DT_SET_ROT)
+#undef DT_SET_ROT
+    return e;
+}
+
+#define DT_DISPATCH(FN, ...)                              \
+    switch (S.nder) {                                     \
+        case 1: WidthOps<1>::FN(__VA_ARGS__); break;      \
+        case 2: WidthOps<2>::FN(__VA_ARGS__); break;      \
+        case 3: WidthOps<3>::FN(__VA_ARGS__); break;      \
+        case 4: WidthOps<4>::FN(__VA_ARGS__); break;      \
+        case 8: WidthOps<8>::FN(__VA_ARGS__); break;      \
+        case 16: WidthOps<16>::FN(__VA_ARGS__); break;    \
+        default: return hipErrorInvalidValue;             \
+    }                                                     \
     return hipGetLastError();
 
 hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
@@ -1391,11 +1440,6 @@ hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q
 hipError_t launch_primary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total, char *q,
                           unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
     DT_DISPATCH(launchPrimary, S, P, view, s0, n, total, q, qc, st, wk, sm)
-}
-template <int N> static void launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,
-                                           float *grad, int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm) {
-    unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, dimg, grad, pmed, st, wk);
 }
 hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,
                           int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm) {
@@ -1464,5 +1508,6 @@ hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out
     hipLaunchKernelGGL(k_philox, dim3(1), dim3(1), 0, sm, ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
     return hipGetLastError();
 }
+#endif  // DT_N
 
 }  // namespace dt
