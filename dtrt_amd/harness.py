@@ -15,8 +15,10 @@ multi-scale pyramid dot product of its tangent image with the image gradient (wi
 clamping and the out-of-range penalty), computed on the device through
 :func:`dtrt_amd.adjoint.pyramid_weights`.
 
-Not provided: ``check_range=True`` (variable ranges; every example of the reference
-uses the default, unbounded ranges).
+``check_range=True`` bounds every variable by the interval in which its material and
+medium values stay valid (scene_manager.py:69-143, 416-437); :meth:`set_arguments`
+clamps out-of-range variables, leaves their tangent inputs at zero and returns the
+quadratic penalty gradient (scene_manager.py:439-454).
 """
 import copy
 import math
@@ -99,6 +101,70 @@ class SceneTransform:
             self.emitter_type, self.emitter_id = args[0], int(args[1])
             self.d_intensity, self.d_kappa = _vec(args[2]), float(args[3])
 
+    def determine_range(self, init_val, tmin, tmax):
+        """Narrows [tmin, tmax] to the variable values that keep this transform's
+        parameters valid (scene_manager.py:69-143): reflectance and albedo channels in
+        [0, 1], non-negative phong lobes and exponent, roughdielectric alpha in
+        [0.01, 0.1] and a non-negative IOR, non-negative sigma_t and density scale.
+        ``init_val`` are the initial parameter values the reference passes. Two of its
+        updates bound tmax by tmin (a negative phong exponent rate, a positive alpha
+        rate); they are kept as they are."""
+        def channels(val, rate, tmin, tmax):   # val + rate * t in [0, 1], per channel
+            for ch in range(3):
+                if rate[ch] == 0.0:
+                    continue
+                a = -val[ch] / rate[ch]
+                b = (1.0 - val[ch]) / rate[ch]
+                tmin = max(tmin, a if rate[ch] > 0 else b)
+                tmax = min(tmax, b if rate[ch] > 0 else a)
+            return tmin, tmax
+
+        def nonneg(val, rate, tmin, tmax):     # val + rate * t >= 0
+            if rate > 0.0:
+                tmin = max(tmin, -val / rate)
+            elif rate < 0.0:
+                tmax = min(tmax, -val / rate)
+            return tmin, tmax
+
+        if self.type_name == "BSDF_VARY":
+            if self.bsdf_type == "diffuse":
+                tmin, tmax = channels(init_val, self.d_reflectance, tmin, tmax)
+            elif self.bsdf_type == "phong":
+                diffuse, specular, exponent = init_val
+                for ch in range(3):
+                    tmin, tmax = nonneg(diffuse[ch], self.d_diffuse[ch], tmin, tmax)
+                    tmin, tmax = nonneg(specular[ch], self.d_specular[ch], tmin, tmax)
+                if self.d_exponent > 0.0:
+                    tmin = max(tmin, -exponent / self.d_exponent)
+                elif self.d_exponent < 0.0:
+                    tmax = min(tmin, -exponent / self.d_exponent)
+            elif self.bsdf_type == "roughdielectric":
+                a = (0.01 - init_val[0]) / self.d_alpha if self.d_alpha != 0.0 else 0.0
+                b = (0.1 - init_val[0]) / self.d_alpha if self.d_alpha != 0.0 else 0.0
+                if self.d_alpha > 0.0:
+                    tmin = max(tmin, a)
+                    tmax = min(tmin, b)
+                elif self.d_alpha < 0.0:
+                    tmin = max(tmin, b)
+                    tmax = min(tmax, a)
+                tmin, tmax = nonneg(init_val[1], self.d_eta, tmin, tmax)
+        elif self.type_name == "MEDIUM_VARY":
+            if self.med_type == "homogeneous":
+                sigma_t, albedo = init_val
+                tmin, tmax = channels(albedo, self.d_albedo, tmin, tmax)
+                tmin, tmax = nonneg(sigma_t, self.d_sigmaT, tmin, tmax)
+            else:
+                albedo, scale = init_val
+                if isinstance(albedo, str):
+                    if np.any(self.d_albedo != 0.0):
+                        raise ValueError("no range for a varying albedo read from a volume file")
+                else:
+                    tmin, tmax = channels(albedo, self.d_albedo, tmin, tmax)
+                tmin, tmax = nonneg(scale, self.d_scalar, tmin, tmax)
+        if not tmax > tmin:
+            raise ValueError("empty variable range (%g, %g)" % (tmin, tmax))
+        return tmin, tmax
+
 
 def _volume_box(m):
     """World AABB centre of a heterogeneous medium's grid (scene_manager.py:276-292)."""
@@ -125,16 +191,41 @@ class SceneManager:
     number of transform sets."""
 
     def __init__(self, desc, transforms, check_range=False):
-        if check_range:
-            raise NotImplementedError("check_range (bounded variables) is not provided")
         if desc.nder != len(transforms):
             raise ValueError("one transform set per derivative lane (nder = %d)" % desc.nder)
         self.desc0 = copy.deepcopy(desc)
         self.transforms = transforms
         self.shape_centers_0 = [s.vertices.astype(np.float64).sum(0) / len(s.vertices) for s in desc.shapes]
         self.volume_centers_0 = [None if m.kind == "homogeneous" else _volume_box(m) for m in desc.media]
-        self.ranges = [[-math.inf, math.inf] for _ in range(desc.nder)]
+        self.ranges = self.determine_range() if check_range else [[-math.inf, math.inf] for _ in range(desc.nder)]
         self.set_arguments(np.zeros(desc.nder))
+
+    def determine_range(self):
+        """Per variable, the intersection of its transforms' valid intervals at the
+        initial parameter values (scene_manager.py:416-437)."""
+        ranges = []
+        for iv, tset in enumerate(self.transforms):
+            tmin, tmax = -math.inf, math.inf
+            for T in tset:
+                if T.type_name == "BSDF_VARY":
+                    b = self.desc0.bsdfs[T.bsdf_id]
+                    if T.bsdf_type == "diffuse":
+                        init = np.asarray(b[1], np.float64)
+                    elif T.bsdf_type == "phong":
+                        init = (np.asarray(b[1], np.float64), np.asarray(b[2], np.float64), float(b[3]))
+                    else:
+                        init = (float(b[1]), float(b[2]))
+                    tmin, tmax = T.determine_range(init, tmin, tmax)
+                elif T.type_name == "MEDIUM_VARY":
+                    m = self.desc0.media[T.med_id]
+                    alb = m.albedo if isinstance(m.albedo, str) else np.asarray(m.albedo, np.float64)
+                    if T.med_type == "homogeneous":
+                        tmin, tmax = T.determine_range((float(m.sigma_t), alb), tmin, tmax)
+                    else:
+                        tmin, tmax = T.determine_range((alb, float(m.scale)), tmin, tmax)
+            ranges.append([tmin, tmax])
+            print("param #%d range = (%.2f, %.2f)" % (iv, tmin, tmax))
+        return ranges
 
     def reset(self):
         self.set_arguments(np.zeros(self.desc0.nder))
@@ -144,9 +235,21 @@ class SceneManager:
 
     def set_arguments(self, var_list):
         """Rebuild the current description for ``var_list``; returns the out-of-range
-        gradient vector (zero with unbounded ranges), as the reference does."""
+        gradient vector (scene_manager.py:439-454): a variable outside its range is
+        clamped to the nearest bound, gets no tangent inputs, and its entry is
+        2 (var - bound)(range width)."""
         nder = self.desc0.nder
         var = np.asarray([float(v) for v in var_list], np.float64)
+        var_grad = np.zeros(nder, np.float32)
+        clamped = [False] * nder
+        for iv in range(nder):
+            lo, hi = self.ranges[iv]
+            if var[iv] < lo or var[iv] > hi:
+                clamped[iv] = True
+                bound = lo if var[iv] < lo else hi
+                var_grad[iv] = 2.0 * (var[iv] - bound) * (hi - lo)
+                print("Variable #%d clamped from %.2f to %.2f" % (iv, var[iv], bound))
+                var[iv] = bound
         d = copy.deepcopy(self.desc0)
         centers = [c.copy() for c in self.shape_centers_0]
         centers_vol = [None if c is None else c.copy() for c in self.volume_centers_0]
@@ -233,7 +336,7 @@ class SceneManager:
                         raise ValueError("emitter %d is not an AreaLightEx" % T.emitter_id)
                     l[2] = tuple(np.asarray(l[2]) + T.d_intensity * t)
                     l[3] = float(l[3]) + T.d_kappa * t
-            for T in tset:   # tangent inputs of lane iv, from the updated values (:564-631)
+            for T in (() if clamped[iv] else tset):   # tangent inputs of lane iv, from the updated values (:556-631)
                 n = T.type_name
                 if n == "CAMERA_TRANSLATE":
                     cam_vel[0, iv] += T.vec_vel
@@ -290,7 +393,7 @@ class SceneManager:
         d.lights = [tuple(l) for l in lights]
         d.phases = [tuple(p) for p in phases]
         self.desc = d
-        return np.zeros(nder, np.float32)
+        return var_grad
 
     def scene(self, mod):
         """Renderer objects of the current description (instantiated through ``mod``)."""

@@ -143,3 +143,51 @@ def test_rotation3_matches_rodrigues():
     R = rotation3([0, 0, 2.0], np.pi / 4)          # |axis| * t = pi/2 about z
     assert np.allclose(R @ [1, 0, 0], [0, 1, 0], atol=1e-12)
     assert np.allclose(rotation3([0, 0, 1e-5], 3.0), np.eye(3))
+
+
+RANGED = [[SceneTransform("BSDF_VARY", "diffuse", 2, [0.1, -0.4, 0.0])],
+          [SceneTransform("MEDIUM_VARY", "homogeneous", 0, -1.4, [0.5, 0, 0])],
+          [SceneTransform("MEDIUM_VARY", "heterogeneous", 1, [0, 0, 0], [0, 0, 0], [0, 0, -0.2], 0.5)],
+          [SceneTransform("BSDF_VARY", "roughdielectric", 1, -0.05, 0.1)]]
+
+
+def test_check_range_intervals():
+    # Hand-derived (scene_manager.py:69-143, 416-437):
+    #  var 0: reflectance (0.1, 0.2, 0.3) + t (0.1, -0.4, 0) in [0, 1]  -> t in [-1, 0.5]
+    #  var 1: albedo 0.5 + 0.5 t in [0, 1] -> [-1, 1]; sigma_t 0.7 - 1.4 t >= 0 -> t <= 0.5
+    #  var 2: albedo 0.8 - 0.2 t in [0, 1] -> [-1, 4]; scale 2 + 0.5 t >= 0 -> t >= -4
+    #  var 3: alpha 0.2 - 0.05 t in [0.01, 0.1] -> [2, 3.8]; IOR 1.5 + 0.1 t >= 0 -> t >= -15
+    sm = SceneManager(_desc(4), RANGED, check_range=True)
+    assert np.allclose(sm.ranges, [[-1, 0.5], [-1, 0.5], [-1, 4], [2, 3.8]])
+
+
+def test_check_range_reference_quirks_give_empty_ranges():
+    # A negative phong exponent rate and a positive alpha rate bound tmax by tmin in
+    # the reference, so their ranges are always empty (its assertion fails).
+    for T in (SceneTransform("BSDF_VARY", "phong", 0, [0, 0, 0], [0, 0, 0], -2.0),
+              SceneTransform("BSDF_VARY", "roughdielectric", 1, 0.05, 0.0)):
+        with pytest.raises(ValueError):
+            SceneManager(_desc(1), [[T]], check_range=True)
+    # the same transforms with the other sign are bounded on one side only
+    sm = SceneManager(_desc(1), [[SceneTransform("BSDF_VARY", "phong", 0, [0, 0, 0], [0, 0, 0], 2.0)]], check_range=True)
+    assert sm.ranges == [[-10.0, np.inf]]
+
+
+def test_check_range_clamps_values_drops_tangents_and_returns_the_penalty():
+    sm = SceneManager(_desc(4), RANGED, check_range=True)
+    g = sm.set_arguments([0.7, -3.0, 0.0, 2.5])
+    # clamped: var 0 to 0.5 (width 1.5), var 1 to -1 (width 1.5); 2 (var - bound) width
+    assert np.allclose(g, [2 * 0.2 * 1.5, 2 * -2.0 * 1.5, 0.0, 0.0])
+    d = sm.desc
+    assert np.allclose(d.bsdfs[2][1], [0.15, 0.0, 0.3])
+    assert len(d.bsdfs[2]) == 2 or not np.any(d.bsdfs[2][2])          # no reflectance tangent
+    m0, m1 = d.media
+    assert m0.sigma_t == pytest.approx(0.7 + 1.4) and np.allclose(m0.albedo, [0.0, 0.6, 0.7])
+    assert m0.d_scale is None or not np.any(m0.d_scale[1])
+    assert np.allclose(m1.d_albedo[2], [0, 0, -0.2]) and m1.d_scale[2] == pytest.approx(0.5)
+    assert d.bsdfs[1][1] == pytest.approx(0.2 - 0.05 * 2.5) and d.bsdfs[1][4][3] == pytest.approx(-0.05)
+    # inside every range nothing is clamped and every lane has its tangents
+    g = sm.set_arguments([0.2, 0.1, 1.0, 3.0])
+    assert not np.any(g) and d is not sm.desc
+    assert np.allclose(sm.desc.bsdfs[2][2][0], [0.1, -0.4, 0.0])
+    assert sm.desc.media[0].d_scale[1] == pytest.approx(-1.4)
