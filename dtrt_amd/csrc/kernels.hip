@@ -35,6 +35,11 @@ constexpr int BLOCK = TRACE_BLOCK;
 // Wide duals (N >= 8 tangent lanes) take the whole 512-entry register file at one
 // wave per SIMD rather than spilling kilobytes per lane to scratch.
 template <int N> constexpr int adWaves() { return N >= 8 ? 1 : AD_WAVES_PER_SIMD; }
+// Grid of k_ad_refill: waves per CU sharing a launch's samples (measured on the
+// 100K-face dielectric scene at nder 8: 64 beats 32 and 128).
+#ifndef AD_REFILL_WAVES_PER_CU
+#define AD_REFILL_WAVES_PER_CU 64
+#endif
 
 DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
     // Blocks are dealt round-robin over the 8 XCDs; give each XCD a contiguous run
@@ -445,206 +450,31 @@ struct AdjCtx {
 // position, then the dual-number estimator of volpathAD.cpp:145-620), accumulated
 // into L. A == SCATTER replays the path of a TOTAL pass with the same stream and adds
 // the reverse-mode density derivative of every contribution into ac->grad.
-template <int N, AdjPass A>
-DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, Ds<N> &L, Pusher &u,
-                     unsigned *stats, const AdjCtx *ac) {
-    const DScene &S = T.S;
+// Camera ray (with tangents) through the jittered position of pixel `pixel`.
+template <int N> DI void adCameraRay(const DView &vw, Rng &r, int pixel, Dv<N> &ro, Dv<N> &rd) {
     const DCamera &cam = vw.cam;
     int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
     int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
     float jx = 0.5f, jy = 0.5f;
     if (vw.sppe >= 0) { jx = r.next(); jy = r.next(); }
-    Dv<N> ro, rd;
     cameraRayAD<N>(cam, ix + jx, iy + jy, ro, rd);
+}
+
+template <int N, AdjPass A>
+DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, Ds<N> &L, Pusher &u,
+                     unsigned *stats, const AdjCtx *ac) {
+    const DScene &S = T.S;
+    const DCamera &cam = vw.cam;
+    Dv<N> ro, rd;
+    adCameraRay<N>(vw, r, pixel, ro, rd);
     HitAD<N> its;
     if (T.rayIntersectAD(ro, rd, false, its)) {
         int med = cam.med_id, depth = P.max_bounces, mode = P.mode;
         Ds<N> beta(f3{1, 1, 1});
         while (mode != 0) {
-            float r1a = r.next(), r1b = r.next();
-            float r2[4] = {r.next(), r.next(), r.next(), r.next()};
-            float r3[3] = {r.next(), r.next(), r.next()};
-            Ds<N> thr(f3{1, 1, 1});
-            bool vol = false;
-            if (med >= 0) {
-                Dv<N> ps;
-                ScatterCtx fs{nullptr, nullptr, 0.f};
-                const ScatterCtx *fsp = nullptr;
-                if constexpr (A == AdjPass::SCATTER) {   // free flight weighs everything from here on
-                    fs = ScatterCtx{ac->grad, ac->pm, dot(ac->adj, ac->Ltot - L.val())};
-                    fsp = &fs;
-                }
-                vol = sampleDistanceAD<N>(S.media[med], ro, rd, its.t, r1a, r1b, r, ps, thr, fsp);
-                if (vol) ro = ps;
-            }
-            if (vol) {
-                if (depth == 0 || thr.isZero(EPS)) break;
-                const DPhase &ph = S.phases[S.media[med].phase_id];
-                f3 rdin = rd.val();
-                if (mode & 8) {   // volume NEE with tangents
-                    f3 wo;
-                    float pdf_nee;
-                    Rng rs;   // stream position before the walk (scatter pass only)
-                    if constexpr (A == AdjPass::SCATTER) rs = r;
-                    Ds<N> value = T.neePointAD(ro, r2, r, med, depth - 1, wo, pdf_nee);
-                    if (!value.isZero(EPS)) {
-                        Df<N> pv = phaseEvalAD<N>(ph, -rd, wo);
-                        if (pv.v > EPS) {
-                            Ds<N> c = beta * (thr * value * (pv * misPower(pdf_nee, phaseEval(ph, -rdin, wo))));
-                            L += c;
-                            if constexpr (A == AdjPass::SCATTER) {   // the shadow walk weighs this term only
-                                ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
-                                Rng r2w = rs;
-                                f3 wo2;
-                                float pdf2;
-                                T.neePointAD(ro, r2, r2w, med, depth - 1, wo2, pdf2, &sc);
-                            }
-                        }
-                    }
-                }
-                f3 wo = phaseSample(ph, -rdin, r3[0], r3[1]);
-                Df<N> pv = phaseSampleWeightAD<N>(ph, -rd, wo);
-                bool cont = false;
-                HitAD<N> its1;
-                Ds<N> thr1;
-                if (pv.v > EPS) {
-                    thr1 = thr * pv;
-                    float pdf_nee = 0.f;
-                    Rng rs;   // stream position before the walk (scatter pass only)
-                    if constexpr (A == AdjPass::SCATTER) rs = r;
-                    Ds<N> att = T.lookForEmitterAD(ro, Dv<N>(wo), false, r, med, depth - 1, its1, pdf_nee, nullptr, nullptr);
-                    if (!att.isZero(EPS) && (mode & 8)) {
-                        Ds<N> c = beta * (thr1 * att * misPower(phaseEval(ph, -wo, wo), pdf_nee));
-                        L += c;
-                        if constexpr (A == AdjPass::SCATTER) {
-                            ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
-                            Rng r2w = rs;
-                            HitAD<N> h2;
-                            float pdf2;
-                            T.lookForEmitterAD(ro, Dv<N>(wo), false, r2w, med, depth - 1, h2, pdf2, nullptr, nullptr, &sc);
-                        }
-                    }
-                    cont = true;
-                }
-                if (P.sppse > 0 && (mode & 32))
-                    edgeTerm<N>(T, P, r, true, ro, nullptr, nullptr, &ph, rdin, thr.val(), beta.val(), med, depth, pixel, u,
-                                stats);
-                if (!cont) break;
-                beta = beta * thr1;
-                rd = Dv<N>(wo);
-                its = its1;
-                depth -= 1;
-                mode &= ~1;
-            } else {
-                if (its.face < 0 || thr.isZero(EPS)) break;
-                const DShape &sh = shapeOf(S, its.face);
-                if (isTransition(sh) && targetMedium(sh, its.gn.val(), -rd.val()) != med) {
-                    atomicAdd(stats + STAT_MED_INCONSISTENT, 1u);
-                    break;
-                }
-                if (sh.light_id >= 0 && (mode & 1)) L += beta * (thr * emitterEvalAD<N>(S.emitters[sh.light_id], its, -rd));
-                if (depth == 0) break;
-                const DBsdf &b = bsdfOf(S, sh);
-                const bool isnull = b.type == BSDF_NULL;
-                Hit its0 = its.val();
-                // Lins boundary term (volpathAD.cpp:321-467): queued as primal sub-paths
-                if (med >= 0 && !its.t.gradZero(EPS) && (mode & 16)) {
-                    int ns = min(max((int)(P.ddist * its.t.gradAbsMax()), 1), 100);
-                    f3 n = dot(its0.gn, rd.val()) < 0.f ? -its0.gn : its0.gn;
-                    f3 sgs = sigS(S.media[med], its0.p - n * SHADOW_EPS);
-                    f3 base = beta.val() * thr.val() * sgs * (1.0f / ns);
-                    float W[N * 3];
-#pragma unroll
-                    for (int j = 0; j < N; ++j) { W[3 * j] = base.x * its.t.d[j]; W[3 * j + 1] = base.y * its.t.d[j]; W[3 * j + 2] = base.z * its.t.d[j]; }
-                    ChildHdr h;
-                    h.org[0] = its0.p.x; h.org[1] = its0.p.y; h.org[2] = its0.p.z;
-                    f3 rv = rd.val();
-                    h.dir[0] = rv.x; h.dir[1] = rv.y; h.dir[2] = rv.z;
-                    h.meta = (isnull ? CHILD_LINS_NULL : CHILD_LINS_SURF) | (depth << 4) | ((med + 1) << 12);
-                    h.face = its0.face;
-                    h.gn[0] = its0.gn.x; h.gn[1] = its0.gn.y; h.gn[2] = its0.gn.z;
-                    h.sn[0] = its0.sn.x; h.sn[1] = its0.sn.y; h.sn[2] = its0.sn.z;
-                    for (int s = 0; s < ns; ++s) push(P, u, h, W, N * 3);
-                }
-                FrameAD<N> shf(its.sn);
-                if (!isnull && (mode & 2)) {   // surface NEE with tangents
-                    Dv<N> woL;
-                    float pdf_nee;
-                    Rng rs;   // stream position before the walk (scatter pass only)
-                    if constexpr (A == AdjPass::SCATTER) rs = r;
-                    Ds<N> value = T.neeSurfaceAD(its, shf, r2, r, med, depth - 1, woL, pdf_nee);
-                    if (!value.isZero(EPS)) {
-                        Ds<N> bv = bsdfEvalAD<N>(b, its.wi, woL);
-                        Ds<N> c = beta * (thr * value * bv * misPower(pdf_nee, bsdfPdf(b, its0.wi, woL.val())));
-                        L += c;
-                        if constexpr (A == AdjPass::SCATTER) {
-                            ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
-                            Rng r2w = rs;
-                            Dv<N> wo2;
-                            float pdf2;
-                            T.neeSurfaceAD(its, shf, r2, r2w, med, depth - 1, wo2, pdf2, &sc);
-                        }
-                    }
-                }
-                // BSDF sampling (bsdf.cpp:11-21; null.cpp:18-23)
-                bool cont = false;
-                Dv<N> nd;
-                Ds<N> bw(f3{1, 1, 1});
-                float bpdf = 1.f;
-                if (isnull) {
-                    nd = rd;   // toWorld(-wi) == incoming direction, tangents included
-                    cont = true;
-                } else {
-                    f3 wo0;
-                    f3 w0 = bsdfSample(b, its0.wi, r3[0], r3[1], r3[2], wo0, bpdf);
-                    if (!allZero(w0, EPS) && bpdf > EPS) {
-                        Frame shv = shf.val();
-                        f3 ww = shv.toWorld(wo0);
-                        Dv<N> woL = shf.toLocal(Dv<N>(ww));
-                        bw = bsdfEvalAD<N>(b, its.wi, woL) * (1.0f / bpdf);
-                        if (!bw.isZero(EPS)) { nd = Dv<N>(ww); cont = true; }
-                    }
-                }
-                HitAD<N> its1;
-                Ds<N> thr1 = thr;
-                int med1 = med, mode1 = mode;
-                Dv<N> no = its.p;
-                if (cont) {
-                    med1 = isTransition(sh) ? targetMedium(sh, its0.gn, nd.val()) : med;
-                    if (isnull) {
-                        T.rayIntersectAD(no, nd, true, its1);
-                    } else {
-                        float pdf_nee = 0.f;
-                        Rng rs;   // stream position before the walk (scatter pass only)
-                        if constexpr (A == AdjPass::SCATTER) rs = r;
-                        Ds<N> att = T.lookForEmitterAD(no, nd, true, r, med1, depth - 1, its1, pdf_nee, nullptr, nullptr);
-                        thr1 = thr * bw;
-                        if (!att.isZero(EPS) && (mode & 2)) {
-                            Ds<N> c = beta * (thr1 * att * misPower(bpdf, pdf_nee));
-                            L += c;
-                            if constexpr (A == AdjPass::SCATTER) {
-                                ScatterCtx sc{ac->grad, ac->pm, dot(ac->adj, c.val())};
-                                Rng r2w = rs;
-                                HitAD<N> h2;
-                                float pdf2;
-                                T.lookForEmitterAD(no, nd, true, r2w, med1, depth - 1, h2, pdf2, nullptr, nullptr, &sc);
-                            }
-                        }
-                        mode1 = mode & ~1;
-                    }
-                }
-                if (!isnull && P.sppse > 0 && (mode & 4))
-                    edgeTerm<N>(T, P, r, false, its.p, &its0, &b, nullptr, f3{0, 0, 0}, thr.val(), beta.val(), med, depth,
-                                pixel, u, stats);
-                if (!cont) break;
-                beta = beta * thr1;
-                ro = no;
-                rd = nd;
-                its = its1;
-                med = med1;
-                mode = mode1;
-                depth -= 1;
-            }
+#define AD_BOUNCE_END break
+#include "ad_bounce.inc"
+#undef AD_BOUNCE_END
         }
     }
 }
@@ -708,6 +538,125 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
         }
     }
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
+}
+
+// State of one AD path between bounces (the locals of adPathSample's loop).
+template <int N> struct AdState {
+    Dv<N> ro, rd;
+    HitAD<N> its;
+    Ds<N> beta;
+    int med, depth, mode;
+};
+
+// adPathSample up to its bounce loop: camera ray and first hit. False when the path
+// has nothing to do.
+template <int N>
+DI bool adBegin(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, AdState<N> &st) {
+    adCameraRay<N>(vw, r, pixel, st.ro, st.rd);
+    if (!T.rayIntersectAD(st.ro, st.rd, false, st.its)) return false;
+    st.med = vw.cam.med_id;
+    st.depth = P.max_bounces;
+    st.mode = P.mode;
+    st.beta = Ds<N>(f3{1, 1, 1});
+    return st.mode != 0;
+}
+
+// One iteration of adPathSample's bounce loop (forward pass). False when the path ends.
+template <int N>
+DI bool adBounce(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, AdState<N> &st, Ds<N> &L, Pusher &u,
+                 unsigned *stats) {
+    constexpr AdjPass A = AdjPass::NONE;
+    const AdjCtx *ac = nullptr;
+    const DScene &S = T.S;
+    Dv<N> &ro = st.ro, &rd = st.rd;
+    HitAD<N> &its = st.its;
+    Ds<N> &beta = st.beta;
+    int &med = st.med, &depth = st.depth, &mode = st.mode;
+#define AD_BOUNCE_END return false
+#include "ad_bounce.inc"
+#undef AD_BOUNCE_END
+    return mode != 0;
+}
+
+// VolPathTracerAD samples with per-bounce lane refill (same estimator and random
+// streams as k_ad_paths<N, AD_VOLPATH>), for the dual widths that run one wave per
+// SIMD: there the hardware has few other waves to switch to, and the lanes of a wave
+// that wait for its longest path dominate. Each wave owns the contiguous range of
+// `per_wave` sample indices of its block; a lane whose path has ended takes the
+// wave's next sample before the next bounce, until the range is used up. Finished
+// samples are summed over the wave per pixel (one atomic per pixel and channel), or
+// listed in ovf when a push overflowed the queue. Indices are into replay[] in a
+// replay pass.
+template <int N>
+__global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
+                                                     uint64_t per_wave, const uint4 *replay, float *acc, char *q,
+                                                     unsigned *qcount, unsigned *stats, uint4 *ovf, unsigned *ovf_count,
+                                                     unsigned long long *work) {
+    const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const uint64_t begin = (uint64_t)blk * per_wave;
+    const uint64_t end = begin < nsamp ? min(nsamp, begin + per_wave) : begin;
+    Tracer<N> T(S);
+    AdState<N> st;
+    Ds<N> L;
+    Rng r;
+    Pusher u{q, qcount, 0, 0, 0, 0, 0, Pusher::NO_FAIL};
+    uint64_t sid = 0, next = begin;
+    int pixel = 0;
+    uint32_t nlk = 0;
+    bool alive = false, have = false, exhausted = false;
+    const int nd = P.npix;
+    while (true) {
+        const bool fin = have && !alive;   // the lane's path has ended: add its sample
+        const bool put = fin && u.failed_at == Pusher::NO_FAIL;
+        if (fin && !put) ovf[atomicAdd(ovf_count, 1u)] = make_uint4((uint32_t)sid, (uint32_t)(sid >> 32), u.failed_at, 0u);
+        uint64_t pm = __ballot(put);
+        while (pm) {   // wave-uniform: one pixel per round
+            const int lead = __ffsll((unsigned long long)pm) - 1;
+            const int gp0 = __shfl(u.gpix, lead);
+            const bool mine = put && u.gpix == gp0;
+            pm &= ~__ballot(mine);
+#pragma unroll
+            for (int k = 0; k < 3 * (N + 1); ++k) {
+                const int c = k % 3, j = k / 3;
+                float x = mine ? (j == 0 ? L.c[c].v : L.c[c].d[j - 1]) : 0.f;
+                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+                if ((int)(threadIdx.x & 63) == lead && x != 0.f) unsafeAtomicAdd(acc + ((size_t)j * nd + gp0) * 3 + c, x);
+            }
+        }
+        if (fin) {
+            nlk += r.nlk;
+            have = false;
+        }
+        const bool want = !alive && !exhausted;
+        const uint64_t m = __ballot(want);
+        if (m == 0 && !__any(alive)) break;   // the wave's range is used up
+        if (want) {
+            const uint64_t li = next + __popcll(m & __lanemask_lt());
+            if (li < end) {
+                uint32_t skip = 0;
+                sid = s0 + li;
+                if (replay) {
+                    const uint4 e = replay[li];
+                    sid = (uint64_t)e.x | ((uint64_t)e.y << 32);
+                    skip = e.z;
+                }
+                const int v = viewOf(S, sid);
+                const DView &vw = S.views[v];
+                const uint64_t ls = sid - vw.s0;   // sample index within the view: keys its random stream
+                pixel = (int)(ls / (uint64_t)vw.spp);
+                u = Pusher{q, qcount, vw.pix0 + pixel, v, (ls + 1) << 20, 0, skip, Pusher::NO_FAIL};
+                r.init(vw.seed, ls);
+                L = Ds<N>();
+                have = true;
+                alive = adBegin<N>(T, P, vw, r, pixel, st);
+            } else {
+                exhausted = true;
+            }
+        }
+        next += (uint64_t)__popcll(m);
+        if (alive) alive = adBounce<N>(T, P, r, pixel, st, L, u, stats);
+    }
+    flushWork(work, 0, T.nbox, T.ntri, nlk, T.nseg);
 }
 
 // The primal integrators: one thread per sample s0 .. s0+n-1 of view 0 (sample k of
@@ -1372,9 +1321,20 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
     else if (P.ad_kind == AD_DIRECT)
         hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf,
                            ovf_count, wk);
-    else
+    else if constexpr (adWaves<N>() == 1) {
+        // one wave per block, AD_REFILL_WAVES_PER_CU of them per CU share the samples
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const uint64_t waves = (uint64_t)std::max(ncu, 1) * AD_REFILL_WAVES_PER_CU;
+        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(nb, waves));
+        const uint64_t per_wave = (n + grid - 1) / grid;
+        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), 0, sm, S, P, s0, n, per_wave, replay, acc, q,
+                           qc, st, ovf, ovf_count, wk);
+    } else {
         hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st,
                            ovf, ovf_count, wk);
+    }
 }
 template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
                                             float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {

@@ -127,10 +127,12 @@ def test_render_into_device_tensor(gpu):
     assert st["ad_samples"] == 16 * 16 * 8 and st["children"] > 0
 
 
-def test_queue_overflow_replay_is_exact(gpu):
+@pytest.mark.parametrize("nder", [3, 8])
+def test_queue_overflow_replay_is_exact(gpu, nder):
     """A queue far smaller than one chunk's children forces many overflow/replay rounds;
-    every child and every sample must still be counted exactly once."""
-    d = scenes.config1(nder=3, res=64, spp=64, grid_res=16, crop=(16, 16, 32, 32))
+    every child and every sample must still be counted exactly once. At nder 8 the AD
+    samples run in the per-bounce refill kernel (k_ad_refill), at nder 3 in k_ad_paths."""
+    d = scenes.config1(nder=nder, res=64, spp=64, grid_res=16, crop=(16, 16, 32, 32))
     sc = scenes.instantiate(gpu, d)
     ref = scenes.render(gpu, d, seed=4, scene=sc, sppe=64).astype(np.float64)
     st_ref = gpu.render_stats(sc)
