@@ -4,6 +4,8 @@
 //                   estimator LiAD (volpathAD.cpp:145-620) iteratively, accumulates the
 //                   interior terms, and pushes every primal sub-path the boundary terms
 //                   need (2 per accepted edge sample, 1 per Lins sample) into an HBM queue.
+//  k_ad_refill<N>   the same for N >= 8 (one wave per SIMD), with per-bounce lane refill:
+//                   a lane whose path has ended starts the wave's next sample.
 //  k_children<N>    one thread per queued sub-path: primal Li / Lins estimate, scattered
 //                   with its per-lane weights into the derivative accumulators.
 //  k_primary<N>     primary-edge samples (integratorAD.cpp:100-165): 2 camera sub-paths each.
