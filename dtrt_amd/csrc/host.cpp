@@ -188,7 +188,12 @@ struct BvhOut {
     int depth = 0;
 };
 
-bool buildBvh(const std::vector<std::array<float, 9>> &tri, BvhOut &out) {
+static float boxArea(const float *l, const float *h) {
+    float dx = std::max(0.f, h[0] - l[0]), dy = std::max(0.f, h[1] - l[1]), dz = std::max(0.f, h[2] - l[2]);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<int> &shape_of, BvhOut &out) {
     const int n = (int)tri.size();
     std::vector<float> cen((size_t)n * 3), lo((size_t)n * 3), hi((size_t)n * 3);
     for (int i = 0; i < n; ++i)
@@ -200,6 +205,50 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, BvhOut &out) {
         }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
+    // Enclosing shapes. SAH prices a split by the chance that a ray from outside the
+    // node enters each child; paths start inside the scene, and a shape that encloses
+    // the others (an environment sphere, a medium's boundary box around its contents)
+    // is then hit only after everything it holds has been tested. Its triangles are
+    // split off as the node's second child when its box holds the boxes of all other
+    // shapes in the node and has at least 4 times their union's area; traversal visits
+    // the first child first when both entries tie (origins inside both boxes).
+    const int nshape = shape_of.empty() ? 0 : 1 + *std::max_element(shape_of.begin(), shape_of.end());
+    std::vector<float> sbox((size_t)nshape * 6);
+    std::vector<int> present;
+    auto enclosingShape = [&](int first, int count) -> int {
+        const int s0 = shape_of[idx[first]];
+        int i = first;
+        while (i < first + count && shape_of[idx[i]] == s0) ++i;
+        if (i == first + count) return -1;   // one shape only
+        present.clear();
+        for (int k = first; k < first + count; ++k) {
+            const int p = idx[k], sh = shape_of[p];
+            float *b = &sbox[(size_t)sh * 6];
+            if (std::find(present.begin(), present.end(), sh) == present.end()) {
+                present.push_back(sh);
+                for (int a = 0; a < 3; ++a) { b[a] = INFINITY; b[3 + a] = -INFINITY; }
+            }
+            for (int a = 0; a < 3; ++a) { b[a] = std::min(b[a], lo[3 * p + a]); b[3 + a] = std::max(b[3 + a], hi[3 * p + a]); }
+            if (present.size() > 64) return -1;   // many small shapes: leave it to SAH
+        }
+        int best = -1;
+        float bestArea = 0.f;
+        for (int sh : present) {
+            float ul[3] = {INFINITY, INFINITY, INFINITY}, uh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int o : present)
+                if (o != sh)
+                    for (int a = 0; a < 3; ++a) {
+                        ul[a] = std::min(ul[a], sbox[(size_t)o * 6 + a]);
+                        uh[a] = std::max(uh[a], sbox[(size_t)o * 6 + 3 + a]);
+                    }
+            const float *b = &sbox[(size_t)sh * 6];
+            bool holds = true;
+            for (int a = 0; a < 3; ++a) holds = holds && b[a] <= ul[a] && b[3 + a] >= uh[a];
+            const float sa = boxArea(b, b + 3);
+            if (holds && sa >= 4.f * boxArea(ul, uh) && sa > bestArea) { best = sh; bestArea = sa; }
+        }
+        return best;
+    };
     std::vector<BuildNode> bn;
     bn.reserve(2 * (size_t)n + 1);
     struct Task { int node, first, count, depth; };
@@ -269,7 +318,11 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, BvhOut &out) {
         const bool tight = t.depth + need >= DTRT_BVH_STACK - 2;
         if (tight && t.depth + need > DTRT_BVH_STACK - 2) return fail("bvh: too many triangles for the traversal stack");
         int mid;
-        if (tight && mustSplit) {
+        const int encl = tight || shape_of.empty() ? -1 : enclosingShape(t.first, t.count);
+        if (encl >= 0) {   // the contents first, the enclosing shape second
+            mid = (int)(std::partition(idx.begin() + t.first, idx.begin() + t.first + t.count,
+                                       [&](int p) { return shape_of[p] != encl; }) - idx.begin());
+        } else if (tight && mustSplit) {
             // depth budget exhausted: median split along the widest centroid axis keeps
             // every leaf below at depth <= DTRT_BVH_STACK - 2
             int ax = 0;
@@ -706,7 +759,7 @@ bool Scene::build() {
     S.NV = vbase;
     S.NF = fbase;
     BvhOut bvh;
-    if (!buildBvh(tri, bvh)) return false;
+    if (!buildBvh(tri, fshape, bvh)) return false;
     S.nnodes = (int)(bvh.nodes.size() / 4);
     S.ntris = (int)tri.size();
     S.nodes = upload(bvh.nodes);
