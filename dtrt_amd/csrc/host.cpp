@@ -251,9 +251,9 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<in
     };
     std::vector<BuildNode> bn;
     bn.reserve(2 * (size_t)n + 1);
-    struct Task { int node, first, count, depth; };
+    struct Task { int node, first, count, depth, shell; float cx, cy, cz; };
     bn.push_back(BuildNode());
-    std::vector<Task> st{{0, 0, n, 1}};
+    std::vector<Task> st{{0, 0, n, 1, 0, 0.f, 0.f, 0.f}};
     const int NB = 16, MAXLEAF = 15;
     int maxDepth = 0;
     auto area = [](const float *l, const float *h) {
@@ -318,10 +318,46 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<in
         const bool tight = t.depth + need >= DTRT_BVH_STACK - 2;
         if (tight && t.depth + need > DTRT_BVH_STACK - 2) return fail("bvh: too many triangles for the traversal stack");
         int mid;
-        const int encl = tight || shape_of.empty() ? -1 : enclosingShape(t.first, t.count);
-        if (encl >= 0) {   // the contents first, the enclosing shape second
+        const int encl = tight || shape_of.empty() || t.shell ? -1 : enclosingShape(t.first, t.count);
+        // Below an enclosing shape's split (of at least 64 triangles), its triangles are
+        // split by direction from the centre of what it encloses (signed dominant axis of
+        // centroid - centre, six classes) until one class is left: the boxes of such
+        // groups keep clear of the interior, where the paths start, unlike SAH's planes
+        // through the centre.
+        auto dirClass = [&](int p) {
+            const float d[3] = {cen[3 * p] - t.cx, cen[3 * p + 1] - t.cy, cen[3 * p + 2] - t.cz};
+            int a = 0;
+            for (int k = 1; k < 3; ++k)
+                if (std::fabs(d[k]) > std::fabs(d[a])) a = k;
+            return 2 * a + (d[a] < 0.f ? 1 : 0);
+        };
+        unsigned classes = 0;
+        if (t.shell && !tight)
+            for (int k = t.first; k < t.first + t.count; ++k) classes |= 1u << dirClass(idx[k]);
+        int childShell = 0;   // 1: children keep the direction split, 2: second child starts it
+        float ccx = t.cx, ccy = t.cy, ccz = t.cz;
+        if (__builtin_popcount(classes) > 1) {   // half of the present classes to each side
+            int cls[6], nc = 0;
+            for (int c = 0; c < 6; ++c)
+                if (classes >> c & 1) cls[nc++] = c;
+            unsigned left = 0;
+            for (int i = 0; i < nc / 2; ++i) left |= 1u << cls[i];
+            mid = (int)(std::partition(idx.begin() + t.first, idx.begin() + t.first + t.count,
+                                       [&](int p) { return (left >> dirClass(p) & 1) != 0; }) - idx.begin());
+            childShell = 1;
+        } else if (encl >= 0) {   // the contents first, the enclosing shape second
             mid = (int)(std::partition(idx.begin() + t.first, idx.begin() + t.first + t.count,
                                        [&](int p) { return shape_of[p] != encl; }) - idx.begin());
+            float ul[3] = {INFINITY, INFINITY, INFINITY}, uh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int k = t.first; k < mid; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    ul[a] = std::min(ul[a], lo[3 * idx[k] + a]);
+                    uh[a] = std::max(uh[a], hi[3 * idx[k] + a]);
+                }
+            ccx = 0.5f * (ul[0] + uh[0]);
+            ccy = 0.5f * (ul[1] + uh[1]);
+            ccz = 0.5f * (ul[2] + uh[2]);
+            childShell = t.first + t.count - mid >= 64 ? 2 : 0;   // small shells: SAH is as good
         } else if (tight && mustSplit) {
             // depth budget exhausted: median split along the widest centroid axis keeps
             // every leaf below at depth <= DTRT_BVH_STACK - 2
@@ -346,8 +382,8 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<in
         bn.push_back(BuildNode());
         bn[t.node].left = l;
         bn[t.node].right = l + 1;
-        st.push_back({l, t.first, mid - t.first, t.depth + 1});
-        st.push_back({l + 1, mid, t.first + t.count - mid, t.depth + 1});
+        st.push_back({l, t.first, mid - t.first, t.depth + 1, childShell == 1, ccx, ccy, ccz});
+        st.push_back({l + 1, mid, t.first + t.count - mid, t.depth + 1, childShell != 0, ccx, ccy, ccz});
     }
     // triangles in leaf order
     out.tris.resize((size_t)n * 3);
