@@ -277,7 +277,9 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<in
         std::memcpy(nd.lo, nlo, sizeof(nlo));
         std::memcpy(nd.hi, nhi, sizeof(nhi));
         bool mustSplit = t.count > MAXLEAF;
-        if (t.count <= 4 && !mustSplit) { nd.first = t.first; nd.count = t.count; continue; }
+        // nodes of at most 2 triangles are leaves (measured: 2 beats 1, 4 and 8 on the
+        // medium, dielectric and surface scenes)
+        if (t.count <= 2 && !mustSplit) { nd.first = t.first; nd.count = t.count; continue; }
         // binned SAH over the three axes
         int bestAxis = -1, bestSplit = -1;
         float bestCost = INFINITY;
