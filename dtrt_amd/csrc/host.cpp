@@ -1909,6 +1909,45 @@ int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, 
     for (int i = 0; i < 12; ++i) work12[i] = s->work_counts[i];
     return 0;
 }
+int dtrt_test_build_bvh(const float *tri9, const int *shape_of, int ntri, int64_t *out8) {
+    if (!tri9 || !out8 || ntri <= 0) {
+        fail("build_bvh: empty input");
+        return -1;
+    }
+    std::vector<std::array<float, 9>> tri((size_t)ntri);
+    for (int i = 0; i < ntri; ++i)
+        for (int k = 0; k < 9; ++k) tri[i][k] = tri9[(size_t)i * 9 + k];
+    std::vector<int> sh;
+    if (shape_of) sh.assign(shape_of, shape_of + ntri);
+    BvhOut b;
+    if (!buildBvh(tri, sh, b)) return -1;
+    // walk the GPU nodes: depth, pushes, leaf sizes, root child sizes
+    const int nn = (int)(b.nodes.size() / 4);
+    int64_t maxDepth = 0, leaves = 0, maxLeaf = 0, rootCount[2] = {0, 0};
+    struct W { int link, depth, root; };
+    std::vector<W> stack{{0, 1, -1}};
+    while (!stack.empty()) {
+        const W w = stack.back();
+        stack.pop_back();
+        if (w.link < 0) {
+            const int enc = ~w.link, cnt = enc & 15;
+            if (cnt == 0) continue;   // the empty slot of a single-leaf root
+            ++leaves;
+            maxLeaf = std::max<int64_t>(maxLeaf, cnt);
+            maxDepth = std::max<int64_t>(maxDepth, w.depth);
+            if (w.root >= 0) rootCount[w.root] += cnt;
+            continue;
+        }
+        const float4 *n = b.nodes.data() + 4 * (size_t)w.link;
+        int c[2];
+        std::memcpy(&c[0], &n[0].w, 4);
+        std::memcpy(&c[1], &n[1].w, 4);
+        for (int k = 0; k < 2; ++k) stack.push_back({c[k], w.depth + 1, w.link == 0 ? k : w.root});
+    }
+    out8[0] = nn; out8[1] = maxDepth; out8[2] = rootCount[0]; out8[3] = rootCount[1];
+    out8[4] = maxDepth - 1; out8[5] = leaves; out8[6] = maxLeaf; out8[7] = 0;
+    return 0;
+}
 int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8) {
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s) return -1;

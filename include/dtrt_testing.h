@@ -48,6 +48,12 @@ int dtrt_test_medium(dtrt_object *scene, int medium, int kind, const float *seg8
 /* Density lookups with tangents: points {x, y, z, nder tangent vectors} ->
  * {value, nder derivatives}. */
 int dtrt_test_grid(dtrt_object *scene, int medium, const float *pts, int n, float *out);
+/* Host-only BVH build of ntri triangles {v0, v1, v2} (9 floats each) with the shape
+ * index of each (may be NULL: one shape). out8: [0] inner nodes, [1] deepest leaf
+ * depth (root = 1), [2] triangles under the root's first child, [3] under its second
+ * child, [4] most stack entries a traversal can push (binary: one per level below the
+ * root), [5] leaves, [6] largest leaf. Needs no GPU. */
+int dtrt_test_build_bvh(const float *tri9, const int *shape_of, int ntri, int64_t *out8);
 /* One Philox4x32-10 block. */
 int dtrt_test_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
 

@@ -79,3 +79,49 @@ def test_errors_are_reported_not_fatal():
         G.Shape(v, bad, None, None, 3, 1, -1, 0, -1, -1, None)
     lib.dtrt_render.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     assert lib.dtrt_render(0, None, None, None) != 0
+
+
+def _bvh(tris, shapes=None):
+    lib = C.CDLL(LIB)
+    lib.dtrt_test_build_bvh.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    s = None if shapes is None else np.ascontiguousarray(shapes, np.int32)
+    out = (C.c_int64 * 8)()
+    rc = lib.dtrt_test_build_bvh(t.ctypes.data, None if s is None else s.ctypes.data, len(t), out)
+    assert rc == 0
+    return dict(nodes=out[0], depth=out[1], root=(out[2], out[3]), pushes=out[4], leaves=out[5], max_leaf=out[6])
+
+
+def _tris(desc):
+    return np.concatenate([np.asarray(s.vertices, np.float32)[np.asarray(s.indices).reshape(-1, 3)].reshape(-1, 9)
+                           for s in desc.shapes])
+
+
+def test_bvh_builder_splits_the_enclosing_shape_off_first():
+    """The benchmark scene: null cube (12) + quad light (2) inside an emissive sphere
+    (320). The sphere's box holds the others with far more area, so the root's first
+    child holds the 14 contents and the second the sphere; without shape ids SAH mixes
+    them."""
+    d = scenes.config1(nder=1, res=8, spp=1, grid_res=4)
+    tris = _tris(d)
+    shapes = np.concatenate([np.full(len(s.indices), i) for i, s in enumerate(d.shapes)])
+    b = _bvh(tris, shapes)
+    assert b["root"] == (14, 320)
+    assert b["leaves"] > 0 and b["max_leaf"] <= 15 and b["pushes"] <= 32
+    plain = _bvh(tris)
+    assert plain["root"] != (14, 320)
+
+
+def test_bvh_builder_bounds_the_traversal_stack():
+    """200K triangles of one shape, all centroids equal (forcing splits by index), and a
+    random soup: every leaf stays within the 32-entry traversal stack and at most 15
+    triangles."""
+    rng = np.random.default_rng(2)
+    same = np.tile(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), (200000, 1))
+    c = rng.random((200000, 3)).astype(np.float32) * 10
+    soup = np.concatenate([c, c + rng.random((200000, 3)).astype(np.float32) * 0.05,
+                           c + rng.random((200000, 3)).astype(np.float32) * 0.05], 1)
+    for tris in (same, soup):
+        b = _bvh(tris)
+        assert b["pushes"] <= 32 and b["max_leaf"] <= 15 and b["root"][0] + b["root"][1] == len(tris)
+    assert _bvh(soup[:2])["leaves"] == 1
