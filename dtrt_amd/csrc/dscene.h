@@ -54,12 +54,7 @@ struct TraceHit {
     uint32_t nbox, ntri;
 };
 
-#ifdef DTRT_TRACE_OUT_OF_LINE
-#define DTRT_TRACE_ATTR __device__ __noinline__
-#else
-#define DTRT_TRACE_ATTR __device__ __forceinline__
-#endif
-DTRT_TRACE_ATTR TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3 o, f3 d, float tnear,
+DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3 o, f3 d, float tnear,
                                       float tfar) {
     TraceHit res{-1, tfar, 0u, 0u};
     if (nnodes == 0) return res;

@@ -360,9 +360,12 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<in
             ccy = 0.5f * (ul[1] + uh[1]);
             ccz = 0.5f * (ul[2] + uh[2]);
             childShell = t.first + t.count - mid >= 64 ? 2 : 0;   // small shells: SAH is as good
-        } else if (tight && mustSplit) {
-            // depth budget exhausted: median split along the widest centroid axis keeps
-            // every leaf below at depth <= DTRT_BVH_STACK - 2
+        } else if (tight) {
+            // depth budget exhausted: a node that fits a leaf becomes one (a split here
+            // could leave a child too large for the last level), a larger one is split at
+            // the median along the widest centroid axis, which keeps every leaf below at
+            // depth <= DTRT_BVH_STACK - 2
+            if (!mustSplit) { nd.first = t.first; nd.count = t.count; continue; }
             int ax = 0;
             for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
             mid = t.first + (t.count + 1) / 2;
@@ -611,6 +614,7 @@ struct Scene : Obj {
         for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf, &tunebuf})
             if (b->p) (void)hipFree(b->p);
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        if (caller_ev) (void)hipEventDestroy(caller_ev);
         if (stream) (void)hipStreamDestroy(stream);
     }
     int nder = 0;
@@ -645,6 +649,17 @@ struct Scene : Obj {
     } graph;
     std::map<uint64_t, PECache> pecache;   // primary edges per camera configuration (Camera::uid)
     hipStream_t stream = nullptr;
+    // The caller's stream (default: the legacy null stream). The scene's own stream is
+    // non-blocking, so every call that reads or writes caller memory first makes it wait
+    // for the work the caller has queued there (a zeroed output, a loss gradient).
+    hipStream_t caller = nullptr;
+    hipEvent_t caller_ev = nullptr;
+    bool afterCaller() {
+        if (!caller_ev) HIPCHK(hipEventCreateWithFlags(&caller_ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(caller_ev, caller));
+        HIPCHK(hipStreamWaitEvent(stream, caller_ev, 0));
+        return true;
+    }
     std::vector<hipEvent_t> ev;
     // last render
     int64_t last_stats[10] = {0};
@@ -979,6 +994,7 @@ constexpr uint64_t FIRST_CHUNK = uint64_t(1) << 16;  // before the children-per-
 bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs,
                         bool out_is_device, int ad_kind) {
     if (nv <= 0) return fail("render: no views");
+    if (!afterCaller()) return false;
     const dtrt_render_options &o0 = opts[0];
     for (int v = 1; v < nv; ++v)
         if (opts[v].max_bounces != o0.max_bounces || opts[v].mode != o0.mode || opts[v].sppse != o0.sppse ||
@@ -1217,6 +1233,7 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
 int Scene::replayViews(const uint64_t *seeds) {
     if (!graph.x) { g_err = "replay: no captured views"; return -1; }
     if (graphBuffers() != graph.bufs) { g_err = "replay: an eager render reallocated the buffers; capture again"; return -1; }
+    if (!afterCaller()) return -1;
     const int nv = (int)graph.hv.size();
     for (int v = 0; v < nv; ++v) {
         graph.hv[v].seed = seeds[v];
@@ -1258,6 +1275,7 @@ int Scene::replayViews(const uint64_t *seeds) {
 // clamped to 0 as the VOL reader does, and the majorant is recomputed.
 bool Scene::setDensity(int med, const float *density, bool on_device) {
     if (!gridMedium(med) || !density) return fail("set_density: bad arguments");
+    if (on_device && !afterCaller()) return false;
     DMedium &d = hmed[med];
     const uint64_t n = (uint64_t)d.res[0] * d.res[1] * d.res[2];
     float *dst = const_cast<float *>(d.density);
@@ -1299,6 +1317,7 @@ bool Scene::setVoxelLanes(int med, const int *vox, int n) {
 bool Scene::renderAdjointViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, int med,
                                const float *dimg, float *grad) {
     if (!gridMedium(med) || !dimg || !grad || nv <= 0) return false;
+    if (!afterCaller()) return false;
     const dtrt_render_options &o = opts[0];
     RenderParams P{};
     P.max_bounces = o.max_bounces; P.sppse = 0;
@@ -1349,6 +1368,7 @@ bool Scene::renderAdjointViews(int nv, const Camera *const *cams, const dtrt_ren
 // 5 ParticleTracer (100000 * spp light particles, splatted and divided by their count).
 bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator) {
     if (o.spp <= 0 || o.max_bounces < 0 || o.max_bounces > 200) return fail("render: bad options");
+    if (!afterCaller()) return false;
     RenderParams P{};
     P.max_bounces = integrator == 4 ? 1 : o.max_bounces;
     const int kind = integrator == 2 ? PRIMAL_SIMPLE : integrator >= 3 ? PRIMAL_PATH : PRIMAL_VOLPATH;
@@ -1475,12 +1495,21 @@ bool readVol(const char *fn, std::vector<float> &data, int *res, int &ch, float 
               std::fread(&ch, 4, 1, f) == 1 && (ch == 1 || ch == 3) && res[0] > 1 && res[1] > 1 && res[2] > 1 &&
               std::fread(mn, 4, 3, f) == 3 && std::fread(mx, 4, 3, f) == 3;
     if (ok) {
-        size_t n = (size_t)res[0] * res[1] * res[2] * ch;
-        data.resize(n);
-        ok = std::fread(data.data(), 4, n, f) == n;
+        // the header's voxel count must match the bytes that follow it before anything
+        // is allocated: a corrupt header must not ask for an arbitrary amount of memory
+        const long here = std::ftell(f);
+        ok = here >= 0 && std::fseek(f, 0, SEEK_END) == 0;
+        const long end = ok ? std::ftell(f) : -1;
+        ok = ok && end >= here && std::fseek(f, here, SEEK_SET) == 0;
+        const uint64_t n = (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2] * (uint64_t)ch;
+        ok = ok && n <= ((uint64_t)1 << 36) && n * 4 <= (uint64_t)(end - here);
+        if (ok) {
+            data.resize((size_t)n);
+            ok = std::fread(data.data(), 4, (size_t)n, f) == (size_t)n;
+        }
     }
     std::fclose(f);
-    if (!ok) return fail(std::string("invalid VOL v3 file ") + fn);
+    if (!ok) return fail(std::string("invalid or truncated VOL v3 file ") + fn);
     for (float &x : data) x = std::max(x, 0.0f);
     return true;
 }
@@ -1540,6 +1569,21 @@ struct TmpFree {
     ~TmpFree() { for (void *p : v) (void)hipFree(p); }
 };
 
+// Runs f and turns any C++ exception into an error message and the error return:
+// nothing thrown crosses the C-ABI.
+template <class R, class F> R guarded(R on_error, F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        g_err = "out of host memory";
+    } catch (const std::exception &e) {
+        g_err = e.what();
+    } catch (...) {
+        g_err = "unknown error";
+    }
+    return on_error;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1556,176 +1600,200 @@ const char *dtrt_last_error(void) { return g_err.c_str(); }
 void dtrt_destroy(dtrt_object *o) { delete reinterpret_cast<Obj *>(o); }
 
 dtrt_object *dtrt_camera_create(int w, int h, const float *c2w, const float *c2ndc, float clip_near, int med_id, const float *vel) {
-    if (!c2w || !c2ndc || w <= 0 || h <= 0) { g_err = "camera: bad arguments"; return nullptr; }
-    auto c = std::make_unique<Camera>();
-    int nd = nderNow();
-    c->w = w; c->h = h; c->clip_near = clip_near; c->med = med_id; c->nder = nd;
-    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) c->c2w[i][j] = c2w[4 * i + j];
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->c2ndc[i][j] = c2ndc[3 * i + j];
-    if (c->c2ndc[0][0] != c->c2ndc[1][1]) { g_err = "camera: cam_to_ndc[0][0] must equal cam_to_ndc[1][1]"; return nullptr; }
-    if (!invert4(c->c2w, c->w2c)) { g_err = "camera: singular cam_to_world"; return nullptr; }
-    double m4[4][4] = {{0}}, i4[4][4];
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m4[i][j] = c->c2ndc[i][j];
-    m4[3][3] = 1;
-    if (!invert4(m4, i4)) { g_err = "camera: singular cam_to_ndc"; return nullptr; }
-    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->ndc2c[i][j] = i4[i][j];
-    c->cpos = D3{c->c2w[0][3], c->c2w[1][3], c->c2w[2][3]};
-    c->cs = D3{c->c2w[0][0], c->c2w[1][0], c->c2w[2][0]};
-    c->ct = D3{c->c2w[0][1], c->c2w[1][1], c->c2w[2][1]};
-    c->cn = D3{c->c2w[0][2], c->c2w[1][2], c->c2w[2][2]};
-    for (int j = 0; j < nd; ++j) {   // camera.cpp:92-111
-        D3 dx = vel ? D3{vel[3 * j], vel[3 * j + 1], vel[3 * j + 2]} : D3{0, 0, 0};
-        D3 dw = vel ? D3{vel[3 * nd + 3 * j], vel[3 * nd + 3 * j + 1], vel[3 * nd + 3 * j + 2]} : D3{0, 0, 0};
-        c->cpos_d[j] = dx;
-        c->cs_d[j] = cross3(dw, c->cs);
-        c->ct_d[j] = cross3(dw, c->ct);
-        c->cn_d[j] = cross3(dw, c->cn);
-    }
-    return handle(c.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!c2w || !c2ndc || w <= 0 || h <= 0) { g_err = "camera: bad arguments"; return nullptr; }
+        auto c = std::make_unique<Camera>();
+        int nd = nderNow();
+        c->w = w; c->h = h; c->clip_near = clip_near; c->med = med_id; c->nder = nd;
+        for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) c->c2w[i][j] = c2w[4 * i + j];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->c2ndc[i][j] = c2ndc[3 * i + j];
+        if (c->c2ndc[0][0] != c->c2ndc[1][1]) { g_err = "camera: cam_to_ndc[0][0] must equal cam_to_ndc[1][1]"; return nullptr; }
+        if (!invert4(c->c2w, c->w2c)) { g_err = "camera: singular cam_to_world"; return nullptr; }
+        double m4[4][4] = {{0}}, i4[4][4];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m4[i][j] = c->c2ndc[i][j];
+        m4[3][3] = 1;
+        if (!invert4(m4, i4)) { g_err = "camera: singular cam_to_ndc"; return nullptr; }
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->ndc2c[i][j] = i4[i][j];
+        c->cpos = D3{c->c2w[0][3], c->c2w[1][3], c->c2w[2][3]};
+        c->cs = D3{c->c2w[0][0], c->c2w[1][0], c->c2w[2][0]};
+        c->ct = D3{c->c2w[0][1], c->c2w[1][1], c->c2w[2][1]};
+        c->cn = D3{c->c2w[0][2], c->c2w[1][2], c->c2w[2][2]};
+        for (int j = 0; j < nd; ++j) {   // camera.cpp:92-111
+            D3 dx = vel ? D3{vel[3 * j], vel[3 * j + 1], vel[3 * j + 2]} : D3{0, 0, 0};
+            D3 dw = vel ? D3{vel[3 * nd + 3 * j], vel[3 * nd + 3 * j + 1], vel[3 * nd + 3 * j + 2]} : D3{0, 0, 0};
+            c->cpos_d[j] = dx;
+            c->cs_d[j] = cross3(dw, c->cs);
+            c->ct_d[j] = cross3(dw, c->ct);
+            c->cn_d[j] = cross3(dw, c->cn);
+        }
+        return handle(c.release());
+    });
 }
 int dtrt_camera_set_rect(dtrt_object *cam, int ox, int oy, int w, int h) {
-    Camera *c = as<Camera>(cam, T_CAMERA);
-    if (!c) return -1;
-    c->ox = ox; c->oy = oy; c->cw = w; c->ch = h;
-    c->uid = ++g_camera_uid;   // the crop clips the primary edges
-    if (c->cropped() && (ox < 0 || oy < 0 || ox + w > c->w || oy + h > c->h)) { g_err = "set_rect: crop outside the image"; return -1; }
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Camera *c = as<Camera>(cam, T_CAMERA);
+        if (!c) return -1;
+        c->ox = ox; c->oy = oy; c->cw = w; c->ch = h;
+        c->uid = ++g_camera_uid;   // the crop clips the primary edges
+        if (c->cropped() && (ox < 0 || oy < 0 || ox + w > c->w || oy + h > c->h)) { g_err = "set_rect: crop outside the image"; return -1; }
+        return 0;
+    });
 }
 
 dtrt_object *dtrt_shape_create(const float *v, const int *idx, const float *uvs, const float *n, int nv, int nf, int light_id,
                                int bsdf_id, int med_int, int med_ext, const float *vel) {
-    (void)uvs;   // no BSDF on the path reads texture coordinates
-    if (!v || !idx || nv <= 0 || nf <= 0) { g_err = "shape: bad arguments"; return nullptr; }
-    auto s = std::make_unique<Shape>();
-    int nd = nderNow();
-    s->nv = nv; s->nf = nf; s->nder = nd; s->light_id = light_id; s->bsdf_id = bsdf_id; s->med_int = med_int; s->med_ext = med_ext;
-    s->v.resize(nv);
-    for (int i = 0; i < nv; ++i) s->v[i] = D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
-    s->f.assign(idx, idx + 3 * (size_t)nf);
-    for (int x : s->f) if (x < 0 || x >= nv) { g_err = "shape: vertex index out of range"; return nullptr; }
-    if (n) {
-        s->n.resize(nv);
-        for (int i = 0; i < nv; ++i) {
-            D3 q{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
-            s->n[i] = q * (1.0 / norm3(q));
-        }
-    }
-    s->vd.assign((size_t)nv * nd, D3{0, 0, 0});
-    s->nd.assign(n ? (size_t)nv * nd : 0, D3{0, 0, 0});
-    if (vel)
-        for (int j = 0; j < nd; ++j)
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        (void)uvs;   // no BSDF on the path reads texture coordinates
+        if (!v || !idx || nv <= 0 || nf <= 0) { g_err = "shape: bad arguments"; return nullptr; }
+        auto s = std::make_unique<Shape>();
+        int nd = nderNow();
+        s->nv = nv; s->nf = nf; s->nder = nd; s->light_id = light_id; s->bsdf_id = bsdf_id; s->med_int = med_int; s->med_ext = med_ext;
+        s->v.resize(nv);
+        for (int i = 0; i < nv; ++i) s->v[i] = D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+        s->f.assign(idx, idx + 3 * (size_t)nf);
+        for (int x : s->f) if (x < 0 || x >= nv) { g_err = "shape: vertex index out of range"; return nullptr; }
+        if (n) {
+            s->n.resize(nv);
             for (int i = 0; i < nv; ++i) {
-                const float *p = vel + ((size_t)j * nv + i) * 3;
-                s->vd[(size_t)i * nd + j] = D3{p[0], p[1], p[2]};
-                if (n) {   // shape.cpp:45 reads the same normal block for every lane
-                    const float *q = vel + (size_t)nv * 3 * nd + (size_t)i * 3;
-                    s->nd[(size_t)i * nd + j] = D3{q[0], q[1], q[2]};
-                }
+                D3 q{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+                s->n[i] = q * (1.0 / norm3(q));
             }
-    computeFaceNormals(*s);
-    if (!constructEdges(*s)) return nullptr;
-    return handle(s.release());
+        }
+        s->vd.assign((size_t)nv * nd, D3{0, 0, 0});
+        s->nd.assign(n ? (size_t)nv * nd : 0, D3{0, 0, 0});
+        if (vel)
+            for (int j = 0; j < nd; ++j)
+                for (int i = 0; i < nv; ++i) {
+                    const float *p = vel + ((size_t)j * nv + i) * 3;
+                    s->vd[(size_t)i * nd + j] = D3{p[0], p[1], p[2]};
+                    if (n) {   // shape.cpp:45 reads the same normal block for every lane
+                        const float *q = vel + (size_t)nv * 3 * nd + (size_t)i * 3;
+                        s->nd[(size_t)i * nd + j] = D3{q[0], q[1], q[2]};
+                    }
+                }
+        computeFaceNormals(*s);
+        if (!constructEdges(*s)) return nullptr;
+        return handle(s.release());
+    });
 }
 
 dtrt_object *dtrt_bsdf_diffuse_create(const float *rgb, const float *d) {
-    if (!rgb) { g_err = "BSDF_diffuse: null reflectance"; return nullptr; }
-    auto b = std::make_unique<Bsdf>();
-    b->d.type = BSDF_DIFFUSE;
-    for (int k = 0; k < 3; ++k) b->d.refl[k] = rgb[k];
-    fillSpecD(b->d.refl_d, d, nderNow());
-    return handle(b.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!rgb) { g_err = "BSDF_diffuse: null reflectance"; return nullptr; }
+        auto b = std::make_unique<Bsdf>();
+        b->d.type = BSDF_DIFFUSE;
+        for (int k = 0; k < 3; ++k) b->d.refl[k] = rgb[k];
+        fillSpecD(b->d.refl_d, d, nderNow());
+        return handle(b.release());
+    });
 }
 dtrt_object *dtrt_bsdf_null_create(void) { auto b = std::make_unique<Bsdf>(); b->d.type = BSDF_NULL; return handle(b.release()); }
 dtrt_object *dtrt_bsdf_phong_create(const float *kd, const float *ks, float ex, const float *dkd, const float *dks, const float *dex) {
-    if (!kd || !ks) { g_err = "BSDF_Phong: null reflectance"; return nullptr; }
-    auto b = std::make_unique<Bsdf>();
-    b->d.type = BSDF_PHONG;
-    for (int k = 0; k < 3; ++k) { b->d.kd[k] = kd[k]; b->d.ks[k] = ks[k]; }
-    b->d.ex = ex;
-    double v0 = ((double)kd[0] + kd[1] + kd[2]) / 3, v1 = ((double)ks[0] + ks[1] + ks[2]) / 3;
-    b->d.specW = (float)(v1 / (v0 + v1));
-    int nd = nderNow();
-    fillSpecD(b->d.kd_d, dkd, nd);
-    fillSpecD(b->d.ks_d, dks, nd);
-    fillScalarD(b->d.ex_d, dex, nd);
-    return handle(b.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!kd || !ks) { g_err = "BSDF_Phong: null reflectance"; return nullptr; }
+        auto b = std::make_unique<Bsdf>();
+        b->d.type = BSDF_PHONG;
+        for (int k = 0; k < 3; ++k) { b->d.kd[k] = kd[k]; b->d.ks[k] = ks[k]; }
+        b->d.ex = ex;
+        double v0 = ((double)kd[0] + kd[1] + kd[2]) / 3, v1 = ((double)ks[0] + ks[1] + ks[2]) / 3;
+        b->d.specW = (float)(v1 / (v0 + v1));
+        int nd = nderNow();
+        fillSpecD(b->d.kd_d, dkd, nd);
+        fillSpecD(b->d.ks_d, dks, nd);
+        fillScalarD(b->d.ex_d, dex, nd);
+        return handle(b.release());
+    });
 }
 dtrt_object *dtrt_bsdf_roughdielectric_create(float alpha, float int_ior, float ext_ior, const float *dalpha, const float *deta) {
-    auto b = std::make_unique<Bsdf>();
-    b->d.type = BSDF_ROUGHDIELECTRIC;
-    b->d.alpha = alpha < 1e-4f ? 1e-4f : alpha;   // microfacet.h:11-14
-    b->d.eta = int_ior / ext_ior;
-    int nd = nderNow();
-    fillScalarD(b->d.alpha_d, dalpha, nd);
-    fillScalarD(b->d.eta_d, deta, nd);
-    return handle(b.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        auto b = std::make_unique<Bsdf>();
+        b->d.type = BSDF_ROUGHDIELECTRIC;
+        b->d.alpha = alpha < 1e-4f ? 1e-4f : alpha;   // microfacet.h:11-14
+        b->d.eta = int_ior / ext_ior;
+        int nd = nderNow();
+        fillScalarD(b->d.alpha_d, dalpha, nd);
+        fillScalarD(b->d.eta_d, deta, nd);
+        return handle(b.release());
+    });
 }
 dtrt_object *dtrt_phase_hg_create(float g, const float *dg) {
-    auto p = std::make_unique<Phase>();
-    p->d.type = PHASE_HG;
-    p->d.g = g;
-    fillScalarD(p->d.g_d, dg, nderNow());
-    return handle(p.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        auto p = std::make_unique<Phase>();
+        p->d.type = PHASE_HG;
+        p->d.g = g;
+        fillScalarD(p->d.g_d, dg, nderNow());
+        return handle(p.release());
+    });
 }
 dtrt_object *dtrt_phase_isotropic_create(void) { auto p = std::make_unique<Phase>(); p->d.type = PHASE_ISO; return handle(p.release()); }
 dtrt_object *dtrt_area_light_create(int shape_id, const float *I, int two_sided) {
-    if (!I) { g_err = "AreaLight: null intensity"; return nullptr; }
-    auto e = std::make_unique<Emitter>();
-    e->d.type = EMIT_AREA; e->d.shape_id = shape_id; e->d.two_sided = two_sided != 0;
-    for (int k = 0; k < 3; ++k) e->d.I[k] = I[k];
-    return handle(e.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!I) { g_err = "AreaLight: null intensity"; return nullptr; }
+        auto e = std::make_unique<Emitter>();
+        e->d.type = EMIT_AREA; e->d.shape_id = shape_id; e->d.two_sided = two_sided != 0;
+        for (int k = 0; k < 3; ++k) e->d.I[k] = I[k];
+        return handle(e.release());
+    });
 }
 dtrt_object *dtrt_area_light_ex_create(int shape_id, const float *I, float kappa, const float *dI, const float *dk) {
-    if (!I) { g_err = "AreaLightEx: null intensity"; return nullptr; }
-    auto e = std::make_unique<Emitter>();
-    e->d.type = EMIT_AREA_EX; e->d.shape_id = shape_id; e->d.kappa = kappa;
-    for (int k = 0; k < 3; ++k) e->d.I[k] = I[k];
-    int nd = nderNow();
-    fillSpecD(e->d.I_d, dI, nd);
-    fillScalarD(e->d.kappa_d, dk, nd);
-    return handle(e.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!I) { g_err = "AreaLightEx: null intensity"; return nullptr; }
+        auto e = std::make_unique<Emitter>();
+        e->d.type = EMIT_AREA_EX; e->d.shape_id = shape_id; e->d.kappa = kappa;
+        for (int k = 0; k < 3; ++k) e->d.I[k] = I[k];
+        int nd = nderNow();
+        fillSpecD(e->d.I_d, dI, nd);
+        fillScalarD(e->d.kappa_d, dk, nd);
+        return handle(e.release());
+    });
 }
 dtrt_object *dtrt_homogeneous_create(float sigma_t, const float *alb, int phase_id, const float *dsig, const float *dalb) {
-    if (!alb) { g_err = "Homogeneous: null albedo"; return nullptr; }
-    auto m = std::make_unique<Medium>();
-    m->d.type = MED_HOMOGENEOUS; m->d.phase_id = phase_id; m->d.sigma_t = sigma_t;
-    for (int k = 0; k < 3; ++k) m->d.albedo[k] = alb[k];
-    m->d.sampling_weight = std::max(alb[0], std::max(alb[1], alb[2]));
-    int nd = nderNow();
-    fillScalarD(m->d.sigma_t_d, dsig, nd);
-    fillSpecD(m->d.albedo_d, dalb, nd);
-    return handle(m.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!alb) { g_err = "Homogeneous: null albedo"; return nullptr; }
+        auto m = std::make_unique<Medium>();
+        m->d.type = MED_HOMOGENEOUS; m->d.phase_id = phase_id; m->d.sigma_t = sigma_t;
+        for (int k = 0; k < 3; ++k) m->d.albedo[k] = alb[k];
+        m->d.sampling_weight = std::max(alb[0], std::max(alb[1], alb[2]));
+        int nd = nderNow();
+        fillScalarD(m->d.sigma_t_d, dsig, nd);
+        fillSpecD(m->d.albedo_d, dalb, nd);
+        return handle(m.release());
+    });
 }
 dtrt_object *dtrt_heterogeneous_create(const char *file, const float *to_world, float scale, const float *albedo3,
                                        const char *albedo_file, int phase_id, const float *trans, const float *rot,
                                        const float *dscale, const float *dalb) {
-    if (!file) { g_err = "Heterogeneous: null density file"; return nullptr; }
-    auto m = std::make_unique<Medium>();
-    int ch = 0;
-    float mn[3], mx[3];
-    if (!readVol(file, m->density, m->d.res, ch, mn, mx)) return nullptr;
-    if (ch != 1) { g_err = "Heterogeneous: density grid must have one channel"; return nullptr; }
-    if (albedo_file) {
-        int ach = 0;
-        float amn[3], amx[3];
-        if (!readVol(albedo_file, m->albedo, m->d.ares, ach, amn, amx)) return nullptr;
-        if (ach != 3) { g_err = "Heterogeneous: albedo grid must have three channels"; return nullptr; }
-        for (int a = 0; a < 3; ++a) { m->d.amin[a] = amn[a]; m->d.agscale[a] = (float)((double)(m->d.ares[a] - 1) / ((double)amx[a] - amn[a])); }
-        albedo3 = nullptr;   // the reference leaves the constant albedo zero in this variant
-    }
-    if (!finishHetero(*m, to_world, scale, albedo3, phase_id, trans, rot, dscale, dalb, mn, mx)) return nullptr;
-    return handle(m.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!file) { g_err = "Heterogeneous: null density file"; return nullptr; }
+        auto m = std::make_unique<Medium>();
+        int ch = 0;
+        float mn[3], mx[3];
+        if (!readVol(file, m->density, m->d.res, ch, mn, mx)) return nullptr;
+        if (ch != 1) { g_err = "Heterogeneous: density grid must have one channel"; return nullptr; }
+        if (albedo_file) {
+            int ach = 0;
+            float amn[3], amx[3];
+            if (!readVol(albedo_file, m->albedo, m->d.ares, ach, amn, amx)) return nullptr;
+            if (ach != 3) { g_err = "Heterogeneous: albedo grid must have three channels"; return nullptr; }
+            for (int a = 0; a < 3; ++a) { m->d.amin[a] = amn[a]; m->d.agscale[a] = (float)((double)(m->d.ares[a] - 1) / ((double)amx[a] - amn[a])); }
+            albedo3 = nullptr;   // the reference leaves the constant albedo zero in this variant
+        }
+        if (!finishHetero(*m, to_world, scale, albedo3, phase_id, trans, rot, dscale, dalb, mn, mx)) return nullptr;
+        return handle(m.release());
+    });
 }
 dtrt_object *dtrt_heterogeneous_create_grid(const float *density, int rx, int ry, int rz, const float *bmin, const float *bmax,
                                             const float *to_world, float scale, const float *albedo3, int phase_id,
                                             const float *trans, const float *rot, const float *dscale, const float *dalb) {
-    if (!density || rx < 2 || ry < 2 || rz < 2 || !bmin || !bmax) { g_err = "grid: bad arguments"; return nullptr; }
-    auto m = std::make_unique<Medium>();
-    m->d.res[0] = rx; m->d.res[1] = ry; m->d.res[2] = rz;
-    m->density.assign(density, density + (size_t)rx * ry * rz);
-    for (float &x : m->density) x = std::max(x, 0.0f);
-    if (!finishHetero(*m, to_world, scale, albedo3, phase_id, trans, rot, dscale, dalb, bmin, bmax)) return nullptr;
-    return handle(m.release());
+    return guarded<dtrt_object *>(nullptr, [&]() -> dtrt_object * {
+        if (!density || rx < 2 || ry < 2 || rz < 2 || !bmin || !bmax) { g_err = "grid: bad arguments"; return nullptr; }
+        auto m = std::make_unique<Medium>();
+        m->d.res[0] = rx; m->d.res[1] = ry; m->d.res[2] = rz;
+        m->density.assign(density, density + (size_t)rx * ry * rz);
+        for (float &x : m->density) x = std::max(x, 0.0f);
+        if (!finishHetero(*m, to_world, scale, albedo3, phase_id, trans, rot, dscale, dalb, bmin, bmax)) return nullptr;
+        return handle(m.release());
+    });
 }
 
 dtrt_object *dtrt_scene_create(dtrt_object *camera, dtrt_object *const *shapes, int ns, dtrt_object *const *bsdfs, int nb,
@@ -1759,9 +1827,11 @@ dtrt_object *dtrt_scene_create(dtrt_object *camera, dtrt_object *const *shapes, 
     }
 }
 int dtrt_scene_init_edges(dtrt_object *scene, const float *w) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || !w) { if (s) g_err = "initEdges: null weights"; return -1; }
-    return s->initEdges(w) ? 0 : -1;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || !w) { if (s) g_err = "initEdges: null weights"; return -1; }
+        return s->initEdges(w) ? 0 : -1;
+    });
 }
 
 int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
@@ -1806,9 +1876,11 @@ int dtrt_scene_capture_views(dtrt_object *scene, int nviews, dtrt_object *const 
 }
 
 int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || !seeds) { if (s) g_err = "replay_views: bad arguments"; return -1; }
-    return s->replayViews(seeds);
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || !seeds) { if (s) g_err = "replay_views: bad arguments"; return -1; }
+        return s->replayViews(seeds);
+    });
 }
 
 static int renderCommon(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *img, bool dev) {
@@ -1835,9 +1907,18 @@ int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_opt
     return renderCommon(integrator, scene, o, image, true);
 }
 int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || !out) return -1;
+        for (int i = 0; i < 10; ++i) out[i] = s->last_stats[i];
+        return 0;
+    });
+}
+
+int dtrt_scene_set_stream(dtrt_object *scene, void *stream) {
     Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || !out) return -1;
-    for (int i = 0; i < 10; ++i) out[i] = s->last_stats[i];
+    if (!s) return -1;
+    s->caller = static_cast<hipStream_t>(stream);
     return 0;
 }
 
@@ -1856,9 +1937,11 @@ int dtrt_scene_autotune(int integrator, dtrt_object *scene, const dtrt_render_op
 }
 
 int dtrt_scene_set_density(dtrt_object *scene, int medium, const float *density, int on_device) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s) return -1;
-    return s->setDensity(medium, density, on_device != 0) ? 0 : -1;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s) return -1;
+        return s->setDensity(medium, density, on_device != 0) ? 0 : -1;
+    });
 }
 
 int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
@@ -1882,150 +1965,185 @@ int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dt
 
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
                         const float *d_image_device, float *d_density_device) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || !options) { g_err = "render_adjoint: bad arguments"; return -1; }
-    if (integrator != 0) { g_err = "render_adjoint: only VolPathTracerAD (0) is implemented"; return -1; }
-    return s->renderAdjoint(*options, medium, d_image_device, d_density_device) ? 0 : -1;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || !options) { g_err = "render_adjoint: bad arguments"; return -1; }
+        if (integrator != 0) { g_err = "render_adjoint: only VolPathTracerAD (0) is implemented"; return -1; }
+        return s->renderAdjoint(*options, medium, d_image_device, d_density_device) ? 0 : -1;
+    });
 }
 
 // ------------------------------------------------------------- test / bench hooks
 int dtrt_test_set_voxel_lanes(dtrt_object *scene, int medium, const int *voxels, int n) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s) return -1;
-    return s->setVoxelLanes(medium, voxels, n) ? 0 : -1;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s) return -1;
+        return s->setVoxelLanes(medium, voxels, n) ? 0 : -1;
+    });
 }
 
 int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s) return -1;
-    s->queue_budget = bytes ? (size_t)bytes : (size_t(6) << 30);
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s) return -1;
+        s->queue_budget = bytes ? (size_t)bytes : (size_t(6) << 30);
+        return 0;
+    });
 }
 
 int dtrt_test_kernel_times(dtrt_object *scene, double *ms4, int64_t *launches4, uint64_t *work12) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s) return -1;
-    for (int i = 0; i < 4; ++i) { ms4[i] = s->kernel_ms[i]; launches4[i] = s->kernel_launches[i]; }
-    for (int i = 0; i < 12; ++i) work12[i] = s->work_counts[i];
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s) return -1;
+        for (int i = 0; i < 4; ++i) { ms4[i] = s->kernel_ms[i]; launches4[i] = s->kernel_launches[i]; }
+        for (int i = 0; i < 12; ++i) work12[i] = s->work_counts[i];
+        return 0;
+    });
 }
 int dtrt_test_build_bvh(const float *tri9, const int *shape_of, int ntri, int64_t *out8) {
-    if (!tri9 || !out8 || ntri <= 0) {
-        fail("build_bvh: empty input");
-        return -1;
-    }
-    std::vector<std::array<float, 9>> tri((size_t)ntri);
-    for (int i = 0; i < ntri; ++i)
-        for (int k = 0; k < 9; ++k) tri[i][k] = tri9[(size_t)i * 9 + k];
-    std::vector<int> sh;
-    if (shape_of) sh.assign(shape_of, shape_of + ntri);
-    BvhOut b;
-    if (!buildBvh(tri, sh, b)) return -1;
-    // walk the GPU nodes: depth, pushes, leaf sizes, root child sizes
-    const int nn = (int)(b.nodes.size() / 4);
-    int64_t maxDepth = 0, leaves = 0, maxLeaf = 0, rootCount[2] = {0, 0};
-    struct W { int link, depth, root; };
-    std::vector<W> stack{{0, 1, -1}};
-    while (!stack.empty()) {
-        const W w = stack.back();
-        stack.pop_back();
-        if (w.link < 0) {
-            const int enc = ~w.link, cnt = enc & 15;
-            if (cnt == 0) continue;   // the empty slot of a single-leaf root
-            ++leaves;
-            maxLeaf = std::max<int64_t>(maxLeaf, cnt);
-            maxDepth = std::max<int64_t>(maxDepth, w.depth);
-            if (w.root >= 0) rootCount[w.root] += cnt;
-            continue;
+    return guarded<int>(-1, [&]() -> int {
+        if (!tri9 || !out8 || ntri <= 0) {
+            fail("build_bvh: empty input");
+            return -1;
         }
-        const float4 *n = b.nodes.data() + 4 * (size_t)w.link;
-        int c[2];
-        std::memcpy(&c[0], &n[0].w, 4);
-        std::memcpy(&c[1], &n[1].w, 4);
-        for (int k = 0; k < 2; ++k) stack.push_back({c[k], w.depth + 1, w.link == 0 ? k : w.root});
-    }
-    out8[0] = nn; out8[1] = maxDepth; out8[2] = rootCount[0]; out8[3] = rootCount[1];
-    out8[4] = maxDepth - 1; out8[5] = leaves; out8[6] = maxLeaf; out8[7] = 0;
-    return 0;
+        std::vector<std::array<float, 9>> tri((size_t)ntri);
+        for (int i = 0; i < ntri; ++i)
+            for (int k = 0; k < 9; ++k) tri[i][k] = tri9[(size_t)i * 9 + k];
+        std::vector<int> sh;
+        if (shape_of) sh.assign(shape_of, shape_of + ntri);
+        BvhOut b;
+        if (!buildBvh(tri, sh, b)) return -1;
+        // walk the GPU nodes: depth, pushes, leaf sizes, root child sizes
+        const int nn = (int)(b.nodes.size() / 4);
+        int64_t maxDepth = 0, leaves = 0, maxLeaf = 0, rootCount[2] = {0, 0};
+        struct W { int link, depth, root; };
+        std::vector<W> stack{{0, 1, -1}};
+        while (!stack.empty()) {
+            const W w = stack.back();
+            stack.pop_back();
+            if (w.link < 0) {
+                const int enc = ~w.link, cnt = enc & 15;
+                if (cnt == 0) continue;   // the empty slot of a single-leaf root
+                ++leaves;
+                maxLeaf = std::max<int64_t>(maxLeaf, cnt);
+                maxDepth = std::max<int64_t>(maxDepth, w.depth);
+                if (w.root >= 0) rootCount[w.root] += cnt;
+                continue;
+            }
+            const float4 *n = b.nodes.data() + 4 * (size_t)w.link;
+            int c[2];
+            std::memcpy(&c[0], &n[0].w, 4);
+            std::memcpy(&c[1], &n[1].w, 4);
+            for (int k = 0; k < 2; ++k) stack.push_back({c[k], w.depth + 1, w.link == 0 ? k : w.root});
+        }
+        out8[0] = nn; out8[1] = maxDepth; out8[2] = rootCount[0]; out8[3] = rootCount[1];
+        out8[4] = maxDepth - 1; out8[5] = leaves; out8[6] = maxLeaf; out8[7] = 0;
+        return 0;
+    });
 }
 int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s) return -1;
-    int64_t ne = 0;
-    for (Shape *sh : s->shapes) ne += (int64_t)sh->edges.size();
-    Scene::PECache pe;
-    if (!s->primaryEdges(s->cam, pe)) return -1;
-    out8[0] = s->S.nnodes; out8[1] = s->S.ntris; out8[2] = ne; out8[3] = pe.npe;
-    std::memcpy(&out8[4], &pe.pe_sum, 8);
-    out8[5] = s->S.edges_valid; out8[6] = s->S.NV; out8[7] = s->S.NF;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s) return -1;
+        int64_t ne = 0;
+        for (Shape *sh : s->shapes) ne += (int64_t)sh->edges.size();
+        Scene::PECache pe;
+        if (!s->primaryEdges(s->cam, pe)) return -1;
+        out8[0] = s->S.nnodes; out8[1] = s->S.ntris; out8[2] = ne; out8[3] = pe.npe;
+        std::memcpy(&out8[4], &pe.pe_sum, 8);
+        out8[5] = s->S.edges_valid; out8[6] = s->S.NV; out8[7] = s->S.NF;
+        return 0;
+    });
+}
+int dtrt_test_medium_data(dtrt_object *medium, int which, float *out, int64_t cap, double *meta8) {
+    Medium *m = as<Medium>(medium, T_MEDIUM);
+    if (!m || !meta8) return -1;
+    const std::vector<float> &v = which ? m->albedo : m->density;
+    const int *res = which ? m->d.ares : m->d.res;
+    const double meta[8] = {(double)res[0], (double)res[1], (double)res[2], which ? 3.0 : 1.0, (double)v.size(),
+                            (double)m->d.maxDensity, (double)m->d.scale, 0.0};
+    std::memcpy(meta8, meta, sizeof(meta));
+    if (out) std::memcpy(out, v.data(), (size_t)std::min<int64_t>(cap, (int64_t)v.size()) * sizeof(float));
     return 0;
 }
 int dtrt_test_shape_edges(dtrt_object *shape, int *out_v0v1f0f1, double *out_len, int cap) {
-    Shape *s = as<Shape>(shape, T_SHAPE);
-    if (!s) return -1;
-    int n = (int)s->edges.size();
-    for (int i = 0; i < std::min(n, cap); ++i) {
-        out_v0v1f0f1[4 * i] = s->edges[i].v0; out_v0v1f0f1[4 * i + 1] = s->edges[i].v1;
-        out_v0v1f0f1[4 * i + 2] = s->edges[i].f0; out_v0v1f0f1[4 * i + 3] = s->edges[i].f1;
-        out_len[i] = s->edges[i].len;
-    }
-    return n;
+    return guarded<int>(-1, [&]() -> int {
+        Shape *s = as<Shape>(shape, T_SHAPE);
+        if (!s) return -1;
+        int n = (int)s->edges.size();
+        for (int i = 0; i < std::min(n, cap); ++i) {
+            out_v0v1f0f1[4 * i] = s->edges[i].v0; out_v0v1f0f1[4 * i + 1] = s->edges[i].v1;
+            out_v0v1f0f1[4 * i + 2] = s->edges[i].f0; out_v0v1f0f1[4 * i + 3] = s->edges[i].f1;
+            out_len[i] = s->edges[i].len;
+        }
+        return n;
+    });
 }
 
 int dtrt_test_trace(dtrt_object *scene, const float *rays8, int n, int *out_face, float *out_t) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || n < 0) return -1;
-    TmpFree tf;
-    float *dr, *dt_;
-    int *df;
-    if (!devTmp(tf.v, dr, (size_t)n * 8) || !devTmp(tf.v, df, n) || !devTmp(tf.v, dt_, n)) return -1;
-    if (hipMemcpy(dr, rays8, (size_t)n * 8 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
-    if (launch_trace(s->S, dr, n, df, dt_, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "trace failed"; return -1; }
-    (void)hipMemcpy(out_face, df, (size_t)n * 4, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(out_t, dt_, (size_t)n * 4, hipMemcpyDeviceToHost);
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || n < 0) return -1;
+        TmpFree tf;
+        float *dr, *dt_;
+        int *df;
+        if (!devTmp(tf.v, dr, (size_t)n * 8) || !devTmp(tf.v, df, n) || !devTmp(tf.v, dt_, n)) return -1;
+        if (hipMemcpy(dr, rays8, (size_t)n * 8 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+        if (launch_trace(s->S, dr, n, df, dt_, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "trace failed"; return -1; }
+        (void)hipMemcpy(out_face, df, (size_t)n * 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(out_t, dt_, (size_t)n * 4, hipMemcpyDeviceToHost);
+        return 0;
+    });
 }
 int dtrt_test_li(dtrt_object *scene, const float *rays6, int n, int depth, int med, uint64_t seed, float *out3) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || n < 0 || med >= (int)s->media.size()) return -1;
-    TmpFree tf;
-    float *dr, *dout;
-    if (!devTmp(tf.v, dr, (size_t)n * 6) || !devTmp(tf.v, dout, (size_t)n * 3)) return -1;
-    if (hipMemcpy(dr, rays6, (size_t)n * 6 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
-    if (launch_li(s->S, dr, n, depth, med, seed, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "li failed"; return -1; }
-    (void)hipMemcpy(out3, dout, (size_t)n * 3 * 4, hipMemcpyDeviceToHost);
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || n < 0 || med >= (int)s->media.size()) return -1;
+        TmpFree tf;
+        float *dr, *dout;
+        if (!devTmp(tf.v, dr, (size_t)n * 6) || !devTmp(tf.v, dout, (size_t)n * 3)) return -1;
+        if (hipMemcpy(dr, rays6, (size_t)n * 6 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+        if (launch_li(s->S, dr, n, depth, med, seed, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "li failed"; return -1; }
+        (void)hipMemcpy(out3, dout, (size_t)n * 3 * 4, hipMemcpyDeviceToHost);
+        return 0;
+    });
 }
 int dtrt_test_medium(dtrt_object *scene, int med, int kind, const float *seg8, int n, uint64_t seed, float *out) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || med < 0 || med >= (int)s->media.size() || n < 0) { g_err = "medium test: bad arguments"; return -1; }
-    TmpFree tf;
-    float *dr, *dout;
-    if (!devTmp(tf.v, dr, (size_t)n * 8) || !devTmp(tf.v, dout, n)) return -1;
-    if (hipMemcpy(dr, seg8, (size_t)n * 8 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
-    if (launch_medium(s->S, med, kind, dr, n, seed, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "medium failed"; return -1; }
-    (void)hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost);
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || med < 0 || med >= (int)s->media.size() || n < 0) { g_err = "medium test: bad arguments"; return -1; }
+        TmpFree tf;
+        float *dr, *dout;
+        if (!devTmp(tf.v, dr, (size_t)n * 8) || !devTmp(tf.v, dout, n)) return -1;
+        if (hipMemcpy(dr, seg8, (size_t)n * 8 * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+        if (launch_medium(s->S, med, kind, dr, n, seed, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "medium failed"; return -1; }
+        (void)hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost);
+        return 0;
+    });
 }
 int dtrt_test_grid(dtrt_object *scene, int med, const float *pts, int n, float *out) {
-    Scene *s = as<Scene>(scene, T_SCENE);
-    if (!s || med < 0 || med >= (int)s->media.size() || s->media[med]->d.type != MED_HETEROGENEOUS || n < 0) { g_err = "grid test: bad arguments"; return -1; }
-    const int nd = s->nder;
-    TmpFree tf;
-    float *dp, *dout;
-    if (!devTmp(tf.v, dp, (size_t)n * (3 + 3 * nd)) || !devTmp(tf.v, dout, (size_t)n * (1 + nd))) return -1;
-    if (hipMemcpy(dp, pts, (size_t)n * (3 + 3 * nd) * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
-    if (launch_grid(s->S, med, dp, n, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "grid failed"; return -1; }
-    (void)hipMemcpy(out, dout, (size_t)n * (1 + nd) * 4, hipMemcpyDeviceToHost);
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s || med < 0 || med >= (int)s->media.size() || s->media[med]->d.type != MED_HETEROGENEOUS || n < 0) { g_err = "grid test: bad arguments"; return -1; }
+        const int nd = s->nder;
+        TmpFree tf;
+        float *dp, *dout;
+        if (!devTmp(tf.v, dp, (size_t)n * (3 + 3 * nd)) || !devTmp(tf.v, dout, (size_t)n * (1 + nd))) return -1;
+        if (hipMemcpy(dp, pts, (size_t)n * (3 + 3 * nd) * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+        if (launch_grid(s->S, med, dp, n, dout, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) { g_err = "grid failed"; return -1; }
+        (void)hipMemcpy(out, dout, (size_t)n * (1 + nd) * 4, hipMemcpyDeviceToHost);
+        return 0;
+    });
 }
 int dtrt_test_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
-    TmpFree tf;
-    uint32_t *d;
-    if (!devTmp(tf.v, d, 4)) return -1;
-    if (launch_philox(ctr4, key2, d, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) { g_err = "philox failed"; return -1; }
-    (void)hipMemcpy(out4, d, 16, hipMemcpyDeviceToHost);
-    return 0;
+    return guarded<int>(-1, [&]() -> int {
+        TmpFree tf;
+        uint32_t *d;
+        if (!devTmp(tf.v, d, 4)) return -1;
+        if (launch_philox(ctr4, key2, d, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) { g_err = "philox failed"; return -1; }
+        (void)hipMemcpy(out4, d, 16, hipMemcpyDeviceToHost);
+        return 0;
+    });
 }
 
 }  // extern "C"

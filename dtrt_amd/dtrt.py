@@ -94,6 +94,17 @@ def _chk(rc, what):
         raise RuntimeError("%s: %s" % (what, _lib.dtrt_last_error().decode()))
 
 
+_lib.dtrt_scene_set_stream.argtypes = [_P, _P]
+
+
+def _after_torch(scene):
+    """Orders the scene's next device-memory call after the work torch has queued on its
+    current stream (zeroed outputs, loss gradients built just before the call)."""
+    import torch
+    if torch.cuda.is_initialized():
+        _chk(_lib.dtrt_scene_set_stream(scene._h, torch.cuda.current_stream().cuda_stream), "set_stream")
+
+
 def kernel_times(scene):
     import numpy as np
     ms = (C.c_double * 4)()
@@ -125,6 +136,7 @@ def render_views_adjoint(scene, cameras, options, medium, d_images, d_density, i
     cams = (_P * n)(*[None if c is None else c._h for c in (cameras or [None] * n)])
     opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
     flat = torch.cat([g.reshape(-1) for g in d_images]).float().contiguous()
+    _after_torch(scene)
     _chk(_lib.dtrt_render_views_adjoint(integrator, scene._h, n, cams, opts, int(medium), C.c_void_p(flat.data_ptr()),
                                         C.c_void_p(d_density.data_ptr())), "render_views_adjoint")
     torch.cuda.synchronize()
@@ -142,6 +154,8 @@ def render_views(scene, cameras, options, images, integrator=0):
     opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
     on_dev = hasattr(images[0], "data_ptr")
     ptrs = (_P * n)(*[im.data_ptr() if on_dev else im.ctypes.data for im in images])
+    if on_dev:
+        _after_torch(scene)
     _chk(_lib.dtrt_render_views(integrator, scene._h, n, cams, opts, ptrs, int(on_dev)), "render_views")
 
 
@@ -151,12 +165,14 @@ def capture_views(scene, cameras, options, images):
     cams = (_P * n)(*[None if c is None else c._h for c in (cameras or [None] * n)])
     opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
     ptrs = (_P * n)(*[im.data_ptr() for im in images])
+    _after_torch(scene)
     _chk(_lib.dtrt_scene_capture_views(scene._h, n, cams, opts, ptrs), "capture_views")
 
 
 def replay_views(scene, seeds):
     """``dtrt_scene_replay_views``: 0 = graph replay, 1 = eager fallback (overflow)."""
     s = (C.c_uint64 * len(seeds))(*[int(x) for x in seeds])
+    _after_torch(scene)
     rc = _lib.dtrt_scene_replay_views(scene._h, s)
     if rc < 0:
         raise RuntimeError("replay_views: " + _lib.dtrt_last_error().decode())
@@ -168,6 +184,8 @@ def set_density(scene, medium, density):
     import numpy as np
     if hasattr(density, "data_ptr"):
         t = density.detach().contiguous().float()
+        if t.is_cuda:
+            _after_torch(scene)
         _chk(_lib.dtrt_scene_set_density(scene._h, int(medium), t.data_ptr(), int(t.is_cuda)), "set_density")
     else:
         a = np.ascontiguousarray(density, dtype=np.float32)
@@ -177,6 +195,7 @@ def set_density(scene, medium, density):
 def render_adjoint(scene, options, medium, d_image_ptr, d_density_ptr, integrator=0):
     """``dtrt_render_adjoint``: device pointers to d loss/d image [H][W][3] and the accumulated grid gradient."""
     st = options._struct()
+    _after_torch(scene)
     _chk(_lib.dtrt_render_adjoint(integrator, scene._h, C.byref(st), int(medium), d_image_ptr, d_density_ptr), "render_adjoint")
 
 
@@ -265,4 +284,5 @@ def philox(ctr, key):
 def render_device(integrator, scene, options, out_ptr):
     """``dtrt_render_device``: ``out_ptr`` is a device address (e.g. ``tensor.data_ptr()`` of a HIP tensor)."""
     st = options._struct()
+    _after_torch(scene)
     _chk(_lib.dtrt_render_device(integrator, scene._h, C.byref(st), out_ptr), "render_device")

@@ -200,6 +200,15 @@ int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dt
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
                         const float *d_image_device, float *d_density_device);
 
+/* No pybind counterpart (the reference's render reads host memory only). Names the
+ * caller's HIP stream (hipStream_t; NULL = the legacy null stream, the default). Every
+ * later call on `scene` that reads or writes caller device memory (dtrt_render_device,
+ * device images of dtrt_render_views, the adjoints, dtrt_scene_set_density from device
+ * memory, graph replays) first waits for the work queued on that stream, so a zeroed
+ * output or a loss gradient the caller has just queued is complete before the library
+ * reads it. The calls return after their own work has finished. */
+int dtrt_scene_set_stream(dtrt_object *scene, void *stream);
+
 /* Counters of the last render on `scene` (the warnings the reference prints):
  * out[0] negative pixels, [1] non-finite tangents, [2] delta-tracking caps,
  * [3] invalid edge weights, [4] non-orthogonal edge tangents, [5] medium

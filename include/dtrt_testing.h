@@ -34,6 +34,11 @@ int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes);
 /* [0] BVH nodes, [1] triangles, [2] edges, [3] primary edges, [4] primary-edge
  * length sum (double bits), [5] secondary edges usable, [6] vertices, [7] faces. */
 int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8);
+/* Host copy of a heterogeneous medium's grid as the constructor stored it: which = 0
+ * density [z][y][x], 1 albedo [z][y][x][3]; up to `cap` floats into out (may be NULL).
+ * meta8: {rx, ry, rz, channels, floats stored, max density, scale, 0} (doubles).
+ * Returns 0, or -1 for a handle that is not a medium. */
+int dtrt_test_medium_data(dtrt_object *medium, int which, float *out, int64_t cap, double *meta8);
 /* Edge list of a shape (v0, v1, f0, f1 local ids; f1 = -1 on boundaries) and lengths;
  * returns the number of edges. */
 int dtrt_test_shape_edges(dtrt_object *shape, int *out_v0v1f0f1, double *out_len, int cap);

@@ -125,3 +125,17 @@ def test_bvh_builder_bounds_the_traversal_stack():
         b = _bvh(tris)
         assert b["pushes"] <= 32 and b["max_leaf"] <= 15 and b["root"][0] + b["root"][1] == len(tris)
     assert _bvh(soup[:2])["leaves"] == 1
+
+
+def test_bvh_builder_depth_budget_on_geometric_spacing():
+    """2,000 small triangles spaced geometrically along x (ratio 1.02): SAH peels the
+    sparse end off one triangle at a time, so the builder reaches its depth budget with
+    nodes that fit a leaf. Those become leaves; the build succeeds within the stack."""
+    x = np.cumsum(1.02 ** np.arange(2000)).astype(np.float32) * 1e-3
+    tri = np.zeros((2000, 9), np.float32)
+    tri[:, 0] = x
+    tri[:, 3] = x + 1e-4
+    tri[:, 7] = 1e-4
+    tri[:, 6] = x
+    b = _bvh(tri)
+    assert b["pushes"] <= 32 and b["max_leaf"] <= 15 and b["root"][0] + b["root"][1] == 2000
