@@ -23,16 +23,25 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-munsafe-fp-at
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include")]
 
 
-# kernels.hip is compiled once per dual width N (its AD, child, primary-edge,
-# adjoint and finalize kernels; each object holds its own frame-rotation constant)
-# and once without DT_N (the width-independent kernels and the dispatchers).
+# kernels.hip is compiled once per estimator precision (DT_REAL double in namespace
+# dt::p64, float in dt::p32) and, within each, once per dual width N (its AD, child,
+# primary-edge, adjoint and finalize kernels; each object holds its own
+# frame-rotation constant) and once without DT_N (the width-independent kernels and
+# the dispatchers over nder). dispatch.cpp picks the precision, host.cpp is the C-ABI.
 WIDTHS = (16, 8, 4, 3, 2, 1)   # widest first: the longest compiles start first
+PRECISIONS = (("p64", "double"), ("p32", "float"))
 
 
 def _jobs():
     kern = os.path.join(SRC, "kernels.hip")
-    jobs = [(kern, "kernels_n%d" % n, ["-DDT_N=%d" % n, "-DDT_FRAME_ROT=c_frame_rot_n%d" % n]) for n in WIDTHS]
-    jobs.append((kern, "kernels", []))
+    jobs = []
+    for ns, real in PRECISIONS:
+        prec = ["-DDT_REAL=" + real, "-DDT_PNS=" + ns]
+        jobs += [(kern, "kernels_%s_n%d" % (ns, n), prec + ["-DDT_N=%d" % n, "-DDT_FRAME_ROT=c_frame_rot_n%d" % n])
+                 for n in WIDTHS]
+    for ns, real in PRECISIONS:
+        jobs.append((kern, "kernels_" + ns, ["-DDT_REAL=" + real, "-DDT_PNS=" + ns]))
+    jobs.append((os.path.join(SRC, "dispatch.cpp"), "dispatch", []))
     jobs.append((os.path.join(SRC, "host.cpp"), "host", []))
     return jobs
 

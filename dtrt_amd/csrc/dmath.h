@@ -1,7 +1,13 @@
-// Device math layer for the gfx950 path tracer: fp32 dual numbers with N tangent
-// lanes, a Philox4x32-10 counter-based RNG, frames, warps, Fresnel and the
-// Moller-Trumbore refinement. Semantics follow the reference's SimpleAD
-// (include/simpleAD.h:20-1040) and utils (src/utils.cpp); the arithmetic is fp32.
+// Device math layer for the gfx950 path tracer: dual numbers with N tangent lanes, a
+// Philox4x32-10 counter-based RNG, frames, warps, Fresnel and the Moller-Trumbore
+// refinement. Semantics follow the reference's SimpleAD (include/simpleAD.h:20-1040)
+// and utils (src/utils.cpp).
+//
+// The estimator's scalar type `real` is fixed per compilation: DT_REAL = double
+// (namespace dt::p64, the reference's own precision, include/fwd.h:10-13) or float
+// (dt::p32). Scene data stays fp32 in memory (the reference reads fp32 vertices,
+// grids and parameters too) and is widened on load; BVH traversal is fp32 in both,
+// as Embree's is (src/scene.cpp:17-43), and every hit is refined in `real`.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -9,61 +15,76 @@
 #define DI __device__ __forceinline__
 #define HDI __host__ __device__ __forceinline__
 
-namespace dt {
+#ifndef DT_REAL
+#define DT_REAL float
+#endif
+#ifndef DT_PNS
+#define DT_PNS p32
+#endif
 
-// fwd.h:14, config.h:6-7. The fp32 build keeps the reference's values; tests that
-// are meaningless at fp32 resolution are noted where they are made.
-constexpr float EPS = 1e-7f;
-constexpr float SHADOW_EPS = 1e-3f;
-constexpr float ANGLE_EPS = 1e-4f;
-constexpr float PI = 3.14159265358979323846f;
-constexpr float INV_PI = 0.31830988618379067154f;
-constexpr float INV_TWOPI = 0.15915494309189534561f;
-constexpr float INV_FOURPI = 0.07957747154594766788f;
-constexpr float FINF = __builtin_huge_valf();
+namespace dt {
+namespace DT_PNS {
+
+using real = DT_REAL;
+constexpr bool kF64 = sizeof(real) == 8;
+// `1.5_r`: a literal of the estimator's type
+constexpr real operator""_r(long double v) { return (real)v; }
+constexpr real operator""_r(unsigned long long v) { return (real)v; }
+DI void rsincos(float a, float *s, float *c) { sincosf(a, s, c); }
+DI void rsincos(double a, double *s, double *c) { sincos(a, s, c); }
+
+// fwd.h:14, config.h:6-7.
+constexpr real EPS = 1e-7_r;
+constexpr real SHADOW_EPS = 1e-3_r;
+constexpr real ANGLE_EPS = 1e-4_r;
+constexpr real PI = 3.14159265358979323846_r;
+constexpr real INV_PI = 0.31830988618379067154_r;
+constexpr real INV_TWOPI = 0.15915494309189534561_r;
+constexpr real INV_FOURPI = 0.07957747154594766788_r;
+constexpr real FINF = (real)__builtin_huge_val();
 
 // ------------------------------------------------------------------- float3
 struct f3 {
-    float x, y, z;
-    HDI float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    real x, y, z;
+    HDI real operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
 };
-HDI f3 mk(float a, float b, float c) { return f3{a, b, c}; }
+HDI f3 mk(real a, real b, real c) { return f3{a, b, c}; }
 HDI f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 HDI f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 HDI f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
-HDI f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-HDI f3 operator*(float s, f3 a) { return {a.x * s, a.y * s, a.z * s}; }
+HDI f3 operator*(f3 a, real s) { return {a.x * s, a.y * s, a.z * s}; }
+HDI f3 operator*(real s, f3 a) { return {a.x * s, a.y * s, a.z * s}; }
 HDI f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
-HDI f3 operator/(f3 a, float s) { float r = 1.0f / s; return {a.x * r, a.y * r, a.z * r}; }
-HDI float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+HDI f3 operator/(f3 a, real s) { real r = 1.0_r / s; return {a.x * r, a.y * r, a.z * r}; }
+HDI real dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 HDI f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-HDI float length(f3 a) { return sqrtf(dot(a, a)); }
-HDI f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
-HDI bool allZero(f3 a, float prec) { return fabsf(a.x) <= prec && fabsf(a.y) <= prec && fabsf(a.z) <= prec; }
-HDI float signum(float v) { return copysignf(1.0f, v); }
+HDI real length(f3 a) { return sqrt(dot(a, a)); }
+HDI f3 normalize(f3 a) { return a * (1.0_r / sqrt(dot(a, a))); }
+HDI bool allZero(f3 a, real prec) { return fabs(a.x) <= prec && fabs(a.y) <= prec && fabs(a.z) <= prec; }
+HDI real signum(real v) { return copysign(1.0_r, v); }
 
 // -------------------------------------------------------------- dual numbers
 template <int N> struct Df {
-    float v;
-    float d[N];
-    HDI Df() : v(0.f) {
+    real v;
+    real d[N];
+    HDI Df() : v(0._r) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) d[i] = 0.f;
+        for (int i = 0; i < N; ++i) d[i] = 0._r;
     }
-    HDI Df(float x) : v(x) {
+    HDI Df(real x) : v(x) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) d[i] = 0.f;
+        for (int i = 0; i < N; ++i) d[i] = 0._r;
     }
-    HDI bool gradZero(float prec) const {
+    HDI bool gradZero(real prec) const {
         bool z = true;
 #pragma unroll
-        for (int i = 0; i < N; ++i) z = z && fabsf(d[i]) <= prec;
+        for (int i = 0; i < N; ++i) z = z && fabs(d[i]) <= prec;
         return z;
     }
-    HDI float gradAbsMax() const {
-        float m = 0.f;
+    HDI real gradAbsMax() const {
+        real m = 0._r;
 #pragma unroll
-        for (int i = 0; i < N; ++i) m = fmaxf(m, fabsf(d[i]));
+        for (int i = 0; i < N; ++i) m = fmax(m, fabs(d[i]));
         return m;
     }
 };
@@ -73,20 +94,20 @@ template <int N> HDI Df<N> operator-(const Df<N> &a, const Df<N> &b) { Df<N> r(a
 template <int N> HDI Df<N> operator-(const Df<N> &a) { Df<N> r(-a.v); DF_LOOP r.d[i] = -a.d[i]; return r; }
 template <int N> HDI Df<N> operator*(const Df<N> &a, const Df<N> &b) { Df<N> r(a.v * b.v); DF_LOOP r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
 template <int N> HDI Df<N> operator/(const Df<N> &a, const Df<N> &b) {
-    float ib = 1.0f / b.v, q = a.v * ib;
+    real ib = 1.0_r / b.v, q = a.v * ib;
     Df<N> r(q);
     DF_LOOP r.d[i] = (a.d[i] - q * b.d[i]) * ib;
     return r;
 }
-template <int N> HDI Df<N> operator+(const Df<N> &a, float s) { Df<N> r = a; r.v += s; return r; }
-template <int N> HDI Df<N> operator+(float s, const Df<N> &a) { Df<N> r = a; r.v += s; return r; }
-template <int N> HDI Df<N> operator-(const Df<N> &a, float s) { Df<N> r = a; r.v -= s; return r; }
-template <int N> HDI Df<N> operator-(float s, const Df<N> &a) { Df<N> r(s - a.v); DF_LOOP r.d[i] = -a.d[i]; return r; }
-template <int N> HDI Df<N> operator*(const Df<N> &a, float s) { Df<N> r(a.v * s); DF_LOOP r.d[i] = a.d[i] * s; return r; }
-template <int N> HDI Df<N> operator*(float s, const Df<N> &a) { return a * s; }
-template <int N> HDI Df<N> operator/(const Df<N> &a, float s) { return a * (1.0f / s); }
-template <int N> HDI Df<N> operator/(float s, const Df<N> &a) {
-    float ia = 1.0f / a.v, q = s * ia;
+template <int N> HDI Df<N> operator+(const Df<N> &a, real s) { Df<N> r = a; r.v += s; return r; }
+template <int N> HDI Df<N> operator+(real s, const Df<N> &a) { Df<N> r = a; r.v += s; return r; }
+template <int N> HDI Df<N> operator-(const Df<N> &a, real s) { Df<N> r = a; r.v -= s; return r; }
+template <int N> HDI Df<N> operator-(real s, const Df<N> &a) { Df<N> r(s - a.v); DF_LOOP r.d[i] = -a.d[i]; return r; }
+template <int N> HDI Df<N> operator*(const Df<N> &a, real s) { Df<N> r(a.v * s); DF_LOOP r.d[i] = a.d[i] * s; return r; }
+template <int N> HDI Df<N> operator*(real s, const Df<N> &a) { return a * s; }
+template <int N> HDI Df<N> operator/(const Df<N> &a, real s) { return a * (1.0_r / s); }
+template <int N> HDI Df<N> operator/(real s, const Df<N> &a) {
+    real ia = 1.0_r / a.v, q = s * ia;
     Df<N> r(q);
     DF_LOOP r.d[i] = -q * ia * a.d[i];
     return r;
@@ -94,12 +115,12 @@ template <int N> HDI Df<N> operator/(float s, const Df<N> &a) {
 template <int N> HDI Df<N> &operator+=(Df<N> &a, const Df<N> &b) { a = a + b; return a; }
 template <int N> HDI Df<N> &operator*=(Df<N> &a, const Df<N> &b) { a = a * b; return a; }
 // simpleAD.h:202-230
-template <int N> HDI Df<N> dsqrt(const Df<N> &a) { float t = sqrtf(a.v), h = 0.5f / t; Df<N> r(t); DF_LOOP r.d[i] = a.d[i] * h; return r; }
-template <int N> HDI Df<N> dexp(const Df<N> &a) { float e = expf(a.v); Df<N> r(e); DF_LOOP r.d[i] = a.d[i] * e; return r; }
-template <int N> HDI Df<N> dlog(const Df<N> &a) { float ia = 1.0f / a.v; Df<N> r(logf(a.v)); DF_LOOP r.d[i] = a.d[i] * ia; return r; }
-template <int N> HDI Df<N> dpow(const Df<N> &a, float p) { float t = powf(a.v, p - 1.0f); Df<N> r(a.v * t); DF_LOOP r.d[i] = a.d[i] * p * t; return r; }
+template <int N> HDI Df<N> dsqrt(const Df<N> &a) { real t = sqrt(a.v), h = 0.5_r / t; Df<N> r(t); DF_LOOP r.d[i] = a.d[i] * h; return r; }
+template <int N> HDI Df<N> dexp(const Df<N> &a) { real e = exp(a.v); Df<N> r(e); DF_LOOP r.d[i] = a.d[i] * e; return r; }
+template <int N> HDI Df<N> dlog(const Df<N> &a) { real ia = 1.0_r / a.v; Df<N> r(log(a.v)); DF_LOOP r.d[i] = a.d[i] * ia; return r; }
+template <int N> HDI Df<N> dpow(const Df<N> &a, real p) { real t = pow(a.v, p - 1.0_r); Df<N> r(a.v * t); DF_LOOP r.d[i] = a.d[i] * p * t; return r; }
 template <int N> HDI Df<N> dpow(const Df<N> &a, const Df<N> &p) { return dexp(p * dlog(a)); }
-template <int N> HDI Df<N> dabs(const Df<N> &a) { return a.v > 0.f ? a : -a; }
+template <int N> HDI Df<N> dabs(const Df<N> &a) { return a.v > 0._r ? a : -a; }
 
 template <int N> struct Dv {
     Df<N> x, y, z;
@@ -109,14 +130,14 @@ template <int N> struct Dv {
     HDI f3 val() const { return f3{x.v, y.v, z.v}; }
     HDI f3 grad(int i) const { return f3{x.d[i], y.d[i], z.d[i]}; }
     HDI void setGrad(int i, const f3 &g) { x.d[i] = g.x; y.d[i] = g.y; z.d[i] = g.z; }
-    HDI bool gradZero(float prec) const { return x.gradZero(prec) && y.gradZero(prec) && z.gradZero(prec); }
+    HDI bool gradZero(real prec) const { return x.gradZero(prec) && y.gradZero(prec) && z.gradZero(prec); }
 };
 template <int N> HDI Dv<N> operator+(const Dv<N> &a, const Dv<N> &b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 template <int N> HDI Dv<N> operator-(const Dv<N> &a, const Dv<N> &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 template <int N> HDI Dv<N> operator-(const Dv<N> &a) { return {-a.x, -a.y, -a.z}; }
 template <int N> HDI Dv<N> operator*(const Dv<N> &a, const Df<N> &s) { return {a.x * s, a.y * s, a.z * s}; }
-template <int N> HDI Dv<N> operator*(const Dv<N> &a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-template <int N> HDI Dv<N> operator/(const Dv<N> &a, const Df<N> &s) { Df<N> r = 1.0f / s; return a * r; }
+template <int N> HDI Dv<N> operator*(const Dv<N> &a, real s) { return {a.x * s, a.y * s, a.z * s}; }
+template <int N> HDI Dv<N> operator/(const Dv<N> &a, const Df<N> &s) { Df<N> r = 1.0_r / s; return a * r; }
 template <int N> HDI Df<N> ddot(const Dv<N> &a, const Dv<N> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <int N> HDI Df<N> ddot(const Dv<N> &a, const f3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <int N> HDI Dv<N> dcross(const Dv<N> &a, const Dv<N> &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
@@ -130,12 +151,12 @@ template <int N> struct Ds {
     HDI Ds(const f3 &s) { c[0] = Df<N>(s.x); c[1] = Df<N>(s.y); c[2] = Df<N>(s.z); }
     HDI static Ds fill(const Df<N> &a) { Ds r; r.c[0] = a; r.c[1] = a; r.c[2] = a; return r; }
     HDI f3 val() const { return f3{c[0].v, c[1].v, c[2].v}; }
-    HDI bool isZero(float prec) const { return allZero(val(), prec); }
+    HDI bool isZero(real prec) const { return allZero(val(), prec); }
 };
 template <int N> HDI Ds<N> operator+(const Ds<N> &a, const Ds<N> &b) { Ds<N> r; for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] + b.c[k]; return r; }
 template <int N> HDI Ds<N> operator*(const Ds<N> &a, const Ds<N> &b) { Ds<N> r; for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] * b.c[k]; return r; }
 template <int N> HDI Ds<N> operator*(const Ds<N> &a, const Df<N> &s) { Ds<N> r; for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] * s; return r; }
-template <int N> HDI Ds<N> operator*(const Ds<N> &a, float s) { Ds<N> r; for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] * s; return r; }
+template <int N> HDI Ds<N> operator*(const Ds<N> &a, real s) { Ds<N> r; for (int k = 0; k < 3; ++k) r.c[k] = a.c[k] * s; return r; }
 template <int N> HDI Ds<N> &operator+=(Ds<N> &a, const Ds<N> &b) { a = a + b; return a; }
 
 // ------------------------------------------------------------------- Philox
@@ -174,7 +195,7 @@ struct Rng {
         blk = 0;
         pos = 4;
     }
-    HDI float next() {
+    HDI real next() {
         if (pos == 4) {
             u4 o = philox4x32_10(u4{s0, s1, blk++, 0x9e3779b9u}, k0, k1);
             b0 = o.a; b1 = o.b; b2 = o.c; b3 = o.d;
@@ -182,25 +203,26 @@ struct Rng {
         }
         uint32_t x = pos == 0 ? b0 : pos == 1 ? b1 : pos == 2 ? b2 : b3;
         pos++;
-        return (float)(x >> 8) * (1.0f / 16777216.0f);   // [0, 1) with 24 bits
+        if constexpr (kF64) return (real)x * (1.0_r / 4294967296.0_r);   // [0, 1) with 32 bits
+        else return (real)(x >> 8) * (1.0_r / 16777216.0_r);             // [0, 1) with 24 bits
     }
 };
 
 // ------------------------------------------------------------------- frames
 // utils.cpp:54-67 — orthonormal basis built after a fixed rotation of the
 // normal (0.1 rad about normalize(0.1,0.2,0.3)); R computed by Rodrigues on the host.
-struct FrameRot { float m[9]; };
+struct FrameRot { real m[9]; };
 // Each kernel object defines its own copy under its own name.
 #ifndef DT_FRAME_ROT
 #define DT_FRAME_ROT c_frame_rot
 #endif
 extern __constant__ FrameRot DT_FRAME_ROT;
 
-DI f3 rotMul(const float *m, f3 v, bool tr) {
+DI f3 rotMul(const real *m, f3 v, bool tr) {
     return tr ? f3{m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z, m[2] * v.x + m[5] * v.y + m[8] * v.z}
               : f3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z};
 }
-template <int N> DI Dv<N> rotMul(const float *m, const Dv<N> &v, bool tr) {
+template <int N> DI Dv<N> rotMul(const real *m, const Dv<N> &v, bool tr) {
     Dv<N> o;
     if (tr) {
         o.x = v.x * m[0] + v.y * m[3] + v.z * m[6];
@@ -217,14 +239,14 @@ struct Frame {
     f3 s, t, n;
     DI Frame() {}
     DI explicit Frame(f3 nn) : n(nn) {
-        const float *R = DT_FRAME_ROT.m;
+        const real *R = DT_FRAME_ROT.m;
         f3 q = rotMul(R, nn, false);
-        if (fabsf(q.x) > fabsf(q.y)) {
-            float il = 1.0f / sqrtf(q.x * q.x + q.z * q.z);
-            t = f3{q.z * il, 0.f, -q.x * il};
+        if (fabs(q.x) > fabs(q.y)) {
+            real il = 1.0_r / sqrt(q.x * q.x + q.z * q.z);
+            t = f3{q.z * il, 0._r, -q.x * il};
         } else {
-            float il = 1.0f / sqrtf(q.y * q.y + q.z * q.z);
-            t = f3{0.f, q.z * il, -q.y * il};
+            real il = 1.0_r / sqrt(q.y * q.y + q.z * q.z);
+            t = f3{0._r, q.z * il, -q.y * il};
         }
         s = rotMul(R, cross(t, q), true);
         t = rotMul(R, t, true);
@@ -236,14 +258,14 @@ template <int N> struct FrameAD {   // frameAD.h:9-82 via coordinateSystemAD (ut
     Dv<N> s, t, n;
     DI FrameAD() {}
     DI explicit FrameAD(const Dv<N> &nn) : n(nn) {
-        const float *R = DT_FRAME_ROT.m;
+        const real *R = DT_FRAME_ROT.m;
         Dv<N> q = rotMul(R, nn, false);
-        if (fabsf(q.x.v) > fabsf(q.y.v)) {
-            Df<N> il = 1.0f / dsqrt(q.x * q.x + q.z * q.z);
-            t = Dv<N>(q.z * il, Df<N>(0.f), -q.x * il);
+        if (fabs(q.x.v) > fabs(q.y.v)) {
+            Df<N> il = 1.0_r / dsqrt(q.x * q.x + q.z * q.z);
+            t = Dv<N>(q.z * il, Df<N>(0._r), -q.x * il);
         } else {
-            Df<N> il = 1.0f / dsqrt(q.y * q.y + q.z * q.z);
-            t = Dv<N>(Df<N>(0.f), q.z * il, -q.y * il);
+            Df<N> il = 1.0_r / dsqrt(q.y * q.y + q.z * q.z);
+            t = Dv<N>(Df<N>(0._r), q.z * il, -q.y * il);
         }
         s = rotMul(R, dcross(t, q), true);
         t = rotMul(R, t, true);
@@ -255,80 +277,80 @@ template <int N> struct FrameAD {   // frameAD.h:9-82 via coordinateSystemAD (ut
 
 // -------------------------------------------------------------------- warps
 // utils.cpp:18-52 (the reference's sin/cos naming swap in the concentric map is kept)
-DI f3 squareToCosineHemisphere(float sx, float sy) {
-    float r1 = 2.0f * sx - 1.0f, r2 = 2.0f * sy - 1.0f, phi, r;
-    if (r1 == 0.f && r2 == 0.f) { r = phi = 0.f; }
-    else if (r1 * r1 > r2 * r2) { r = r1; phi = (PI / 4.0f) * (r2 / r1); }
-    else { r = r2; phi = (PI / 2.0f) - (r1 / r2) * (PI / 4.0f); }
-    float sp, cp;
-    sincosf(phi, &sp, &cp);
-    float px = r * sp, py = r * cp;
-    return f3{px, py, sqrtf(fmaxf(0.f, 1.0f - px * px - py * py))};
+DI f3 squareToCosineHemisphere(real sx, real sy) {
+    real r1 = 2.0_r * sx - 1.0_r, r2 = 2.0_r * sy - 1.0_r, phi, r;
+    if (r1 == 0._r && r2 == 0._r) { r = phi = 0._r; }
+    else if (r1 * r1 > r2 * r2) { r = r1; phi = (PI / 4.0_r) * (r2 / r1); }
+    else { r = r2; phi = (PI / 2.0_r) - (r1 / r2) * (PI / 4.0_r); }
+    real sp, cp;
+    rsincos(phi, &sp, &cp);
+    real px = r * sp, py = r * cp;
+    return f3{px, py, sqrt(fmax(0._r, 1.0_r - px * px - py * py))};
 }
-DI f3 squareToUniformSphere(float sx, float sy) {
-    float z = 1.0f - 2.0f * sy, r = sqrtf(fmaxf(0.f, 1.0f - z * z)), sp, cp;
-    sincosf(2.0f * PI * sx, &sp, &cp);
+DI f3 squareToUniformSphere(real sx, real sy) {
+    real z = 1.0_r - 2.0_r * sy, r = sqrt(fmax(0._r, 1.0_r - z * z)), sp, cp;
+    rsincos(2.0_r * PI * sx, &sp, &cp);
     return f3{r * cp, r * sp, z};
 }
-HDI float luminance(f3 v) { return 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z; }
+HDI real luminance(f3 v) { return 0.212671_r * v.x + 0.715160_r * v.y + 0.072169_r * v.z; }
 
 // utils.cpp:119-148
-DI float fresnelDielectricExt(float ci_, float &ct_, float eta) {
-    if (fabsf(eta - 1.0f) < EPS) { ct_ = -ci_; return 0.f; }
-    float scale = ci_ > 0.f ? 1.0f / eta : eta;
-    float ct2 = 1.0f - (1.0f - ci_ * ci_) * (scale * scale);
-    if (ct2 < EPS) { ct_ = 0.f; return 1.0f; }
-    float ci = fabsf(ci_), ct = sqrtf(ct2);
-    float Rs = (ci - eta * ct) / (ci + eta * ct), Rp = (eta * ci - ct) / (eta * ci + ct);
-    ct_ = ci_ > 0.f ? -ct : ct;
-    return 0.5f * (Rs * Rs + Rp * Rp);
+DI real fresnelDielectricExt(real ci_, real &ct_, real eta) {
+    if (fabs(eta - 1.0_r) < EPS) { ct_ = -ci_; return 0._r; }
+    real scale = ci_ > 0._r ? 1.0_r / eta : eta;
+    real ct2 = 1.0_r - (1.0_r - ci_ * ci_) * (scale * scale);
+    if (ct2 < EPS) { ct_ = 0._r; return 1.0_r; }
+    real ci = fabs(ci_), ct = sqrt(ct2);
+    real Rs = (ci - eta * ct) / (ci + eta * ct), Rp = (eta * ci - ct) / (eta * ci + ct);
+    ct_ = ci_ > 0._r ? -ct : ct;
+    return 0.5_r * (Rs * Rs + Rp * Rp);
 }
-DI float fresnelDielectricExt(float ci, float eta) { float c; return fresnelDielectricExt(ci, c, eta); }
+DI real fresnelDielectricExt(real ci, real eta) { real c; return fresnelDielectricExt(ci, c, eta); }
 template <int N> DI Df<N> fresnelDielectricExtAD(const Df<N> &ci_, const Df<N> &eta) {   // utils.cpp:150-184
-    if (fabsf(eta.v - 1.0f) < EPS) return Df<N>(0.f);
-    Df<N> scale = ci_.v > 0.f ? 1.0f / eta : eta;
-    Df<N> ct2 = 1.0f - (1.0f - ci_ * ci_) * (scale * scale);
-    if (ct2.v < EPS) return Df<N>(1.0f);
+    if (fabs(eta.v - 1.0_r) < EPS) return Df<N>(0._r);
+    Df<N> scale = ci_.v > 0._r ? 1.0_r / eta : eta;
+    Df<N> ct2 = 1.0_r - (1.0_r - ci_ * ci_) * (scale * scale);
+    if (ct2.v < EPS) return Df<N>(1.0_r);
     Df<N> ci = dabs(ci_), ct = dsqrt(ct2);
     Df<N> Rs = (ci - eta * ct) / (ci + eta * ct), Rp = (eta * ci - ct) / (eta * ci + ct);
-    return 0.5f * (Rs * Rs + Rp * Rp);
+    return 0.5_r * (Rs * Rs + Rp * Rp);
 }
-DI f3 reflect(f3 wi, f3 n) { return 2.0f * dot(wi, n) * n - wi; }
-DI f3 refract(f3 wi, f3 n, float eta, float ct) { if (ct < 0.f) eta = 1.0f / eta; return n * (dot(wi, n) * eta + ct) - wi * eta; }
+DI f3 reflect(f3 wi, f3 n) { return 2.0_r * dot(wi, n) * n - wi; }
+DI f3 refract(f3 wi, f3 n, real eta, real ct) { if (ct < 0._r) eta = 1.0_r / eta; return n * (dot(wi, n) * eta + ct) - wi * eta; }
 
 // math_func.cpp:6-23 — Abramowitz & Stegun 7.1.26 (published coefficients)
-DI float erf_as(float x) {
-    float sign = signum(x);
-    x = fabsf(x);
-    float t = 1.0f / (1.0f + 0.3275911f * x);
-    float y = 1.0f - (((((1.061405429f * t - 1.453152027f) * t) + 1.421413741f) * t - 0.284496736f) * t + 0.254829592f) * t * expf(-x * x);
+DI real erf_as(real x) {
+    real sign = signum(x);
+    x = fabs(x);
+    real t = 1.0_r / (1.0_r + 0.3275911_r * x);
+    real y = 1.0_r - (((((1.061405429_r * t - 1.453152027_r) * t) + 1.421413741_r) * t - 0.284496736_r) * t + 0.254829592_r) * t * exp(-x * x);
     return sign * y;
 }
 // math_func.cpp:26-53 — M. Giles, "Approximating the erfinv function", GPU Computing Gems (2011)
-DI float erfinv_giles(float x) {
-    float w = -logf((1.0f - x) * (1.0f + x)), p;
-    if (w < 5.0f) {
-        w = w - 2.5f;
-        p = 2.81022636e-08f;
-        p = 3.43273939e-07f + p * w;
-        p = -3.5233877e-06f + p * w;
-        p = -4.39150654e-06f + p * w;
-        p = 0.00021858087f + p * w;
-        p = -0.00125372503f + p * w;
-        p = -0.00417768164f + p * w;
-        p = 0.246640727f + p * w;
-        p = 1.50140941f + p * w;
+DI real erfinv_giles(real x) {
+    real w = -log((1.0_r - x) * (1.0_r + x)), p;
+    if (w < 5.0_r) {
+        w = w - 2.5_r;
+        p = 2.81022636e-08_r;
+        p = 3.43273939e-07_r + p * w;
+        p = -3.5233877e-06_r + p * w;
+        p = -4.39150654e-06_r + p * w;
+        p = 0.00021858087_r + p * w;
+        p = -0.00125372503_r + p * w;
+        p = -0.00417768164_r + p * w;
+        p = 0.246640727_r + p * w;
+        p = 1.50140941_r + p * w;
     } else {
-        w = sqrtf(w) - 3.0f;
-        p = -0.000200214257f;
-        p = 0.000100950558f + p * w;
-        p = 0.00134934322f + p * w;
-        p = -0.00367342844f + p * w;
-        p = 0.00573950773f + p * w;
-        p = -0.0076224613f + p * w;
-        p = 0.00943887047f + p * w;
-        p = 1.00167406f + p * w;
-        p = 2.83297682f + p * w;
+        w = sqrt(w) - 3.0_r;
+        p = -0.000200214257_r;
+        p = 0.000100950558_r + p * w;
+        p = 0.00134934322_r + p * w;
+        p = -0.00367342844_r + p * w;
+        p = 0.00573950773_r + p * w;
+        p = -0.0076224613_r + p * w;
+        p = 0.00943887047_r + p * w;
+        p = 1.00167406_r + p * w;
+        p = 2.83297682_r + p * w;
     }
     return p * x;
 }
@@ -337,9 +359,9 @@ DI float erfinv_giles(float x) {
 // clamped away from zero), primal and with tangents.
 DI f3 rayTriangleUVT(f3 v0, f3 v1, f3 v2, f3 o, f3 d) {
     f3 e1 = v1 - v0, e2 = v2 - v0, pv = cross(d, e2);
-    float div = dot(pv, e1);
-    if (fabsf(div) < EPS) div = div > 0.f ? EPS : -EPS;
-    float idv = 1.0f / div;
+    real div = dot(pv, e1);
+    if (fabs(div) < EPS) div = div > 0._r ? EPS : -EPS;
+    real idv = 1.0_r / div;
     f3 s = o - v0, q = cross(s, e1);
     return f3{dot(s, pv) * idv, dot(d, q) * idv, dot(e2, q) * idv};
 }
@@ -348,12 +370,13 @@ DI void rayTriangleUVT_AD(const Dv<N> &v0, const Dv<N> &v1, const Dv<N> &v2, con
                           Df<N> &v, Df<N> &t) {
     Dv<N> e1 = v1 - v0, e2 = v2 - v0, pv = dcross(d, e2);
     Df<N> div = ddot(pv, e1);
-    if (fabsf(div.v) < EPS) div = Df<N>(div.v > 0.f ? EPS : -EPS);
-    Df<N> idv = 1.0f / div;
+    if (fabs(div.v) < EPS) div = Df<N>(div.v > 0._r ? EPS : -EPS);
+    Df<N> idv = 1.0_r / div;
     Dv<N> s = o - v0, q = dcross(s, e1);
     u = ddot(s, pv) * idv;
     v = ddot(d, q) * idv;
     t = ddot(e2, q) * idv;
 }
 
+}  // namespace DT_PNS
 }  // namespace dt

@@ -38,6 +38,17 @@ int nderNow() {
     return g_nder;
 }
 
+// Estimator precision of scenes built from now on: 64 (double, the reference's
+// precision, include/fwd.h:10-13) unless DTRT_PRECISION=32 or dtrt_set_precision(32).
+int g_prec = -1;
+int precisionNow() {
+    if (g_prec < 0) {
+        const char *e = std::getenv("DTRT_PRECISION");
+        g_prec = (e && std::atoi(e) == 32) ? 32 : 64;
+    }
+    return g_prec;
+}
+
 bool fail(const std::string &m) { g_err = m; return false; }
 
 #define HIPCHK(x)                                                                        \
@@ -725,6 +736,7 @@ struct Scene : Obj {
         HIPCHK(hipStreamSynchronize(stream));
         return true;
     }
+    size_t accBytes() const { return S.f64 ? 8 : 4; }   // accumulator element: the estimator's type
     bool gridMedium(int med) const {
         if (med < 0 || med >= (int)hmed.size() || hmed[med].type != MED_HETEROGENEOUS || !hmed[med].density)
             return fail("medium index does not name a heterogeneous grid medium of this scene");
@@ -743,9 +755,9 @@ bool frameRotationReady() {
     double ax = 0.1f, ay = 0.2f, az = 0.3f, l = std::sqrt(ax * ax + ay * ay + az * az);
     ax /= l; ay /= l; az /= l;
     double th = (double)0.1f, c = std::cos(th), s = std::sin(th), t = 1 - c;
-    float m[9] = {(float)(c + ax * ax * t), (float)(ax * ay * t - az * s), (float)(ax * az * t + ay * s),
-                  (float)(ay * ax * t + az * s), (float)(c + ay * ay * t), (float)(ay * az * t - ax * s),
-                  (float)(az * ax * t - ay * s), (float)(az * ay * t + ax * s), (float)(c + az * az * t)};
+    const double m[9] = {c + ax * ax * t, ax * ay * t - az * s, ax * az * t + ay * s,
+                         ay * ax * t + az * s, c + ay * ay * t, ay * az * t - ax * s,
+                         az * ax * t - ay * s, az * ay * t + ax * s, c + az * az * t};
     HIPCHK(setFrameRotation(m));
     done.push_back(dev);
     return true;
@@ -756,6 +768,7 @@ bool Scene::build() {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     std::memset(&S, 0, sizeof(S));
     S.nder = nder;
+    S.f64 = precisionNow() == 64;
     S.nshapes = (int)shapes.size();
     S.nlights = (int)emitters.size();
     S.nmedia = (int)media.size();
@@ -1005,7 +1018,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
-    P.rec_stride = (int)((sizeof(ChildHdr) + 12 * nder + 15) / 16 * 16);
+    P.rec_stride = childRecordStride(nder, S.f64 != 0);
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
@@ -1026,12 +1039,13 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
     const uint64_t max_chunk = std::min<uint64_t>(tune_max_chunk, std::max<uint64_t>(total, 64));
     P.qcap = (unsigned)cap;
-    if (!ensure(acc, img * 4) || !ensure(acc_pe, (size_t)nder * npix * 3 * 4) || !ensure(qcount, 64) ||
+    const size_t es = accBytes();
+    if (!ensure(acc, img * es) || !ensure(acc_pe, (size_t)nder * npix * 3 * es) || !ensure(qcount, 64) ||
         !ensure(stats, 64 * 4) || !ensure(work, 16 * 8) || !ensure(queue, (size_t)cap * P.rec_stride) ||
         !ensure(ovf[0], (size_t)max_chunk * 16) || !ensure(ovf[1], (size_t)max_chunk * 16))
         return false;
-    HIPCHK(hipMemsetAsync(acc.p, 0, img * 4, stream));
-    HIPCHK(hipMemsetAsync(acc_pe.p, 0, (size_t)nder * npix * 3 * 4, stream));
+    HIPCHK(hipMemsetAsync(acc.p, 0, img * es, stream));
+    HIPCHK(hipMemsetAsync(acc_pe.p, 0, (size_t)nder * npix * 3 * es, stream));
     HIPCHK(hipMemsetAsync(stats.p, 0, 64 * 4, stream));
     HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
     // per-view output images in device memory (staged through outbuf for host outputs)
@@ -1072,7 +1086,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     unsigned *qc = static_cast<unsigned *>(qcount.p);   // [0] queued records, [1] overflowed samples, [2] consumed records
     unsigned *st = static_cast<unsigned *>(stats.p);    // [0..STAT_COUNT) counters, [32..) discarded (replays)
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
-    float *A = static_cast<float *>(acc.p), *APE = static_cast<float *>(acc_pe.p);
+    void *A = acc.p, *APE = acc_pe.p;
     char *Q = static_cast<char *>(queue.p);
     uint4 *OV[2] = {static_cast<uint4 *>(ovf[0].p), static_cast<uint4 *>(ovf[1].p)};
     uint64_t children = 0, replayed = 0;
@@ -1159,7 +1173,7 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
     const dtrt_render_options &o0 = opts[0];
     P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
-    P.rec_stride = (int)((sizeof(ChildHdr) + 12 * nder + 15) / 16 * 16);
+    P.rec_stride = childRecordStride(nder, S.f64 != 0);
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
@@ -1178,15 +1192,15 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
     unsigned *qc = static_cast<unsigned *>(qcount.p);
     unsigned *st = static_cast<unsigned *>(stats.p);
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
-    float *A = static_cast<float *>(acc.p), *APE = static_cast<float *>(acc_pe.p);
+    void *A = acc.p, *APE = acc_pe.p;
     char *Q = static_cast<char *>(queue.p);
     uint4 *OV = static_cast<uint4 *>(ovf[0].p);
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     auto issue = [&]() -> hipError_t {
         hipError_t e;
 #define GCHK(x) if ((e = (x)) != hipSuccess) return e
-        GCHK(launch_fill32(A, img, 0u, stream));
-        GCHK(launch_fill32(APE, (size_t)nder * npix * 3, 0u, stream));
+        GCHK(launch_fill32(A, img * accBytes() / 4, 0u, stream));
+        GCHK(launch_fill32(APE, (size_t)nder * npix * 3 * accBytes() / 4, 0u, stream));
         GCHK(launch_fill32(st, 64, 0u, stream));
         GCHK(launch_fill32(wk, 32, 0u, stream));
         for (int v = 0; v < nv; ++v) GCHK(launch_fill32(images[v], (size_t)(nder + 1) * hv[v].npix * 3, 0u, stream));
@@ -1380,20 +1394,20 @@ bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_d
     if (!setupViews(1, &c, &ov, hv, pe_total)) return false;
     P.npix = hv[0].npix;
     const size_t n3 = (size_t)P.npix * 3;
-    if (!ensure(acc, n3 * 4) || !ensure(work, 16 * 8)) return false;
+    if (!ensure(acc, n3 * accBytes()) || !ensure(work, 16 * 8)) return false;
     float *dout = out;
     if (!out_is_device) {
         if (!ensure(outbuf, n3 * 4)) return false;
         dout = static_cast<float *>(outbuf.p);
     }
-    HIPCHK(hipMemsetAsync(acc.p, 0, n3 * 4, stream));
+    HIPCHK(hipMemsetAsync(acc.p, 0, n3 * accBytes(), stream));
     HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
     if (ev.empty()) {
         ev.resize(2);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     }
     unsigned long long *wk = static_cast<unsigned long long *>(work.p);
-    float *A = static_cast<float *>(acc.p);
+    void *A = acc.p;
     // ParticleTracer: spp blocks of 100000 particles (ptracer.cpp:11-12); others: spp per pixel
     const uint64_t total = integrator == 5 ? (uint64_t)100000 * o.spp : (uint64_t)P.npix * o.spp;
     const uint64_t chunk = uint64_t(1) << (integrator == 5 ? 22 : 26);
@@ -1405,7 +1419,7 @@ bool Scene::renderPrimal(const dtrt_render_options &o, float *out, bool out_is_d
         else HIPCHK(launch_primal_paths(S, P, kind, s0, std::min(chunk, total - s0), A, wk, stream));
     }
     HIPCHK(hipEventRecord(ev[1], stream));
-    HIPCHK(launch_scale(A, dout, n3, (float)(1.0 / (double)(integrator == 5 ? total : o.spp)), stream));
+    HIPCHK(launch_scale(S, A, dout, n3, 1.0 / (double)(integrator == 5 ? total : o.spp), stream));
     if (!out_is_device) HIPCHK(hipMemcpyAsync(out, dout, n3 * 4, hipMemcpyDeviceToHost, stream));
     unsigned long long hw[12];
     HIPCHK(hipMemcpyAsync(hw, wk, sizeof(hw), hipMemcpyDeviceToHost, stream));
@@ -1595,6 +1609,12 @@ int dtrt_set_nder(int n) {
     return 0;
 }
 int dtrt_nder(void) { return nderNow(); }
+int dtrt_set_precision(int bits) {
+    if (bits != 32 && bits != 64) { g_err = "precision must be 32 or 64 bits"; return -1; }
+    g_prec = bits;
+    return 0;
+}
+int dtrt_precision(void) { return precisionNow(); }
 double dtrt_angle_eps(void) { return (double)1e-4f; }
 const char *dtrt_last_error(void) { return g_err.c_str(); }
 void dtrt_destroy(dtrt_object *o) { delete reinterpret_cast<Obj *>(o); }
@@ -1915,6 +1935,10 @@ int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
     });
 }
 
+int dtrt_scene_precision(dtrt_object *scene) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    return s ? (s->S.f64 ? 64 : 32) : -1;
+}
 int dtrt_scene_set_stream(dtrt_object *scene, void *stream) {
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s) return -1;

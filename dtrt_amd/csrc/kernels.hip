@@ -22,7 +22,21 @@
 // the seven code objects build in parallel. Each object holds its own copy of the
 // frame rotation constant, named by DT_FRAME_ROT; setFrameRotation fills all seven.
 namespace dt {
+namespace DT_PNS {
 
+using ChildHdr = ChildHdrT<real>;
+// Edge-sample acceptance tests of volpathAD.cpp:545-584: the slerp weights must sum to
+// one within Epsilon and the direction's tangents must be orthogonal to it within
+// Epsilon. fp32 rounding needs wider bounds (1e-5, and 1e-4 relative to the tangent size).
+constexpr real SLERP_SUM_TOL = kF64 ? 1e-7_r : 1e-5_r;
+DI real orthTol(real gmax) { return kF64 ? EPS : 1e-4_r * (1.0_r + gmax); }
+// Angle between the edge's two vertices as seen from x: acos of the normalised dot
+// product as the reference (volpathAD.cpp:547); in fp32 the atan2 form, which stays
+// accurate for small angles.
+DI real edgeAngle(f3 e0, f3 e1, f3 c01, real l0, real l1) {
+    if constexpr (kF64) return acos(dot(e0, e1) / (l0 * l1));
+    else return atan2(length(c01) / (l0 * l1), dot(e0, e1) / (l0 * l1));
+}
 __constant__ FrameRot DT_FRAME_ROT;
 
 constexpr int BLOCK = TRACE_BLOCK;
@@ -62,12 +76,12 @@ DI int viewOf(const DScene &S, uint64_t sid) {
     return lo;
 }
 
-DI bool writeRecord(const RenderParams &P, char *q, unsigned *qcount, const ChildHdr &h, const float *w, int nw) {
+DI bool writeRecord(const RenderParams &P, char *q, unsigned *qcount, const ChildHdr &h, const real *w, int nw) {
     unsigned idx = atomicAdd(qcount, 1u);
     if (idx >= P.qcap) return false;
     char *rec = q + (size_t)idx * P.rec_stride;
     *reinterpret_cast<ChildHdr *>(rec) = h;
-    float *wd = reinterpret_cast<float *>(rec + sizeof(ChildHdr));
+    real *wd = reinterpret_cast<real *>(rec + sizeof(ChildHdr));
     for (int i = 0; i < nw; ++i) wd[i] = w[i];
     return true;
 }
@@ -87,7 +101,7 @@ struct Pusher {
     static constexpr uint32_t NO_FAIL = 0xFFFFFFFFu;
 };
 
-DI void push(const RenderParams &P, Pusher &u, ChildHdr &h, const float *w, int nw) {
+DI void push(const RenderParams &P, Pusher &u, ChildHdr &h, const real *w, int nw) {
     const uint32_t k = ++u.child;
     if (k <= u.skip || u.failed_at != Pusher::NO_FAIL) return;
     const uint64_t ck = u.key + k;
@@ -110,10 +124,10 @@ DI void flushWork(unsigned long long *work, int base, uint32_t nbox, uint32_t nt
             if (v[k]) atomicAdd(work + base + k, (unsigned long long)v[k]);
 }
 
-template <int N> DI void cameraRayAD(const DCamera &c, float x, float y, Dv<N> &o, Dv<N> &d) {   // camera.cpp:75-83
+template <int N> DI void cameraRayAD(const DCamera &c, real x, real y, Dv<N> &o, Dv<N> &d) {   // camera.cpp:75-83
     x /= c.width;
     y /= c.height;
-    f3 n3{(x - 0.5f) * 2.0f, (y - 0.5f) * (-2.0f) / c.aspect, 1.0f};
+    f3 n3{(x - 0.5_r) * 2.0_r, (y - 0.5_r) * (-2.0_r) / c.aspect, 1.0_r};
     f3 dc = normalize(f3{c.ndc2c[0] * n3.x + c.ndc2c[1] * n3.y + c.ndc2c[2] * n3.z,
                          c.ndc2c[3] * n3.x + c.ndc2c[4] * n3.y + c.ndc2c[5] * n3.z,
                          c.ndc2c[6] * n3.x + c.ndc2c[7] * n3.y + c.ndc2c[8] * n3.z});
@@ -128,10 +142,10 @@ template <int N> DI void cameraRayAD(const DCamera &c, float x, float y, Dv<N> &
 
 // camera.cpp:40-50 (primal, jittered pixel): origin and direction through the
 // camera-to-world matrix, as VolPathTracer's camera rays are built.
-DI void cameraRay(const DCamera &c, float x, float y, f3 &o, f3 &d) {
+DI void cameraRay(const DCamera &c, real x, real y, f3 &o, f3 &d) {
     x /= c.width;
     y /= c.height;
-    f3 n3{(x - 0.5f) * 2.0f, (y - 0.5f) * (-2.0f) / c.aspect, 1.0f};
+    f3 n3{(x - 0.5_r) * 2.0_r, (y - 0.5_r) * (-2.0_r) / c.aspect, 1.0_r};
     f3 dc = normalize(f3{c.ndc2c[0] * n3.x + c.ndc2c[1] * n3.y + c.ndc2c[2] * n3.z,
                          c.ndc2c[3] * n3.x + c.ndc2c[4] * n3.y + c.ndc2c[5] * n3.z,
                          c.ndc2c[6] * n3.x + c.ndc2c[7] * n3.y + c.ndc2c[8] * n3.z});
@@ -153,7 +167,7 @@ DI int isSilhouette(const DScene &S, int4 e, f3 p, bool hasNormals) {
         if (k == e.x || k == e.y) k = fi[2];
         f3 v = ldg3(S.V + 3 * (size_t)k);
         f3 n = s == 0 ? cross(v0 - v, v1 - v) : cross(v1 - v, v0 - v);
-        ff[s] = dot(n, p - v) > 0.f;   // sign test; normalisation does not change it
+        ff[s] = dot(n, p - v) > 0._r;   // sign test; normalisation does not change it
     }
     if (ff[0] != ff[1]) return 2;
     return hasNormals ? 0 : 1;
@@ -169,12 +183,12 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
     if (!S.edges_valid) return;
     const int ne = P.sppse;
     for (int i = 0; i < ne; ++i) {
-        float rnd = (i + r.next()) / ne, pdfs, pdfe;
+        real rnd = (i + r.next()) / ne, pdfs, pdfe;
         int sid = sampleReuse(S.shape_edge_cdf, S.nshapes, rnd, &pdfs);
         const DShape &es = S.shapes[sid];
         if (es.ne <= 0) continue;
         int k = sampleReuse(S.edge_cdf + es.ebase + sid, es.ne, rnd, &pdfe);
-        float t = rnd, edgePdf = pdfs * pdfe;
+        real t = rnd, edgePdf = pdfs * pdfe;
         int4 e = S.edges[es.ebase + k];
         int sil = isSilhouette(S, e, x.val(), es.has_normals);
         if (!(sil > 0 && S.bsdfs[es.bsdf_id].type != BSDF_NULL)) continue;
@@ -182,26 +196,24 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
         Dv<N> e1 = ldDv<N>(S.V + 3 * (size_t)e.y, S.Vd + (size_t)e.y * N * 3) - x;
         f3 c01 = cross(e0.val(), e1.val());
         f3 nrm = normalize(c01);
-        float l0 = length(e0.val()), l1 = length(e1.val());
-        // angle between the two edge vertices as seen from x (atan2 form: well conditioned in fp32)
-        float theta = atan2f(length(c01) / (l0 * l1), dot(e0.val(), e1.val()) / (l0 * l1));
+        real l0 = length(e0.val()), l1 = length(e1.val());
+        const real theta = edgeAngle(e0.val(), e1.val(), c01, l0, l1);
         if (theta < EPS || theta > PI - EPS) continue;
-        float sinT = sinf(theta);
-        float w0 = sinf(theta * (1.0f - t)) / sinT / l0, w1 = sinf(theta * t) / sinT / l1;
-        float ws = w0 + w1;
+        real sinT = sin(theta);
+        real w0 = sin(theta * (1.0_r - t)) / sinT / l0, w1 = sin(theta * t) / sinT / l1;
+        real ws = w0 + w1;
         w0 /= ws;
         w1 /= ws;
-        // fp32 rounding: the sum test uses 1e-5 instead of the reference's 1e-7
-        if (w0 < -EPS || w1 < -EPS || fabsf(w0 + w1 - 1.0f) > 1e-5f) { atomicAdd(stats + STAT_BAD_EDGE_WEIGHT, 1u); continue; }
+        if (w0 < -EPS || w1 < -EPS || fabs(w0 + w1 - 1.0_r) > SLERP_SUM_TOL) { atomicAdd(stats + STAT_BAD_EDGE_WEIGHT, 1u); continue; }
         Dv<N> w = e0 * w0 + e1 * w1;
         Df<N> dist = dnorm(w);
         w = w / dist;
         edgePdf /= theta;
         f3 wv = w.val();
-        float orth = 0.f, gmax = 0.f;
+        real orth = 0._r, gmax = 0._r;
 #pragma unroll
-        for (int j = 0; j < N; ++j) { orth = fmaxf(orth, fabsf(dot(w.grad(j), wv))); gmax = fmaxf(gmax, length(w.grad(j))); }
-        if (orth > 1e-4f * (1.0f + gmax)) { atomicAdd(stats + STAT_NONORTHO, 1u); continue; }
+        for (int j = 0; j < N; ++j) { orth = fmax(orth, fabs(dot(w.grad(j), wv))); gmax = fmax(gmax, length(w.grad(j))); }
+        if (orth > orthTol(gmax)) { atomicAdd(stats + STAT_NONORTHO, 1u); continue; }
         if (w.gradZero(EPS)) continue;
         f3 fv;
         int med1 = med;
@@ -214,13 +226,13 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
             if (isTransition(sh0)) med1 = targetMedium(sh0, its0->gn, wv);
         }
         if (allZero(fv, EPS)) continue;
-        float trans = T.evalTransmittance(x.val(), wv, !volume, med1, dist.v, r, depth - 1);
+        real trans = T.evalTransmittance(x.val(), wv, !volume, med1, dist.v, r, depth - 1);
         if (!(trans > EPS)) continue;
-        float W[N * 3];
-        f3 base = betav * fv * (1.0f / (edgePdf * ne));
+        real W[N * 3];
+        f3 base = betav * fv * (1.0_r / (edgePdf * ne));
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            float nd = dot(nrm, w.grad(j));
+            real nd = dot(nrm, w.grad(j));
             W[3 * j] = base.x * nd; W[3 * j + 1] = base.y * nd; W[3 * j + 2] = base.z * nd;
         }
         ChildHdr h;
@@ -251,7 +263,7 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
     const DCamera &cam = vw.cam;
     const int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
     const int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
-    float jx = 0.5f, jy = 0.5f;
+    real jx = 0.5_r, jy = 0.5_r;
     if (vw.sppe >= 0) { jx = r.next(); jy = r.next(); }
     Dv<N> ro, rd;
     cameraRayAD<N>(cam, ix + jx, iy + jy, ro, rd);
@@ -272,23 +284,23 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
         const Hit its0 = its.val();
         FrameAD<N> shf(its.sn);
         if (mode & 2) {   // light sampling
-            float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+            real r4[4] = {r.next(), r.next(), r.next(), r.next()};
             Dv<N> woL;
-            float pdf_nee;
+            real pdf_nee;
             Ds<N> value = T.neeDirectAD(its, shf, r4, woL, pdf_nee);
             if (!value.isZero(EPS)) {
                 Ds<N> bv = bsdfEvalAD<N>(b, its.wi, woL);
-                const float bp = bsdfPdf(b, its0.wi, woL.val());
-                const float mis = DIRECT ? pdf_nee / (pdf_nee + bp) : misPower(pdf_nee, bp);
+                const real bp = bsdfPdf(b, its0.wi, woL.val());
+                const real mis = DIRECT ? pdf_nee / (pdf_nee + bp) : misPower(pdf_nee, bp);
                 L += beta * (value * bv * mis);
             }
         }
         // BSDF sampling (bsdf.cpp:11-21; null.cpp:20-25)
-        float r3[3] = {r.next(), r.next(), r.next()};
+        real r3[3] = {r.next(), r.next(), r.next()};
         bool cont = false;
         Dv<N> nd;
         Ds<N> bw;
-        float bpdf = 1.f;
+        real bpdf = 1._r;
         if (isnull) {
             nd = rd;
             bw = Ds<N>(f3{1, 1, 1});
@@ -298,7 +310,7 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
             f3 w0 = bsdfSample(b, its0.wi, r3[0], r3[1], r3[2], wo0, bpdf);
             if (!allZero(w0, EPS) && bpdf > EPS) {
                 const f3 ww = shf.val().toWorld(wo0);
-                bw = bsdfEvalAD<N>(b, its.wi, shf.toLocal(Dv<N>(ww))) * (1.0f / bpdf);
+                bw = bsdfEvalAD<N>(b, its.wi, shf.toLocal(Dv<N>(ww))) * (1.0_r / bpdf);
                 if (!bw.isZero(EPS)) { nd = Dv<N>(ww); cont = true; }
             }
         }
@@ -309,11 +321,11 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
                 const DShape &sh1 = shapeOf(S, its1.face);
                 if (sh1.light_id >= 0 && (mode & 2)) {
                     Ds<N> lc = emitterEvalAD<N>(S.emitters[sh1.light_id], its1, -nd);
-                    if (!allZero(lc.val(), 0.f)) {
+                    if (!allZero(lc.val(), 0._r)) {
                         const f3 dd = its1.p.val() - its0.p;
-                        const float G = its1.wi.z.v / dot(dd, dd);
-                        const float pn = S.light_pmf[sh1.light_id] / (S.light_areas[sh1.light_id] * G);
-                        const float mis = DIRECT ? bpdf / (pn + bpdf) : misPower(bpdf, pn);
+                        const real G = its1.wi.z.v / dot(dd, dd);
+                        const real pn = S.light_pmf[sh1.light_id] / (S.light_areas[sh1.light_id] * G);
+                        const real mis = DIRECT ? bpdf / (pn + bpdf) : misPower(bpdf, pn);
                         L += beta * (bw * lc * mis);
                     }
                 }
@@ -323,63 +335,63 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
         if (P.sppse > 0 && (mode & 4) && S.edges_valid) {
             const int ne = P.sppse;
             const Dv<N> &x1 = its.p;
-            float der[N * 3];
+            real der[N * 3];
 #pragma unroll
-            for (int k = 0; k < N * 3; ++k) der[k] = 0.f;
+            for (int k = 0; k < N * 3; ++k) der[k] = 0._r;
             for (int i = 0; i < ne; ++i) {
-                float rnd = r.next(), pdfs, pdfe;
+                real rnd = r.next(), pdfs, pdfe;
                 const int sid = sampleReuse(S.shape_edge_cdf, S.nshapes, rnd, &pdfs);
                 const DShape &es = S.shapes[sid];
                 if (es.ne <= 0) continue;
                 const int k = sampleReuse(S.edge_cdf + es.ebase + sid, es.ne, rnd, &pdfe);
-                const float t = rnd;
-                float edgePdf = pdfs * pdfe;
+                const real t = rnd;
+                real edgePdf = pdfs * pdfe;
                 const int4 e = S.edges[es.ebase + k];
                 if (isSilhouette(S, e, x1.val(), es.has_normals) <= 0) continue;
                 const Dv<N> v0 = ldDv<N>(S.V + 3 * (size_t)e.x, S.Vd + (size_t)e.x * N * 3);
                 const Dv<N> v1 = ldDv<N>(S.V + 3 * (size_t)e.y, S.Vd + (size_t)e.y * N * 3);
                 Dv<N> w;
                 f3 nrm;
-                float dist, geo = 1.f;
+                real dist, geo = 1._r;
                 if constexpr (DIRECT) {   // linear in t, pdf per unit length (scene.h:103-108)
                     edgePdf /= length(v1.val() - v0.val());
-                    Dv<N> y1 = v0 * (1.0f - t) + v1 * t;
+                    Dv<N> y1 = v0 * (1.0_r - t) + v1 * t;
                     w = y1 - x1;
                     nrm = normalize(cross(v0.val() - x1.val(), v1.val() - x1.val()));
                     Df<N> dAD = dnorm(w);
                     w = w / dAD;
                     dist = dAD.v;
-                    const float ct = -dot(w.val(), normalize(v1.val() - v0.val()));
-                    geo = sqrtf(fmaxf(0.f, 1.0f - ct * ct)) / dist;
+                    const real ct = -dot(w.val(), normalize(v1.val() - v0.val()));
+                    geo = sqrt(fmax(0._r, 1.0_r - ct * ct)) / dist;
                 } else {   // angle-uniform point on the edge (pathAD.cpp:136-164)
                     const Dv<N> e0 = v0 - x1, e1 = v1 - x1;
                     const f3 c01 = cross(e0.val(), e1.val());
                     nrm = normalize(c01);
-                    const float l0 = length(e0.val()), l1 = length(e1.val());
-                    const float theta = atan2f(length(c01) / (l0 * l1), dot(e0.val(), e1.val()) / (l0 * l1));
+                    const real l0 = length(e0.val()), l1 = length(e1.val());
+                    const real theta = edgeAngle(e0.val(), e1.val(), c01, l0, l1);
                     if (theta < EPS || theta > PI - EPS) continue;
-                    const float sinT = sinf(theta);
-                    float w0 = sinf(theta * (1.0f - t)) / sinT / l0, w1 = sinf(theta * t) / sinT / l1;
-                    const float ws = w0 + w1;
+                    const real sinT = sin(theta);
+                    real w0 = sin(theta * (1.0_r - t)) / sinT / l0, w1 = sin(theta * t) / sinT / l1;
+                    const real ws = w0 + w1;
                     w0 /= ws;
                     w1 /= ws;
-                    if (w0 < -EPS || w1 < -EPS || fabsf(w0 + w1 - 1.0f) > 1e-5f) { atomicAdd(stats + STAT_BAD_EDGE_WEIGHT, 1u); continue; }
+                    if (w0 < -EPS || w1 < -EPS || fabs(w0 + w1 - 1.0_r) > SLERP_SUM_TOL) { atomicAdd(stats + STAT_BAD_EDGE_WEIGHT, 1u); continue; }
                     w = e0 * w0 + e1 * w1;
                     Df<N> dAD = dnorm(w);
                     w = w / dAD;
                     dist = dAD.v;
                     edgePdf /= theta;
-                    float orth = 0.f, gmax = 0.f;
+                    real orth = 0._r, gmax = 0._r;
 #pragma unroll
-                    for (int j = 0; j < N; ++j) { orth = fmaxf(orth, fabsf(dot(w.grad(j), w.val()))); gmax = fmaxf(gmax, length(w.grad(j))); }
-                    if (orth > 1e-4f * (1.0f + gmax)) { atomicAdd(stats + STAT_NONORTHO, 1u); continue; }
+                    for (int j = 0; j < N; ++j) { orth = fmax(orth, fabs(dot(w.grad(j), w.val()))); gmax = fmax(gmax, length(w.grad(j))); }
+                    if (orth > orthTol(gmax)) { atomicAdd(stats + STAT_NONORTHO, 1u); continue; }
                     if (w.gradZero(EPS)) continue;
                 }
                 const f3 wv = w.val();
                 const f3 fv = bsdfEval(b, its0.wi, shf.val().toLocal(wv));
                 if (allZero(fv, EPS)) continue;
-                float th;   // scene.cpp:150-176: nothing between x1 and the edge point
-                if (T.trace(x1.val(), wv, SHADOW_EPS, (1.0f - SHADOW_EPS) * dist, th) >= 0) continue;
+                real th;   // scene.cpp:150-176: nothing between x1 and the edge point
+                if (T.trace(x1.val(), wv, SHADOW_EPS, (1.0_r - SHADOW_EPS) * dist, th) >= 0) continue;
                 const f3 base = beta.val() * fv * (geo / (edgePdf * ne));
                 if constexpr (DIRECT) {
                     f3 delta{0, 0, 0};
@@ -397,16 +409,16 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
                     if (allZero(delta, EPS)) continue;
 #pragma unroll
                     for (int j = 0; j < N; ++j) {
-                        const float nd_ = dot(nrm, w.grad(j));
+                        const real nd_ = dot(nrm, w.grad(j));
                         der[3 * j] += nd_ * base.x * delta.x;
                         der[3 * j + 1] += nd_ * base.y * delta.y;
                         der[3 * j + 2] += nd_ * base.z * delta.z;
                     }
                 } else {
-                    float W[N * 3];
+                    real W[N * 3];
 #pragma unroll
                     for (int j = 0; j < N; ++j) {
-                        const float nd_ = dot(nrm, w.grad(j));
+                        const real nd_ = dot(nrm, w.grad(j));
                         W[3 * j] = base.x * nd_; W[3 * j + 1] = base.y * nd_; W[3 * j + 2] = base.z * nd_;
                     }
                     ChildHdr h;
@@ -442,7 +454,7 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
 // Passes of the path-replay adjoint (k_adjoint); NONE in the forward kernel.
 enum class AdjPass { NONE, TOTAL, SCATTER };
 struct AdjCtx {
-    float *grad;         // d loss / d node density of the grid of *pm (accumulated)
+    float *grad;        // d loss / d node density of the grid of *pm (accumulated)
     const DMedium *pm;
     f3 adj;              // d loss / d pixel value of this sample (already / spp)
     f3 Ltot;             // the sample's radiance, from the TOTAL pass
@@ -457,7 +469,7 @@ template <int N> DI void adCameraRay(const DView &vw, Rng &r, int pixel, Dv<N> &
     const DCamera &cam = vw.cam;
     int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
     int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
-    float jx = 0.5f, jy = 0.5f;
+    real jx = 0.5_r, jy = 0.5_r;
     if (vw.sppe >= 0) { jx = r.next(); jy = r.next(); }
     cameraRayAD<N>(cam, ix + jx, iy + jy, ro, rd);
 }
@@ -486,7 +498,7 @@ DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &
 // Samples that overflow (again) are appended to ovf.
 template <int N, int I>   // I: AdKind
 __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
-                                                    const uint4 *replay, float *acc, char *q, unsigned *qcount,
+                                                    const uint4 *replay, real *acc, char *q, unsigned *qcount,
                                                     unsigned *stats, uint4 *ovf, unsigned *ovf_count,
                                                     unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
@@ -519,24 +531,24 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
     const bool put = active && done;
     const int gp0 = __shfl(gpix, 0);
     if (!replay && __all(!active || gpix == gp0)) {   // the wave covers one pixel
-        float sum[3 * (N + 1)];
+        real sum[3 * (N + 1)];
         for (int c = 0; c < 3; ++c) {
-            sum[c] = put ? L.c[c].v : 0.f;
+            sum[c] = put ? L.c[c].v : 0._r;
 #pragma unroll
-            for (int j = 0; j < N; ++j) sum[3 + 3 * j + c] = put ? L.c[c].d[j] : 0.f;
+            for (int j = 0; j < N; ++j) sum[3 + 3 * j + c] = put ? L.c[c].d[j] : 0._r;
         }
 #pragma unroll
         for (int k = 0; k < 3 * (N + 1); ++k)
             for (int off = 32; off > 0; off >>= 1) sum[k] += __shfl_xor(sum[k], off);
         if (threadIdx.x == 0 && active)
             for (int k = 0; k < 3 * (N + 1); ++k)
-                if (sum[k] != 0.f) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + gpix) * 3 + (k % 3), sum[k]);
+                if (sum[k] != 0._r) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + gpix) * 3 + (k % 3), sum[k]);
     } else if (put) {
         for (int c = 0; c < 3; ++c) {
-            if (L.c[c].v != 0.f) unsafeAtomicAdd(acc + (size_t)gpix * 3 + c, L.c[c].v);
+            if (L.c[c].v != 0._r) unsafeAtomicAdd(acc + (size_t)gpix * 3 + c, L.c[c].v);
 #pragma unroll
             for (int j = 0; j < N; ++j)
-                if (L.c[c].d[j] != 0.f) unsafeAtomicAdd(acc + ((size_t)(j + 1) * nd + gpix) * 3 + c, L.c[c].d[j]);
+                if (L.c[c].d[j] != 0._r) unsafeAtomicAdd(acc + ((size_t)(j + 1) * nd + gpix) * 3 + c, L.c[c].d[j]);
         }
     }
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
@@ -591,7 +603,7 @@ DI bool adBounce(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, AdState
 // replay pass.
 template <int N>
 __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
-                                                     uint64_t per_wave, const uint4 *replay, float *acc, char *q,
+                                                     uint64_t per_wave, const uint4 *replay, real *acc, char *q,
                                                      unsigned *qcount, unsigned *stats, uint4 *ovf, unsigned *ovf_count,
                                                      unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
@@ -620,9 +632,9 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, Ren
 #pragma unroll
             for (int k = 0; k < 3 * (N + 1); ++k) {
                 const int c = k % 3, j = k / 3;
-                float x = mine ? (j == 0 ? L.c[c].v : L.c[c].d[j - 1]) : 0.f;
+                real x = mine ? (j == 0 ? L.c[c].v : L.c[c].d[j - 1]) : 0._r;
                 for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-                if ((int)(threadIdx.x & 63) == lead && x != 0.f) unsafeAtomicAdd(acc + ((size_t)j * nd + gp0) * 3 + c, x);
+                if ((int)(threadIdx.x & 63) == lead && x != 0._r) unsafeAtomicAdd(acc + ((size_t)j * nd + gp0) * 3 + c, x);
             }
         }
         if (fin) {
@@ -672,7 +684,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, Ren
 // Radiance sums go to acc[pixel][3].
 template <int KIND>
 __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DScene S, RenderParams P, uint64_t s0,
-                                                                              uint64_t nsamp, float *acc,
+                                                                              uint64_t nsamp, real *acc,
                                                                               unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
@@ -688,7 +700,7 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DS
     if (active) {
         const int ix = cam.cropped ? cam.ox + pixel % cam.cw : pixel % cam.width;
         const int iy = cam.cropped ? cam.oy + pixel / cam.cw : pixel / cam.width;
-        const float jx = r.next(), jy = r.next();
+        const real jx = r.next(), jy = r.next();
         f3 ro, rd;
         cameraRay(cam, ix + jx, iy + jy, ro, rd);
         Hit h;
@@ -706,16 +718,16 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DS
     }
     const int gp0 = __shfl(pixel, 0);
     if (__all(!active || pixel == gp0)) {   // the wave covers one pixel
-        float v[3] = {L.x, L.y, L.z};
+        real v[3] = {L.x, L.y, L.z};
         for (int c = 0; c < 3; ++c)
             for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off);
         if ((threadIdx.x & 63) == 0 && active)
             for (int c = 0; c < 3; ++c)
-                if (v[c] != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + c, v[c]);
+                if (v[c] != 0._r) unsafeAtomicAdd(acc + (size_t)pixel * 3 + c, v[c]);
     } else if (active) {
-        if (L.x != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3, L.x);
-        if (L.y != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + 1, L.y);
-        if (L.z != 0.f) unsafeAtomicAdd(acc + (size_t)pixel * 3 + 2, L.z);
+        if (L.x != 0._r) unsafeAtomicAdd(acc + (size_t)pixel * 3, L.x);
+        if (L.y != 0._r) unsafeAtomicAdd(acc + (size_t)pixel * 3 + 1, L.y);
+        if (L.z != 0._r) unsafeAtomicAdd(acc + (size_t)pixel * 3 + 2, L.z);
     }
     flushWork(work, 0, T.nbox, T.ntri, r.nlk, T.nseg);
 }
@@ -724,94 +736,94 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DS
 // camera.cpp:129-165 — importance of world point p for the pinhole camera: pixel
 // position (px, py) within the output window, direction to the camera; 0 when p is
 // behind the near plane or outside the window.
-DI float cameraSampleDirect(const DCamera &c, f3 p, float &px, float &py, f3 &dir) {
+DI real cameraSampleDirect(const DCamera &c, f3 p, real &px, real &py, f3 &dir) {
     const f3 q{c.w2c[0] * p.x + c.w2c[1] * p.y + c.w2c[2] * p.z + c.w2c[3],
                c.w2c[4] * p.x + c.w2c[5] * p.y + c.w2c[6] * p.z + c.w2c[7],
                c.w2c[8] * p.x + c.w2c[9] * p.y + c.w2c[10] * p.z + c.w2c[11]};
-    if (q.z < c.clip_near) return 0.f;
-    const float f = c.c2ndc[0];
-    float inv_area = 0.25f * f * f * c.aspect;
+    if (q.z < c.clip_near) return 0._r;
+    const real f = c.c2ndc[0];
+    real inv_area = 0.25_r * f * f * c.aspect;
     int xmin = 0, xmax = c.width, ymin = 0, ymax = c.height;
     if (c.cropped) {
-        inv_area *= (float)c.width / (float)c.cw * (float)c.height / (float)c.ch;
+        inv_area *= (real)c.width / (real)c.cw * (real)c.height / (real)c.ch;
         xmin = c.ox; xmax = c.ox + c.cw;
         ymin = c.oy; ymax = c.oy + c.ch;
     }
     const f3 n{c.c2ndc[0] * q.x + c.c2ndc[1] * q.y + c.c2ndc[2] * q.z, c.c2ndc[3] * q.x + c.c2ndc[4] * q.y + c.c2ndc[5] * q.z,
                c.c2ndc[6] * q.x + c.c2ndc[7] * q.y + c.c2ndc[8] * q.z};
-    const float sx = (n.x / n.z * 0.5f + 0.5f) * c.width, sy = (-(n.y / n.z) * 0.5f * c.aspect + 0.5f) * c.height;
-    if (!(sx >= xmin && sx <= xmax && sy >= ymin && sy <= ymax)) return 0.f;
+    const real sx = (n.x / n.z * 0.5_r + 0.5_r) * c.width, sy = (-(n.y / n.z) * 0.5_r * c.aspect + 0.5_r) * c.height;
+    if (!(sx >= xmin && sx <= xmax && sy >= ymin && sy <= ymax)) return 0._r;
     px = sx - xmin;
     py = sy - ymin;
-    const float dist = length(q), inv = 1.0f / dist, invCos = dist / q.z;
+    const real dist = length(q), inv = 1.0_r / dist, invCos = dist / q.z;
     dir = (ld3(c.cpos) - p) * inv;
-    return inv * inv * invCos * invCos * invCos * (inv_area * (float)((xmax - xmin) * (ymax - ymin)));
+    return inv * inv * invCos * invCos * invCos * (inv_area * (real)((xmax - xmin) * (ymax - ymin)));
 }
 
 // Adds v to pixel (px, py) of a w x h window. A point exactly on the window's closing
 // edge (px == w or py == h, which camera.cpp:151-152 accepts) has no pixel and is dropped.
-DI void splat(float *acc, int w, int h, float px, float py, f3 v) {
+DI void splat(real *acc, int w, int h, real px, real py, f3 v) {
     const int ix = (int)px, iy = (int)py;
     if (ix < 0 || iy < 0 || ix >= w || iy >= h) return;
-    float *a = acc + ((size_t)iy * w + ix) * 3;
-    if (v.x != 0.f) unsafeAtomicAdd(a, v.x);
-    if (v.y != 0.f) unsafeAtomicAdd(a + 1, v.y);
-    if (v.z != 0.f) unsafeAtomicAdd(a + 2, v.z);
+    real *a = acc + ((size_t)iy * w + ix) * 3;
+    if (v.x != 0._r) unsafeAtomicAdd(a, v.x);
+    if (v.y != 0._r) unsafeAtomicAdd(a + 1, v.y);
+    if (v.z != 0._r) unsafeAtomicAdd(a + 2, v.z);
 }
 
 // ptracer.cpp:57-167 — one particle: a point on a light (scene.cpp:795-814), its
 // connection to the camera, then a random walk that connects every medium and
 // non-null surface vertex to the camera. The emitter's geometric frame is the sampled
 // face's normal (the reference leaves that frame unset there).
-DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r, float *acc) {
+DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r, real *acc) {
     const DScene &S = T.S;
     if (S.nlights == 0) return;
     const int W = cam.cropped ? cam.cw : cam.width, H = cam.cropped ? cam.ch : cam.height;
     const f3 cpos = ld3(cam.cpos);
-    float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+    real r4[4] = {r.next(), r.next(), r.next(), r.next()};
     f3 lp, ln;
     const int l = T.sampleLight(r4, lp, ln);
     const DEmitter &E = S.emitters[l];
     const DShape &lsh = S.shapes[E.shape_id];
     f3 power = ld3(E.I) * (S.light_areas[l] / S.light_pmf[l]);
     {   // ptracer.cpp:118-126
-        float px, py;
+        real px, py;
         f3 dir;
-        float v = cameraSampleDirect(cam, lp, px, py, dir);
-        if (v != 0.f) {
-            const int m = dot(dir, ln) > 0.f ? lsh.med_ext : lsh.med_int;
+        real v = cameraSampleDirect(cam, lp, px, py, dir);
+        if (v != 0._r) {
+            const int m = dot(dir, ln) > 0._r ? lsh.med_ext : lsh.med_int;
             v *= T.evalTransmittance(lp, dir, true, m, length(lp - cpos), r, max_bounces);
-            if (v != 0.f) splat(acc, W, H, px, py, power * (v * fmaxf(0.f, dot(ln, dir))));
+            if (v != 0._r) splat(acc, W, H, px, py, power * (v * fmax(0._r, dot(ln, dir))));
         }
     }
-    const float u0 = r.next(), u1 = r.next();
+    const real u0 = r.next(), u1 = r.next();
     f3 d = Frame(ln).toWorld(squareToCosineHemisphere(u0, u1));
     power = power * PI;
     f3 o = lp;
-    int med = dot(d, ln) > 0.f ? lsh.med_ext : lsh.med_int;
+    int med = dot(d, ln) > 0._r ? lsh.med_ext : lsh.med_int;
     f3 thr{1, 1, 1};
     bool onSurf = true;
     Hit its;
-    for (int depth = 0; !allZero(thr, 0.f) && depth < max_bounces; ++depth) {
+    for (int depth = 0; !allZero(thr, 0._r) && depth < max_bounces; ++depth) {
         T.rayIntersect(o, d, onSurf, its);
         const int max_inter = max_bounces - depth - 1;
         bool inside = false;
         if (med >= 0) {
-            const float a = r.next(), b = r.next();
+            const real a = r.next(), b = r.next();
             f3 ps;
             inside = sampleDistance(S.media[med], o, d, its.t, a, b, r, ps, thr);
             if (inside) o = ps;
         }
         if (inside) {
             const DPhase &ph = S.phases[S.media[med].phase_id];
-            float px, py;
+            real px, py;
             f3 wo;
-            float v = cameraSampleDirect(cam, o, px, py, wo);   // ptracer.cpp:152-165
-            if (v != 0.f) {
+            real v = cameraSampleDirect(cam, o, px, py, wo);   // ptracer.cpp:152-165
+            if (v != 0._r) {
                 v *= T.evalTransmittance(o, wo, false, med, length(o - cpos), r, max_inter);
-                if (v != 0.f) splat(acc, W, H, px, py, thr * power * (v * phaseEval(ph, -d, wo)));
+                if (v != 0._r) splat(acc, W, H, px, py, thr * power * (v * phaseEval(ph, -d, wo)));
             }
-            const float a = r.next(), b = r.next();
+            const real a = r.next(), b = r.next();
             d = phaseSample(ph, -d, a, b);   // weight eval/pdf = 1
             onSurf = false;
         } else {
@@ -820,31 +832,31 @@ DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r,
             const DBsdf &b = bsdfOf(S, sh);
             const Frame shf(its.sn);
             if (b.type != BSDF_NULL) {   // ptracer.cpp:128-150
-                float px, py;
+                real px, py;
                 f3 wo;
-                float v = cameraSampleDirect(cam, its.p, px, py, wo);
-                if (v != 0.f) {
+                real v = cameraSampleDirect(cam, its.p, px, py, wo);
+                if (v != 0._r) {
                     v *= T.evalTransmittance(its.p, wo, true, targetMedium(sh, its.gn, wo), length(its.p - cpos), r, max_inter);
-                    if (v != 0.f) {
+                    if (v != 0._r) {
                         const f3 wi = shf.toWorld(its.wi), wol = shf.toLocal(wo);
-                        const float wiG = dot(wi, its.gn), woG = dot(wo, its.gn);
-                        if (wiG * its.wi.z > 0.f && woG * wol.z > 0.f) {
-                            const float corr = fabsf((its.wi.z * woG) / (wol.z * wiG));
+                        const real wiG = dot(wi, its.gn), woG = dot(wo, its.gn);
+                        if (wiG * its.wi.z > 0._r && woG * wol.z > 0._r) {
+                            const real corr = fabs((its.wi.z * woG) / (wol.z * wiG));
                             splat(acc, W, H, px, py, bsdfEval(b, its.wi, wol, true) * thr * power * (v * corr));
                         }
                     }
                 }
             }
-            const float q0 = r.next(), q1 = r.next(), q2 = r.next();
+            const real q0 = r.next(), q1 = r.next(), q2 = r.next();
             f3 wol;
-            float bpdf = 0.f;
+            real bpdf = 0._r;
             const f3 bw = bsdfSample(b, its.wi, q0, q1, q2, wol, bpdf, true);
-            if (allZero(bw, 0.f)) break;
+            if (allZero(bw, 0._r)) break;
             const f3 wo = shf.toWorld(wol);
-            const float wiG = dot(-d, its.gn), woG = dot(wo, its.gn);
-            if (wiG * its.wi.z <= 0.f || woG * wol.z <= 0.f) break;   // shading-normal light leaks [Veach p. 158]
+            const real wiG = dot(-d, its.gn), woG = dot(wo, its.gn);
+            if (wiG * its.wi.z <= 0._r || woG * wol.z <= 0._r) break;   // shading-normal light leaks [Veach p. 158]
             thr = thr * bw;
-            if (isTransition(sh)) med = woG > 0.f ? sh.med_ext : sh.med_int;
+            if (isTransition(sh)) med = woG > 0._r ? sh.med_ext : sh.med_int;
             o = its.p;
             d = wo;
             onSurf = true;
@@ -856,7 +868,7 @@ DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r,
 // ParticleTracer::render (ptracer.cpp:10-55): particles s0 .. s0+n-1, each its own
 // stream keyed (seed, particle index), splatted into acc[pixel][3] of view 0's window.
 __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_particles(DScene S, RenderParams P, uint64_t s0,
-                                                                           uint64_t n, float *acc,
+                                                                           uint64_t n, real *acc,
                                                                            unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
@@ -868,9 +880,9 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_particles(DScen
 }
 
 // out[i] = acc[i] * scale over n floats (the pixel mean of VolPathTracer)
-__global__ void k_scale(const float *acc, float *out, uint64_t n, float scale) {
+__global__ void k_scale(const real *acc, float *out, uint64_t n, real scale) {
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) out[i] = acc[i] * scale;
+    if (i < n) out[i] = (float)(acc[i] * scale);
 }
 #endif  // DT_N
 
@@ -895,16 +907,16 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     Pusher u{nullptr, nullptr, (int)gp, v, 0, 0, 0, Pusher::NO_FAIL};
     uint32_t nlk = 0;
     if (li < nsamp) {
-        const float inv = 1.0f / vw.spp;
+        const real inv = 1.0_r / vw.spp;
         const f3 adj{dimg[3 * gp] * inv, dimg[3 * gp + 1] * inv, dimg[3 * gp + 2] * inv};
-        if (adj.x != 0.f || adj.y != 0.f || adj.z != 0.f) {
+        if (adj.x != 0._r || adj.y != 0._r || adj.z != 0._r) {
             Rng r;
             r.init(vw.seed, ls);
             Ds<N> L;
             adPathSample<N, AdjPass::TOTAL>(T, P, vw, r, pixel, L, u, stats, nullptr);
             nlk += r.nlk;
             const f3 Lt = L.val();
-            if (Lt.x != 0.f || Lt.y != 0.f || Lt.z != 0.f) {
+            if (Lt.x != 0._r || Lt.y != 0._r || Lt.z != 0._r) {
                 AdjCtx ac{grad, S.media + pmed, adj, Lt};
                 Rng r2;
                 r2.init(vw.seed, ls);
@@ -945,19 +957,19 @@ __global__ void k_load_density(float *dst, const float *src, uint64_t n, unsigne
 // record's pixel. Lanes that share the first finishing lane's pixel and image are
 // summed over the wave first (one atomic per weight); the others add their own.
 template <int N>
-DI void scatterFinished(const RenderParams &P, const char *q, float *acc, float *acc_pe, bool fin, unsigned rec, f3 Lv) {
+DI void scatterFinished(const RenderParams &P, const char *q, real *acc, real *acc_pe, bool fin, unsigned rec, f3 Lv) {
     const uint64_t m = __ballot(fin);
     if (!m) return;
     const int nd = P.npix;
     int pix = -1;
     bool prim = false;
-    float v[N * 3];
+    real v[N * 3];
 #pragma unroll
-    for (int k = 0; k < N * 3; ++k) v[k] = 0.f;
+    for (int k = 0; k < N * 3; ++k) v[k] = 0._r;
     if (fin) {
         const char *r = q + (size_t)rec * P.rec_stride;
         const ChildHdr *h = reinterpret_cast<const ChildHdr *>(r);
-        const float *W = reinterpret_cast<const float *>(r + sizeof(ChildHdr));
+        const real *W = reinterpret_cast<const real *>(r + sizeof(ChildHdr));
         pix = h->pixel;
         prim = (h->meta & 8) != 0;
 #pragma unroll
@@ -972,22 +984,22 @@ DI void scatterFinished(const RenderParams &P, const char *q, float *acc, float 
     const bool prim0 = __shfl((int)prim, lead) != 0;
     const bool grp = fin && pix == pix0 && prim == prim0;
     if (fin && !grp) {
-        float *dst = prim ? acc_pe : acc + (size_t)nd * 3;
+        real *dst = prim ? acc_pe : acc + (size_t)nd * 3;
 #pragma unroll
         for (int k = 0; k < N * 3; ++k)
-            if (v[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix) * 3 + (k % 3), v[k]);
+            if (v[k] != 0._r) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix) * 3 + (k % 3), v[k]);
     }
 #pragma unroll
     for (int k = 0; k < N * 3; ++k) {
-        float sv = grp ? v[k] : 0.f;
+        real sv = grp ? v[k] : 0._r;
         for (int off = 32; off > 0; off >>= 1) sv += __shfl_xor(sv, off);
         v[k] = sv;
     }
     if ((int)(threadIdx.x & 63) == lead) {
-        float *dst = prim0 ? acc_pe : acc + (size_t)nd * 3;
+        real *dst = prim0 ? acc_pe : acc + (size_t)nd * 3;
 #pragma unroll
         for (int k = 0; k < N * 3; ++k)
-            if (v[k] != 0.f) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix0) * 3 + (k % 3), v[k]);
+            if (v[k] != 0._r) unsafeAtomicAdd(dst + ((size_t)(k / 3) * nd + pix0) * 3 + (k % 3), v[k]);
     }
 }
 
@@ -996,7 +1008,7 @@ DI void scatterFinished(const RenderParams &P, const char *q, float *acc, float 
 // per wave on qnext), so the lanes of a wave stay busy until the queue is drained.
 template <int N>
 __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
-                                                    unsigned *qnext, float *acc, float *acc_pe, unsigned long long *work) {
+                                                    unsigned *qnext, real *acc, real *acc_pe, unsigned long long *work) {
     Tracer<N> T(S);
     const unsigned n = min(*qcount, P.qcap);
     const int lane = threadIdx.x & 63;
@@ -1036,7 +1048,7 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene
                         its0.gn = f3{h.gn[0], h.gn[1], h.gn[2]};
                         its0.sn = f3{h.sn[0], h.sn[1], h.sn[2]};
                         its0.wi = Frame(its0.sn).toLocal(-dir);
-                        its0.t = 0.f;
+                        its0.t = 0._r;
                         pre = T.linsStart(r, its0, dir, med, depth, kind == CHILD_LINS_NULL, path, cw, has);
                     }
                     alive = has;
@@ -1061,7 +1073,7 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene
 template <int N>
 __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children_path(DScene S, RenderParams P, const char *q,
                                                                             const unsigned *qcount, unsigned *qnext,
-                                                                            float *acc, float *acc_pe,
+                                                                            real *acc, real *acc_pe,
                                                                             unsigned long long *work) {
     Tracer<N> T(S);
     const unsigned n = min(*qcount, P.qcap);
@@ -1112,19 +1124,19 @@ template <int N>
 DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, const DView &vw, int view, Rng &r, uint64_t sid,
                           uint64_t total, char *q, unsigned *qcount, unsigned *stats) {
     const DCamera &c = vw.cam;
-    float u = r.next();
+    real u = r.next();
     int idx = sampleReuse(vw.pe_cdf, vw.npe, u, nullptr);   // edge_manager.cpp:112-134
     const int stride = 16 + 4 * N;
     const float *pe = vw.pe + (size_t)idx * stride;
     double px_ = pe[0] + (double)u * (pe[2] - pe[0]), py_ = pe[1] + (double)u * (pe[3] - pe[1]);
-    float pd[N][2];
+    real pd[N][2];
     bool nz = false;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const float *g = pe + 16 + 4 * j;
         pd[j][0] = g[0] + u * (g[2] - g[0]);
         pd[j][1] = g[1] + u * (g[3] - g[1]);
-        nz = nz || fabsf(pd[j][0]) > EPS || fabsf(pd[j][1]) > EPS;
+        nz = nz || fabs(pd[j][0]) > EPS || fabs(pd[j][1]) > EPS;
     }
     if (!nz) return;
     int ipx = (int)floor(px_), ipy = (int)floor(py_);
@@ -1142,7 +1154,7 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, const DView &vw, 
     double cc = sqrt(v1p[0] * v1p[0] + v1p[1] * v1p[1] + v1p[2] * v1p[2]) / sqrt(v1c[0] * v1c[0] + v1c[1] * v1c[1] + v1c[2] * v1c[2]);
     double bx = (1 - t0) * cb, by = t0 * cc, t1 = by / (bx + by);   // utils.cpp:256-266
     double mx = v0c[0] + t1 * (v1c[0] - v0c[0]), my = v0c[1] + t1 * (v1c[1] - v0c[1]), mz = v0c[2] + t1 * (v1c[2] - v0c[2]);
-    float maxD = (float)sqrt(mx * mx + my * my + mz * mz);
+    real maxD = (real)sqrt(mx * mx + my * my + mz * mz);
     // camera rays in double: the two sides of the edge are PRIMARY_DELTA pixels apart
     auto camdir = [&](double x, double y) {
         x /= c.width;
@@ -1152,21 +1164,21 @@ DI void primaryEdgeSample(Tracer<N> &T, const RenderParams &P, const DView &vw, 
                dz = c.ndc2c[6] * n0 + c.ndc2c[7] * n1 + c.ndc2c[8];
         double l = sqrt(dx * dx + dy * dy + dz * dz);
         dx /= l; dy /= l; dz /= l;
-        return f3{(float)(c.c2w[0] * dx + c.c2w[1] * dy + c.c2w[2] * dz), (float)(c.c2w[4] * dx + c.c2w[5] * dy + c.c2w[6] * dz),
-                  (float)(c.c2w[8] * dx + c.c2w[9] * dy + c.c2w[10] * dz)};
+        return f3{(real)(c.c2w[0] * dx + c.c2w[1] * dy + c.c2w[2] * dz), (real)(c.c2w[4] * dx + c.c2w[5] * dy + c.c2w[6] * dz),
+                  (real)(c.c2w[8] * dx + c.c2w[9] * dy + c.c2w[10] * dz)};
     };
     f3 org{c.c2w[3], c.c2w[7], c.c2w[11]};
-    float trans = T.evalTransmittance(org, camdir(px_, py_), false, c.med_id, maxD, r, P.max_bounces - 1);
+    real trans = T.evalTransmittance(org, camdir(px_, py_), false, c.med_id, maxD, r, P.max_bounces - 1);
     if (!(trans > EPS)) return;
     bool inside = c.cropped ? (ipx >= c.ox && ipx < c.ox + c.cw && ipy >= c.oy && ipy < c.oy + c.ch)
                             : (ipx >= 0 && ipx < c.width && ipy >= 0 && ipy < c.height);
     if (!inside) return;
     int pix = vw.pix0 + (c.cropped ? (ipx - c.ox) + c.cw * (ipy - c.oy) : ipx + c.width * ipy);
-    float W[N * 3];
-    float scale = (float)(vw.pe_sum / (double)total);
+    real W[N * 3];
+    real scale = (real)(vw.pe_sum / (double)total);
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        float w = (float)(nx * pd[j][0] + ny * pd[j][1]) * scale;
+        real w = (real)(nx * pd[j][0] + ny * pd[j][1]) * scale;
         W[3 * j] = w; W[3 * j + 1] = w; W[3 * j + 2] = w;
     }
     ChildHdr h;
@@ -1207,29 +1219,29 @@ __global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, int
 
 // One view: pixels pix0 .. pix0+npix of the accumulators into its image.
 template <int N>
-__global__ void k_finalize(RenderParams P, int npix, int pix0, int spp, int has_pe, const float *acc, const float *acc_pe,
+__global__ void k_finalize(RenderParams P, int npix, int pix0, int spp, int has_pe, const real *acc, const real *acc_pe,
                            float *out, unsigned *stats) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const size_t nd = P.npix, g = (size_t)pix0 + p;   // accumulator stride and pixel; output stride is npix
     if (spp > 0) {
-        float inv = 1.0f / spp;
-        float v0 = acc[g * 3] * inv, v1 = acc[g * 3 + 1] * inv, v2 = acc[g * 3 + 2] * inv;
-        if (fminf(v0, fminf(v1, v2)) < 0.f) atomicAdd(stats + STAT_NEG_PIXEL, 1u);
-        else { out[p * 3] = v0; out[p * 3 + 1] = v1; out[p * 3 + 2] = v2; }
+        real inv = 1.0_r / spp;
+        real v0 = acc[g * 3] * inv, v1 = acc[g * 3 + 1] * inv, v2 = acc[g * 3 + 2] * inv;
+        if (fmin(v0, fmin(v1, v2)) < 0._r) atomicAdd(stats + STAT_NEG_PIXEL, 1u);
+        else { out[p * 3] = (float)v0; out[p * 3 + 1] = (float)v1; out[p * 3 + 2] = (float)v2; }
         for (int j = 0; j < N; ++j) {
-            const float *a = acc + ((j + 1) * nd + g) * 3;
-            float g0 = a[0] * inv, g1 = a[1] * inv, g2 = a[2] * inv;
+            const real *a = acc + ((j + 1) * nd + g) * 3;
+            real g0 = a[0] * inv, g1 = a[1] * inv, g2 = a[2] * inv;
             if (!isfinite(g0) || !isfinite(g1) || !isfinite(g2)) { atomicAdd(stats + STAT_NONFINITE, 1u); continue; }
             float *o = out + ((size_t)(j + 1) * npix + p) * 3;
-            o[0] = g0; o[1] = g1; o[2] = g2;
+            o[0] = (float)g0; o[1] = (float)g1; o[2] = (float)g2;
         }
     }
     if (has_pe)
         for (int j = 0; j < N; ++j) {
-            const float *a = acc_pe + (j * nd + g) * 3;
+            const real *a = acc_pe + (j * nd + g) * 3;
             float *o = out + ((size_t)(j + 1) * npix + p) * 3;
-            o[0] += a[0]; o[1] += a[1]; o[2] += a[2];
+            o[0] = (float)(o[0] + a[0]); o[1] = (float)(o[1] + a[1]); o[2] = (float)(o[2] + a[2]);
         }
 }
 
@@ -1241,11 +1253,11 @@ __global__ void k_trace(DScene S, const float *rays, int n, int *out_face, float
     if (i >= n) return;
     const float *r = rays + (size_t)i * 8;
     f3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
-    float t;
+    real t;
     int f = T.trace(o, d, r[6], r[7], t);
     out_face[i] = f;
-    if (f >= 0) { Hit h; refineHit(S, f, o, d, h); out_t[i] = h.t; }
-    else out_t[i] = FINF;
+    if (f >= 0) { Hit h; refineHit(S, f, o, d, h); out_t[i] = (float)h.t; }
+    else out_t[i] = (float)FINF;
 }
 __global__ void k_li(DScene S, const float *rays, int n, int depth, int med, uint64_t seed, float *out) {
     Tracer<1> T(S);
@@ -1255,7 +1267,7 @@ __global__ void k_li(DScene S, const float *rays, int n, int depth, int med, uin
     Rng r;
     r.init(seed, (uint64_t)i);
     f3 L = T.Li(r, f3{rr[0], rr[1], rr[2]}, f3{rr[3], rr[4], rr[5]}, depth, med, true);
-    out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+    out[3 * i] = (float)L.x; out[3 * i + 1] = (float)L.y; out[3 * i + 2] = (float)L.z;
 }
 __global__ void k_medium(DScene S, int med, int kind, const float *seg, int n, uint64_t seed, float *out) {
     int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -1265,9 +1277,9 @@ __global__ void k_medium(DScene S, int med, int kind, const float *seg, int n, u
     r.init(seed, (uint64_t)i);
     f3 o{s[0], s[1], s[2]}, d{s[3], s[4], s[5]};
     const DMedium &m = S.media[med];
-    if (kind == 0) out[i] = mediumTransmittance(m, o, d, s[6], s[7], r);
+    if (kind == 0) out[i] = (float)mediumTransmittance(m, o, d, s[6], s[7], r);
     else {
-        float a = r.next(), b = r.next();
+        real a = r.next(), b = r.next();
         f3 ps, thr{1, 1, 1};
         out[i] = sampleDistance(m, o, d, s[7], a, b, r, ps, thr) ? 1.f : 0.f;
     }
@@ -1284,8 +1296,8 @@ template <int N> __global__ void k_grid(DScene S, int med, const float *pts, int
     Dv<N> x(f3{p[0], p[1], p[2]});
     for (int j = 0; j < N; ++j) x.setGrad(j, f3{p[3 + 3 * j], p[4 + 3 * j], p[5 + 3 * j]});
     Df<N> v = lookupDensityAD<N>(S.media[med], x);
-    out[(size_t)i * (1 + N)] = v.v;
-    for (int j = 0; j < N; ++j) out[(size_t)i * (1 + N) + 1 + j] = v.d[j];
+    out[(size_t)i * (1 + N)] = (float)v.v;
+    for (int j = 0; j < N; ++j) out[(size_t)i * (1 + N) + 1 + j] = (float)v.d[j];
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1294,27 +1306,33 @@ template <int N> __global__ void k_grid(DScene S, int med, const float *pts, int
 // Launchers of one dual width: defined and instantiated in that width's object,
 // called through the dispatchers of the width-independent one.
 template <int N> struct WidthOps {
-    static hipError_t setFrameRotation(const float *m9);
-    static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
+    static hipError_t setFrameRotation(const double *m9);
+    static void launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, real *acc,
                          char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *wk,
                          hipStream_t sm);
-    static void launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc,
-                               float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm);
+    static void launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, real *acc,
+                               real *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm);
     static void launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total,
                               char *q, unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm);
-    static void launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
-                               const float *acc_pe, float *out, unsigned *st, hipStream_t sm);
+    static void launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const real *acc,
+                               const real *acc_pe, float *out, unsigned *st, hipStream_t sm);
     static void launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);
     static void launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,
                               float *grad, int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm);
 };
 
+template <class Sym> static hipError_t setRotation(const Sym &sym, const double *m9) {
+    FrameRot r;
+    for (int i = 0; i < 9; ++i) r.m[i] = (real)m9[i];
+    return hipMemcpyToSymbol(sym, &r, sizeof(FrameRot));
+}
+
 #ifdef DT_N
-template <int N> hipError_t WidthOps<N>::setFrameRotation(const float *m9) {
-    return hipMemcpyToSymbol(HIP_SYMBOL(DT_FRAME_ROT), m9, sizeof(FrameRot));
+template <int N> hipError_t WidthOps<N>::setFrameRotation(const double *m9) {
+    return setRotation(HIP_SYMBOL(DT_FRAME_ROT), m9);
 }
 template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,
-                                      float *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,
+                                      real *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,
                                       unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     if (P.ad_kind == AD_PATH)
@@ -1339,7 +1357,7 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
     }
 }
 template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
-                                            float *acc, float *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
+                                            real *acc, real *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
     if (P.ad_kind == AD_VOLPATH)
         hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
     else
@@ -1351,8 +1369,8 @@ template <int N> void WidthOps<N>::launchPrimary(const DScene &S, const RenderPa
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, view, s0, n, total, q, qc, st, wk);
 }
-template <int N> void WidthOps<N>::launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
-                                            const float *acc_pe, float *out, unsigned *st, hipStream_t sm) {
+template <int N> void WidthOps<N>::launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const real *acc,
+                                            const real *acc_pe, float *out, unsigned *st, hipStream_t sm) {
     hipLaunchKernelGGL(k_finalize<N>, dim3((npix + 255) / 256), dim3(256), 0, sm, P, npix, pix0, spp, has_pe, acc, acc_pe,
                        out, st);
 }
@@ -1367,10 +1385,10 @@ template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderPa
 }
 template struct WidthOps<DT_N>;
 
-#else  // the width-independent object: dispatchers and the remaining launchers
+#else  // the width-independent object of this precision: dispatchers over S.nder and the remaining launchers
 
-hipError_t setFrameRotation(const float *m9) {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(DT_FRAME_ROT), m9, sizeof(FrameRot));
+hipError_t setFrameRotation(const double *m9) {
+    hipError_t e = setRotation(HIP_SYMBOL(DT_FRAME_ROT), m9);
 #define DT_SET_ROT(n) \
     if (e == hipSuccess) e = WidthOps<n>::setFrameRotation(m9);
     DT_FOR_N(DT_SET_ROT)
@@ -1390,14 +1408,16 @@ hipError_t setFrameRotation(const float *m9) {
     }                                                     \
     return hipGetLastError();
 
-hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
+hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, void *acc,
                            char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *wk,
                            hipStream_t sm) {
-    DT_DISPATCH(launchAd, S, P, s0, n, replay, acc, q, qc, st, ovf, ovf_count, wk, sm)
+    real *A = static_cast<real *>(acc);
+    DT_DISPATCH(launchAd, S, P, s0, n, replay, A, q, qc, st, ovf, ovf_count, wk, sm)
 }
-hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc, float *acc_pe,
+hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, void *acc, void *acc_pe,
                            unsigned long long *wk, unsigned grid, hipStream_t sm) {
-    DT_DISPATCH(launchChildren, S, P, q, qc, acc, acc_pe, wk, grid, sm)
+    real *A = static_cast<real *>(acc), *APE = static_cast<real *>(acc_pe);
+    DT_DISPATCH(launchChildren, S, P, q, qc, A, APE, wk, grid, sm)
 }
 hipError_t launch_primary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total, char *q,
                           unsigned *qc, unsigned *st, unsigned long long *wk, hipStream_t sm) {
@@ -1420,9 +1440,10 @@ hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigne
     hipLaunchKernelGGL(k_load_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sm, dst, src, n, maxbits);
     return hipGetLastError();
 }
-hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
-                           const float *acc_pe, float *out, unsigned *st, hipStream_t sm) {
-    DT_DISPATCH(launchFinalize, P, npix, pix0, spp, has_pe, acc, acc_pe, out, st, sm)
+hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int pix0, int spp, int has_pe, const void *acc,
+                           const void *acc_pe, float *out, unsigned *st, hipStream_t sm) {
+    const real *A = static_cast<const real *>(acc), *APE = static_cast<const real *>(acc_pe);
+    DT_DISPATCH(launchFinalize, P, npix, pix0, spp, has_pe, A, APE, out, st, sm)
 }
 hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm) {
     DT_DISPATCH(launchGrid, S, med, pts, n, out, sm)
@@ -1431,31 +1452,33 @@ hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, fl
     hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, face, t);
     return hipGetLastError();
 }
-hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, float *acc,
+hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,
                               unsigned long long *work, hipStream_t sm) {
     const uint64_t nb = (n + BLOCK - 1) / BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    real *A = static_cast<real *>(acc);
     if (kind == PRIMAL_SIMPLE)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, A, work);
     else if (kind == PRIMAL_PATH)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, A, work);
     else if (kind == PRIMAL_VOLPATH)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, A, work);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
-hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc,
+hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, void *acc,
                             unsigned long long *work, hipStream_t sm) {
     const uint64_t nb = (n + BLOCK - 1) / BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, acc, work);
+    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, static_cast<real *>(acc), work);
     return hipGetLastError();
 }
-hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm) {
+hipError_t launch_scale(const DScene &S, const void *acc, float *out, uint64_t n, double scale, hipStream_t sm) {
+    (void)S;
     const uint64_t nb = (n + BLOCK - 1) / BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scale, dim3((unsigned)nb), dim3(BLOCK), 0, sm, acc, out, n, scale);
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)nb), dim3(BLOCK), 0, sm, static_cast<const real *>(acc), out, n, (real)scale);
     return hipGetLastError();
 }
 hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm) {
@@ -1472,4 +1495,5 @@ hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out
 }
 #endif  // DT_N
 
+}  // namespace DT_PNS
 }  // namespace dt

@@ -34,46 +34,68 @@ enum Stat {
 // an fp32 direction apart up to ~1000-px-wide images.
 constexpr double PRIMARY_DELTA = 1e-4;
 
-hipError_t setFrameRotation(const float *m9);
-// work[0..3] AD paths, [4..7] child sub-paths, [8..11] primary-edge samples:
-// box tests, triangle tests, density lookups, traced segments.
+// The launchers exist once per estimator precision: in namespace p32 (float) and p64
+// (double), each defined by the kernel objects of that precision, and once in
+// namespace dt itself, where they dispatch on S.f64 (dispatch.cpp). Accumulators
+// (`acc`, `acc_pe`) hold the estimator's type: float for p32, double for p64.
+//
+// setFrameRotation: the fixed local-frame rotation (row-major 3x3) into every kernel object.
+// launch_ad_paths: work[0..3] AD paths, [4..7] child sub-paths, [8..11] primary-edge
+// samples: box tests, triangle tests, density lookups, traced segments.
 // AD samples s0 .. s0+n-1, or (replay != nullptr) the n listed {sid lo, sid hi,
 // children delivered, 0} entries; samples whose child pushes overflow the queue are
 // appended to ovf[*ovf_count] (capacity >= n) and contribute nothing in this launch.
-hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay, float *acc,
-                           char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count, unsigned long long *work,
-                           hipStream_t sm);
-// Drains the qc[0] queued records (qc[2] = 0 on entry: the consumption counter).
-hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, float *acc, float *acc_pe,
-                           unsigned long long *work, unsigned grid, hipStream_t sm);
-// Primary-edge samples s0 .. s0+n-1 (of `total`) of view `view`.
-hipError_t launch_primary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n, uint64_t total, char *q,
-                          unsigned *qc, unsigned *st, unsigned long long *work, hipStream_t sm);
-// One view: accumulator pixels pix0 .. pix0+npix into `out` (that view's image).
-hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int pix0, int spp, int has_pe, const float *acc,
-                           const float *acc_pe, float *out, unsigned *st, hipStream_t sm);
-// Path-replay adjoint over AD samples s0 .. s0+n-1 (interior terms only): adds
-// d loss / d node density of medium pmed's grid into grad, given d loss / d pixel.
-hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg, float *grad,
-                          int pmed, unsigned *st, unsigned long long *work, hipStream_t sm);
-// p[0..n) = v (32-bit words); *dst += *src. Kernels, so that captured graphs hold no memset nodes.
-hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm);
-hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm);
-// dst = max(src, 0) over n floats; atomicMax of the result's bits into *maxbits.
-hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm);
-// Primal integrators: samples s0 .. s0+n-1 of view 0, radiance sums into acc[pixel][3].
+// launch_children: drains the qc[0] queued records (qc[2] = 0 on entry: the consumption counter).
+// launch_primary: primary-edge samples s0 .. s0+n-1 (of `total`) of view `view`.
+// launch_finalize: one view, accumulator pixels pix0 .. pix0+npix into `out` (that view's image).
+// launch_adjoint: path-replay adjoint over AD samples s0 .. s0+n-1 (interior terms only):
+// adds d loss / d node density of medium pmed's grid into grad, given d loss / d pixel.
+// launch_fill32: p[0..n) = v (32-bit words); launch_add_u32: *dst += *src. Kernels, so
+// that captured graphs hold no memset nodes.
+// launch_load_density: dst = max(src, 0) over n floats; atomicMax of the result's bits
+// into *maxbits.
+// launch_primal_paths: samples s0 .. s0+n-1 of view 0, radiance sums into acc[pixel][3].
+// launch_particles: ParticleTracer particles s0 .. s0+n-1 splatted into acc[pixel][3]
+// of view 0's window.
+// launch_scale: out[i] = acc[i] * scale, i < n.
+#define DT_LAUNCH_API                                                                                                   \
+    hipError_t setFrameRotation(const double *m9);                                                                       \
+    hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,    \
+                               void *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,          \
+                               unsigned long long *work, hipStream_t sm);                                                \
+    hipError_t launch_children(const DScene &S, const RenderParams &P, const char *q, unsigned *qc, void *acc,          \
+                               void *acc_pe, unsigned long long *work, unsigned grid, hipStream_t sm);                   \
+    hipError_t launch_primary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,                 \
+                              uint64_t total, char *q, unsigned *qc, unsigned *st, unsigned long long *work,             \
+                              hipStream_t sm);                                                                           \
+    hipError_t launch_finalize(const DScene &S, const RenderParams &P, int npix, int pix0, int spp, int has_pe,          \
+                               const void *acc, const void *acc_pe, float *out, unsigned *st, hipStream_t sm);           \
+    hipError_t launch_adjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,        \
+                              float *grad, int pmed, unsigned *st, unsigned long long *work, hipStream_t sm);            \
+    hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm);                                           \
+    hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm);                                       \
+    hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm);         \
+    hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,  \
+                                   unsigned long long *work, hipStream_t sm);                                            \
+    hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, void *acc,             \
+                                unsigned long long *work, hipStream_t sm);                                               \
+    hipError_t launch_scale(const DScene &S, const void *acc, float *out, uint64_t n, double scale, hipStream_t sm);    \
+    hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);             \
+    hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out,       \
+                         hipStream_t sm);                                                                                \
+    hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out,     \
+                             hipStream_t sm);                                                                            \
+    hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);              \
+    hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm);
+
 enum PrimalKind { PRIMAL_VOLPATH = 1, PRIMAL_SIMPLE = 2, PRIMAL_PATH = 3 };
-hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, float *acc,
-                              unsigned long long *work, hipStream_t sm);
-// ParticleTracer: particles s0 .. s0+n-1 splatted into acc[pixel][3] of view 0's window.
-hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, float *acc,
-                            unsigned long long *work, hipStream_t sm);
-// out[i] = acc[i] * scale, i < n.
-hipError_t launch_scale(const float *acc, float *out, uint64_t n, float scale, hipStream_t sm);
-hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm);
-hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm);
-hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out, hipStream_t sm);
-hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);
-hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm);
+
+namespace p32 {
+DT_LAUNCH_API
+}
+namespace p64 {
+DT_LAUNCH_API
+}
+DT_LAUNCH_API
 
 }  // namespace dt

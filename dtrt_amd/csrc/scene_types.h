@@ -80,6 +80,7 @@ struct DMedium {
 
 struct DScene {
     int nder, nshapes, nlights, nmedia;
+    int f64;                    // estimator precision of the scene's kernels: 1 double, 0 float
     int NV, NF, ntris, nnodes;
     const float4 *nodes;        // 4 float4 per node
     const float4 *tris;         // 3 float4 per triangle, leaf order
@@ -128,16 +129,23 @@ struct DView {
 // primary-edge kernel, consumed by the child kernel.
 enum ChildKind { CHILD_LI = 0, CHILD_LINS_NULL = 1, CHILD_LINS_SURF = 2, CHILD_PATH = 3 };   // PATH: path.cpp radiance
 
-struct ChildHdr {
-    float org[3];
-    float dir[3];               // CHILD_LI: ray direction; LINS: incoming ray direction
+// R: the estimator's scalar type (float or double), so a sub-path starts from the
+// exact point and direction its parent computed.
+template <class R> struct ChildHdrT {
+    R org[3];
+    R dir[3];                   // CHILD_LI: ray direction; LINS: incoming ray direction
     int pixel;
     int meta;                   // kind | (max_depth << 4) | ((med + 1) << 12) | (face id in LINS records via `face`)
     uint32_t key_lo, key_hi;    // RNG stream of this child
     int face;                   // LINS: global face id of the boundary hit
-    float gn[3], sn[3];         // LINS: geometric and shading normals at the hit
+    R gn[3], sn[3];             // LINS: geometric and shading normals at the hit
     int view;                   // view of the sample that pushed the record (its seed keys the stream)
 };
-// followed by float w[nder][3] (weights applied to the child's RGB radiance)
+// followed by R w[nder][3] (weights applied to the child's RGB radiance); records are
+// 16-byte aligned
+inline int childRecordStride(int nder, bool f64) {
+    const size_t h = f64 ? sizeof(ChildHdrT<double>) : sizeof(ChildHdrT<float>);
+    return (int)((h + 3 * (size_t)nder * (f64 ? 8 : 4) + 15) / 16 * 16);
+}
 
 }  // namespace dt

@@ -7,8 +7,9 @@
 #include "dscene.h"
 
 namespace dt {
+namespace DT_PNS {
 
-DI float misPower(float a, float b) { a *= a; b *= b; return a / (a + b); }   // volpathAD.cpp:43-47
+DI real misPower(real a, real b) { a *= a; b *= b; return a / (a + b); }   // volpathAD.cpp:43-47
 
 // State of one primal path between bounces (see Tracer::liStep).
 struct LiPath {
@@ -23,7 +24,7 @@ template <int N> struct Tracer {
     uint32_t nbox = 0, ntri = 0, nseg = 0;   // work counters: box tests, triangle tests, traced segments
     DI explicit Tracer(const DScene &s) : S(s) {}
 
-    DI int trace(f3 o, f3 d, float tnear, float tfar, float &t) {
+    DI int trace(f3 o, f3 d, real tnear, real tfar, real &t) {
         nseg++;
         TraceHit h = traceClosest(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar);
         nbox += h.nbox;
@@ -34,32 +35,32 @@ template <int N> struct Tracer {
 
     // scene.cpp:538-575
     DI bool rayIntersect(f3 o, f3 d, bool onSurface, Hit &h) {
-        float t;
-        int f = trace(o, d, onSurface ? SHADOW_EPS : 0.f, FINF, t);
+        real t;
+        int f = trace(o, d, onSurface ? SHADOW_EPS : 0._r, FINF, t);
         if (f < 0) { h.face = -1; h.t = FINF; return false; }
         refineHit(S, f, o, d, h);
         return true;
     }
     // scene.cpp:577-615
     DI bool rayIntersectAD(const Dv<N> &o, const Dv<N> &d, bool onSurface, HitAD<N> &h) {
-        float t;
-        int f = trace(o.val(), d.val(), onSurface ? SHADOW_EPS : 0.f, FINF, t);
+        real t;
+        int f = trace(o.val(), d.val(), onSurface ? SHADOW_EPS : 0._r, FINF, t);
         if (f < 0) { h.face = -1; h.t = Df<N>(FINF); return false; }
         refineHitAD<N>(S, f, o, d, h);
         return true;
     }
 
     // scene.cpp:177-237 — shadow-ray transmittance through null interfaces
-    DI float evalTransmittance(f3 o, f3 d, bool onSurface, int med, float remaining, Rng &r, int max_inter) {
-        float tmin = onSurface ? SHADOW_EPS : 0.f, T = 1.f, tnear = tmin, tfar = (1.0f - SHADOW_EPS) * remaining;
+    DI real evalTransmittance(f3 o, f3 d, bool onSurface, int med, real remaining, Rng &r, int max_inter) {
+        real tmin = onSurface ? SHADOW_EPS : 0._r, T = 1._r, tnear = tmin, tfar = (1.0_r - SHADOW_EPS) * remaining;
         int inter = 0;
-        while (remaining > 0.f) {
-            float tmax = remaining, th;
+        while (remaining > 0._r) {
+            real tmax = remaining, th;
             int f = trace(o, d, tnear, tfar, th);
             Hit nh;
             if (f >= 0) {
                 const DShape &sh = shapeOf(S, f);
-                if (inter == max_inter || bsdfOf(S, sh).type != BSDF_NULL) return 0.f;
+                if (inter == max_inter || bsdfOf(S, sh).type != BSDF_NULL) return 0._r;
                 refineHit(S, f, o, d, nh);
                 if (nh.t < tmax) tmax = nh.t;
             }
@@ -70,29 +71,29 @@ template <int N> struct Tracer {
             o = o + d * tmax;
             tnear = tmin = SHADOW_EPS;
             remaining -= tmax;
-            tfar = (1.0f - SHADOW_EPS) * remaining;
+            tfar = (1.0_r - SHADOW_EPS) * remaining;
             inter++;
         }
         return T;
     }
 
     // scene.cpp:617-701
-    DI f3 lookForEmitter(f3 o, f3 d, bool onSurface, Rng &r, int med, int max_inter, Hit &its, float &pdf_nee) {
-        float tmin = onSurface ? SHADOW_EPS : 0.f, th;
+    DI f3 lookForEmitter(f3 o, f3 d, bool onSurface, Rng &r, int med, int max_inter, Hit &its, real &pdf_nee) {
+        real tmin = onSurface ? SHADOW_EPS : 0._r, th;
         f3 scat = o;
         int f = trace(o, d, tmin, FINF, th);
         if (f < 0) { its.face = -1; its.t = FINF; return f3{0, 0, 0}; }
         refineHit(S, f, o, d, its);
         Hit nh = its;
-        float T = 1.f;
+        real T = 1._r;
         int inter = 0;
         while (true) {
             const DShape &sh = shapeOf(S, f);
-            float tmax = nh.t;
+            real tmax = nh.t;
             if (med >= 0) T *= mediumTransmittance(S.media[med], o, d, tmin, tmax, r);
             if (sh.light_id >= 0) {
                 f3 dv = nh.p - scat;
-                float G = nh.wi.z / dot(dv, dv);
+                real G = nh.wi.z / dot(dv, dv);
                 pdf_nee = S.light_pmf[sh.light_id] / (S.light_areas[sh.light_id] * G);
                 return emitterEval(S.emitters[sh.light_id], nh.gn, nh.sn, -d) * T;
             }
@@ -109,14 +110,14 @@ template <int N> struct Tracer {
 
     // scene.cpp:703-793 — with tangents; reports the far hit's face and AD distance
     DI Ds<N> lookForEmitterAD(Dv<N> o, const Dv<N> &d, bool onSurface, Rng &r, int med, int max_inter, HitAD<N> &its,
-                              float &pdf_nee, int *far_face, float *far_t, const ScatterCtx *scx = nullptr) {
-        float tmin = onSurface ? SHADOW_EPS : 0.f, th;
+                              real &pdf_nee, int *far_face, real *far_t, const ScatterCtx *scx = nullptr) {
+        real tmin = onSurface ? SHADOW_EPS : 0._r, th;
         f3 scat = o.val(), dv = d.val();
         int f = trace(scat, dv, tmin, FINF, th);
         if (f < 0) { its.face = -1; its.t = Df<N>(FINF); return Ds<N>(); }
         refineHitAD<N>(S, f, o, d, its);
         HitAD<N> nh = its;
-        Df<N> T(1.f), dist(0.f);
+        Df<N> T(1._r), dist(0._r);
         int inter = 0;
         while (true) {
             const DShape &sh = shapeOf(S, f);
@@ -125,7 +126,7 @@ template <int N> struct Tracer {
             if (med >= 0) T *= mediumTransmittanceAD<N>(S.media[med], o, d, tmin, tmax, r, scx);
             if (sh.light_id >= 0) {
                 f3 dd = nh.p.val() - scat;
-                float G = nh.wi.z.v / dot(dd, dd);
+                real G = nh.wi.z.v / dot(dd, dd);
                 pdf_nee = S.light_pmf[sh.light_id] / (S.light_areas[sh.light_id] * G);
                 if (far_face) { *far_face = f; *far_t = dist.v; }
                 return emitterEvalAD<N>(S.emitters[sh.light_id], nh, -d) * T;
@@ -142,14 +143,14 @@ template <int N> struct Tracer {
     }
 
     // scene.cpp:401-410 — light by power, triangle by area, point by (1-sqrt u, sqrt u * v)
-    DI int sampleLight(const float *r4, f3 &pos, f3 &nrm) {
+    DI int sampleLight(const real *r4, f3 &pos, f3 &nrm) {
         int l = upperBound(S.light_cdf, S.nlights, r4[0]);
         const DEmitter &e = S.emitters[l];
         const DShape &sh = S.shapes[e.shape_id];
         int k = upperBound(S.area_cdf_pool + S.area_cdf_begin[l], sh.nf, r4[1]);
         const int *fi = S.F + 3 * (size_t)(sh.fbase + k);
         f3 v0 = ldg3(S.V + 3 * (size_t)fi[0]), v1 = ldg3(S.V + 3 * (size_t)fi[1]), v2 = ldg3(S.V + 3 * (size_t)fi[2]);
-        float a = sqrtf(r4[2]), b1 = 1.0f - a, b2 = a * r4[3];
+        real a = sqrt(r4[2]), b1 = 1.0_r - a, b2 = a * r4[3];
         f3 e1 = v1 - v0, e2 = v2 - v0;
         nrm = normalize(cross(e1, e2));
         pos = v0 + e1 * b1 + e2 * b2;
@@ -157,17 +158,17 @@ template <int N> struct Tracer {
     }
 
     // scene.cpp:399-426 (point in a medium; wo in world space)
-    DI f3 neePoint(f3 ps, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf) {
+    DI f3 neePoint(f3 ps, const real *r4, Rng &r, int med, int max_inter, f3 &wo, real &pdf) {
         if (S.nlights == 0) return f3{0, 0, 0};
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
         f3 dir = lp - ps;
         pdf = S.light_pmf[l] / S.light_areas[l];
-        if (dot(dir, ln) < 0.f && pdf != 0.f) {
-            float dist = length(dir);
+        if (dot(dir, ln) < 0._r && pdf != 0._r) {
+            real dist = length(dir);
             dir = dir / dist;
-            float T = evalTransmittance(ps, dir, false, med, dist, r, max_inter);
-            if (T != 0.f) {
+            real T = evalTransmittance(ps, dir, false, med, dist, r, max_inter);
+            if (T != 0._r) {
                 pdf *= dist * dist / dot(ln, -dir);
                 wo = dir;
                 return emitterEvalNorm(S.emitters[l], ln, -dir) * (T / pdf);
@@ -176,7 +177,7 @@ template <int N> struct Tracer {
         return f3{0, 0, 0};
     }
     // scene.cpp:456-491 (surface point; wo in the shading frame)
-    DI f3 neeSurface(const Hit &its, const Frame &shf, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf,
+    DI f3 neeSurface(const Hit &its, const Frame &shf, const real *r4, Rng &r, int med, int max_inter, f3 &wo, real &pdf,
                      bool flag) {
         if (S.nlights == 0) return f3{0, 0, 0};
         f3 lp, ln;
@@ -186,16 +187,16 @@ template <int N> struct Tracer {
         const DShape &lsh = S.shapes[S.emitters[l].shape_id];
         int lbt = bsdfOf(S, lsh).type;
         bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
-        if ((transm || dot(dir, its.sn) > 0.f) && dot(dir, ln) < 0.f && pdf != 0.f) {
+        if ((transm || dot(dir, its.sn) > 0._r) && dot(dir, ln) < 0._r && pdf != 0._r) {
             const DShape &sh = shapeOf(S, its.face);
             if (isTransition(sh)) med = targetMedium(sh, its.gn, dir);
-            float dist = length(dir);
+            real dist = length(dir);
             dir = dir / dist;
             wo = shf.toLocal(dir);
             if (flag && signum(its.wi.z) != signum(wo.z)) --max_inter;
             if (max_inter >= 0) {
-                float T = evalTransmittance(its.p, dir, true, med, dist, r, max_inter);
-                if (T != 0.f) {
+                real T = evalTransmittance(its.p, dir, true, med, dist, r, max_inter);
+                if (T != 0._r) {
                     pdf *= dist * dist / dot(ln, -dir);
                     return emitterEvalNorm(S.emitters[l], ln, -dir) * (T / pdf);
                 }
@@ -204,30 +205,30 @@ template <int N> struct Tracer {
         return f3{0, 0, 0};
     }
     // scene.cpp:428-454
-    DI Ds<N> neePointAD(const Dv<N> &ps, const float *r4, Rng &r, int med, int max_inter, f3 &wo, float &pdf,
+    DI Ds<N> neePointAD(const Dv<N> &ps, const real *r4, Rng &r, int med, int max_inter, f3 &wo, real &pdf,
                         const ScatterCtx *scx = nullptr) {
         if (S.nlights == 0) return Ds<N>();
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
         f3 dir = lp - ps.val();
-        if (dot(dir, ln) < 0.f) {
-            float dist = length(dir);
+        if (dot(dir, ln) < 0._r) {
+            real dist = length(dir);
             dir = dir / dist;
             HitAD<N> inear;
             int ff = -1;
-            float ft = 0.f;
+            real ft = 0._r;
             Ds<N> ret = lookForEmitterAD(ps, Dv<N>(dir), false, r, med, max_inter, inear, pdf, &ff, &ft, scx);
             if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
-                fabsf(ft - dist) < SHADOW_EPS) {
+                fabs(ft - dist) < SHADOW_EPS) {
                 wo = dir;
-                return ret * (1.0f / pdf);
+                return ret * (1.0_r / pdf);
             }
         }
         return Ds<N>();
     }
     // scene.cpp:493-524
-    DI Ds<N> neeSurfaceAD(const HitAD<N> &its, const FrameAD<N> &shf, const float *r4, Rng &r, int med, int max_inter,
-                          Dv<N> &wo, float &pdf, const ScatterCtx *scx = nullptr) {
+    DI Ds<N> neeSurfaceAD(const HitAD<N> &its, const FrameAD<N> &shf, const real *r4, Rng &r, int med, int max_inter,
+                          Dv<N> &wo, real &pdf, const ScatterCtx *scx = nullptr) {
         if (S.nlights == 0) return Ds<N>();
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
@@ -235,19 +236,19 @@ template <int N> struct Tracer {
         const DShape &lsh = S.shapes[S.emitters[l].shape_id];
         int lbt = bsdfOf(S, lsh).type;
         bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
-        if ((transm || dot(dir, its.sn.val()) > 0.f) && dot(dir, ln) < 0.f) {
+        if ((transm || dot(dir, its.sn.val()) > 0._r) && dot(dir, ln) < 0._r) {
             const DShape &sh = shapeOf(S, its.face);
             if (isTransition(sh)) med = targetMedium(sh, its.gn.val(), dir);
-            float dist = length(dir);
+            real dist = length(dir);
             dir = dir / dist;
             HitAD<N> inear;
             int ff = -1;
-            float ft = 0.f;
+            real ft = 0._r;
             Ds<N> ret = lookForEmitterAD(its.p, Dv<N>(dir), true, r, med, max_inter, inear, pdf, &ff, &ft, scx);
             if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
-                fabsf(ft - dist) < SHADOW_EPS) {
+                fabs(ft - dist) < SHADOW_EPS) {
                 wo = shf.toLocal(Dv<N>(dir));
-                return ret * (1.0f / pdf);
+                return ret * (1.0_r / pdf);
             }
         }
         return Ds<N>();
@@ -256,23 +257,23 @@ template <int N> struct Tracer {
     // scene.cpp:346-397 — sampleEmitterDirectAD: no media; the shadow ray must first hit
     // the sampled light within ShadowEpsilon of the sampled distance; the value's
     // tangents come from the emitter only (pdf and geometry term are primal)
-    DI Ds<N> neeDirectAD(const HitAD<N> &its, const FrameAD<N> &shf, const float *r4, Dv<N> &wo, float &pdf) {
+    DI Ds<N> neeDirectAD(const HitAD<N> &its, const FrameAD<N> &shf, const real *r4, Dv<N> &wo, real &pdf) {
         if (S.nlights == 0) return Ds<N>();
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
         f3 dir = lp - its.p.val();
         pdf = S.light_pmf[l] / S.light_areas[l];
-        if (dot(dir, its.sn.val()) > 0.f && dot(dir, ln) < 0.f && pdf != 0.f) {
-            float dist = length(dir), th;
+        if (dot(dir, its.sn.val()) > 0._r && dot(dir, ln) < 0._r && pdf != 0._r) {
+            real dist = length(dir), th;
             dir = dir / dist;
             int f = trace(its.p.val(), dir, SHADOW_EPS, FINF, th);
             if (f >= 0 && __ldg(S.face_shape + f) == S.emitters[l].shape_id) {
                 HitAD<N> nh;
                 refineHitAD<N>(S, f, its.p, Dv<N>(dir), nh);
-                if (fabsf(nh.t.v - dist) < SHADOW_EPS) {
+                if (fabs(nh.t.v - dist) < SHADOW_EPS) {
                     pdf *= dist * dist / dot(ln, -dir);
                     wo = shf.toLocal(Dv<N>(dir));
-                    return emitterEvalNormAD<N>(S.emitters[l], nh.sn, Dv<N>(-dir)) * (1.0f / pdf);
+                    return emitterEvalNormAD<N>(S.emitters[l], nh.sn, Dv<N>(-dir)) * (1.0_r / pdf);
                 }
             }
         }
@@ -304,9 +305,9 @@ template <int N> struct Tracer {
     DI bool liStep(LiPath &p, Rng &r) {
         if (p.depth > p.max_depth) return false;
         const int max_inter = p.max_depth - p.depth - 1;
-        float r1a = r.next(), r1b = r.next();
-        float r2[4] = {r.next(), r.next(), r.next(), r.next()};
-        float r3[3] = {r.next(), r.next(), r.next()};
+        real r1a = r.next(), r1b = r.next();
+        real r2[4] = {r.next(), r.next(), r.next(), r.next()};
+        real r3[3] = {r.next(), r.next(), r.next()};
         bool inside = false;
         if (p.med >= 0) {
             f3 ps;
@@ -318,10 +319,10 @@ template <int N> struct Tracer {
             if (allZero(p.thr, EPS)) return false;
             const DPhase &ph = S.phases[S.media[p.med].phase_id];
             f3 wo;
-            float pdf_nee;
+            real pdf_nee;
             f3 value = neePoint(p.ro, r2, r, p.med, max_inter, wo, pdf_nee);
             if (!allZero(value, EPS)) {
-                float pv = phaseEval(ph, -p.rd, wo);
+                real pv = phaseEval(ph, -p.rd, wo);
                 if (pv > EPS) p.ret = p.ret + p.thr * value * (misPower(pdf_nee, pv) * pv);
             }
             wo = phaseSample(ph, -p.rd, r3[0], r3[1]);
@@ -338,13 +339,13 @@ template <int N> struct Tracer {
             const DBsdf &b = bsdfOf(S, sh);
             Frame shf(p.its.sn);
             f3 wo;
-            float pdf_nee;
+            real pdf_nee;
             if (b.type != BSDF_NULL) {
                 f3 value = neeSurface(p.its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
                 if (!allZero(value, EPS))
                     p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, p.its.wi, wo));
             }
-            float bpdf = 0.f;
+            real bpdf = 0._r;
             f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
             if (allZero(bw, EPS)) return false;
             wo = shf.toWorld(wo);
@@ -356,7 +357,7 @@ template <int N> struct Tracer {
             } else {
                 f3 att = lookForEmitter(p.ro, p.rd, true, r, p.med, max_inter, p.its, pdf_nee);
                 p.thr = p.thr * bw;
-                if (!allZero(att, 1e-12f)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
+                if (!allZero(att, 1e-12_r)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
                 p.incEmission = false;
             }
         }
@@ -370,9 +371,9 @@ template <int N> struct Tracer {
     DI bool liSimpleStep(LiPath &p, Rng &r) {
         if (p.depth > p.max_depth) return false;
         const int max_inter = p.max_depth - p.depth - 1;
-        float r1a = r.next(), r1b = r.next();
-        float r2[4] = {r.next(), r.next(), r.next(), r.next()};
-        float r3[3] = {r.next(), r.next(), r.next()};
+        real r1a = r.next(), r1b = r.next();
+        real r2[4] = {r.next(), r.next(), r.next(), r.next()};
+        real r3[3] = {r.next(), r.next(), r.next()};
         bool inside = false;
         if (p.med >= 0) {
             f3 ps;
@@ -381,14 +382,14 @@ template <int N> struct Tracer {
         }
         if (inside) {
             if (p.depth >= p.max_depth) return false;
-            if (allZero(p.thr, 0.f)) return false;
+            if (allZero(p.thr, 0._r)) return false;
             const DPhase &ph = S.phases[S.media[p.med].phase_id];
             f3 wo;
-            float pdf_nee;
+            real pdf_nee;
             f3 value = neePoint(p.ro, r2, r, p.med, max_inter, wo, pdf_nee);
-            if (!allZero(value, 0.f)) {
-                float pv = phaseEval(ph, -p.rd, wo);
-                if (pv != 0.f) p.ret = p.ret + p.thr * value * pv;
+            if (!allZero(value, 0._r)) {
+                real pv = phaseEval(ph, -p.rd, wo);
+                if (pv != 0._r) p.ret = p.ret + p.thr * value * pv;
             }
             p.rd = phaseSample(ph, -p.rd, r3[0], r3[1]);   // weight eval/pdf = 1
             rayIntersect(p.ro, p.rd, false, p.its);
@@ -403,14 +404,14 @@ template <int N> struct Tracer {
             Frame shf(p.its.sn);
             f3 wo;
             if (b.type != BSDF_NULL) {
-                float pdf_nee;
+                real pdf_nee;
                 f3 value = neeSurface(p.its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
-                if (!allZero(value, 0.f)) p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo);
+                if (!allZero(value, 0._r)) p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo);
                 p.incEmission = false;
             }
-            float bpdf = 0.f;
+            real bpdf = 0._r;
             f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
-            if (allZero(bw, 0.f)) return false;
+            if (allZero(bw, 0._r)) return false;
             p.thr = p.thr * bw;
             wo = shf.toWorld(wo);
             p.ro = p.its.p;
@@ -424,19 +425,19 @@ template <int N> struct Tracer {
 
     // scene.cpp:302-344 — sampleEmitterDirect: no media; the light point is visible
     // when no surface lies within [ShadowEpsilon, (1 - ShadowEpsilon) * dist]
-    DI f3 neeDirect(const Hit &its, const Frame &shf, const float *r4, f3 &wo, float &pdf) {
+    DI f3 neeDirect(const Hit &its, const Frame &shf, const real *r4, f3 &wo, real &pdf) {
         if (S.nlights == 0) return f3{0, 0, 0};
         f3 lp, ln;
         int l = sampleLight(r4, lp, ln);
         f3 dir = lp - its.p;
         pdf = S.light_pmf[l] / S.light_areas[l];
-        if (dot(dir, its.sn) > 0.f && dot(dir, ln) < 0.f && pdf != 0.f) {
-            float dist = length(dir), th;
+        if (dot(dir, its.sn) > 0._r && dot(dir, ln) < 0._r && pdf != 0._r) {
+            real dist = length(dir), th;
             dir = dir / dist;
-            if (trace(its.p, dir, SHADOW_EPS, (1.0f - SHADOW_EPS) * dist, th) < 0) {
+            if (trace(its.p, dir, SHADOW_EPS, (1.0_r - SHADOW_EPS) * dist, th) < 0) {
                 pdf *= dist * dist / dot(ln, -dir);
                 wo = shf.toLocal(dir);
-                return emitterEvalNorm(S.emitters[l], ln, -dir) * (1.0f / pdf);
+                return emitterEvalNorm(S.emitters[l], ln, -dir) * (1.0_r / pdf);
             }
         }
         return f3{0, 0, 0};
@@ -454,16 +455,16 @@ template <int N> struct Tracer {
         if (p.depth >= p.max_depth) return false;
         const DBsdf &b = bsdfOf(S, sh);
         Frame shf(p.its.sn);
-        float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+        real r4[4] = {r.next(), r.next(), r.next(), r.next()};
         f3 wo;
-        float pdf_nee;
+        real pdf_nee;
         f3 value = neeDirect(p.its, shf, r4, wo, pdf_nee);
-        if (!allZero(value, 0.f))
+        if (!allZero(value, 0._r))
             p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, p.its.wi, wo));
-        float r3[3] = {r.next(), r.next(), r.next()};
-        float bpdf = 0.f;
+        real r3[3] = {r.next(), r.next(), r.next()};
+        real bpdf = 0._r;
         f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
-        if (allZero(bw, 0.f)) return false;
+        if (allZero(bw, 0._r)) return false;
         p.ro = p.its.p;
         p.rd = shf.toWorld(wo);
         if (!rayIntersect(p.ro, p.rd, true, p.its)) return false;
@@ -471,10 +472,10 @@ template <int N> struct Tracer {
         const DShape &hs = shapeOf(S, p.its.face);
         if (hs.light_id >= 0) {
             f3 lc = emitterEval(S.emitters[hs.light_id], p.its.gn, p.its.sn, -p.rd);
-            if (!allZero(lc, 0.f)) {
+            if (!allZero(lc, 0._r)) {
                 f3 dd = p.its.p - p.ro;
-                float G = p.its.wi.z / dot(dd, dd);
-                float pn = S.light_pmf[hs.light_id] / (S.light_areas[hs.light_id] * G);
+                real G = p.its.wi.z / dot(dd, dd);
+                real pn = S.light_pmf[hs.light_id] / (S.light_areas[hs.light_id] * G);
                 p.ret = p.ret + p.thr * lc * misPower(bpdf, pn);
             }
         }
@@ -500,10 +501,10 @@ template <int N> struct Tracer {
         const DBsdf &b = bsdfOf(S, shp);
         const DPhase &ph = S.phases[S.media[med].phase_id];
         Frame shf(its0.sn);
-        f3 n = dot(its0.gn, rdv) < 0.f ? -its0.gn : its0.gn;
+        f3 n = dot(its0.gn, rdv) < 0._r ? -its0.gn : its0.gn;
         f3 Lins{0, 0, 0}, wo;
-        float pdf_nee, pv = 0.f;
-        float r4[4] = {r.next(), r.next(), r.next(), r.next()};
+        real pdf_nee, pv = 0._r;
+        real r4[4] = {r.next(), r.next(), r.next(), r.next()};
         if (null_bsdf) {
             f3 att = neeSurface(its0, shf, r4, r, med, max_depth - 1, wo, pdf_nee, true);
             if (!allZero(att, EPS)) {
@@ -511,9 +512,9 @@ template <int N> struct Tracer {
                 pv = phaseEval(ph, -rdv, wo);
                 if (pv > EPS) Lins = Lins + att * (pv * misPower(pdf_nee, pv));
             }
-            float u1 = r.next(), u2 = r.next();
+            real u1 = r.next(), u2 = r.next();
             wo = phaseSample(ph, -rdv, u1, u2);
-            pv = 1.f;
+            pv = 1._r;
             int md1 = max_depth - 1;
             if (dot(wo, n) > EPS) {
                 if (shp.light_id >= 0) Lins = Lins + emitterEval(S.emitters[shp.light_id], its0.gn, its0.sn, -wo) * pv;
@@ -537,10 +538,10 @@ template <int N> struct Tracer {
                     if (pv > EPS) Lins = Lins + att * (pv * misPower(pdf_nee, pv));
                 }
             }
-            float u1 = r.next(), u2 = r.next();
+            real u1 = r.next(), u2 = r.next();
             wo = phaseSample(ph, -rdv, u1, u2);
-            pv = 1.f;
-            float wdn = dot(wo, n);
+            pv = 1._r;
+            real wdn = dot(wo, n);
             if (wdn < -EPS) {
                 Hit h1;
                 f3 att2 = lookForEmitter(its0.p, wo, true, r, med, max_depth - 1, h1, pdf_nee);
@@ -553,11 +554,11 @@ template <int N> struct Tracer {
                 if (max_depth > 1) {
                     Hit itsb = its0;
                     itsb.wi = shf.toLocal(-wo);
-                    float r4b[4] = {r.next(), r.next(), r.next(), r.next()};
+                    real r4b[4] = {r.next(), r.next(), r.next(), r.next()};
                     f3 wl;
                     f3 att3 = neeSurface(itsb, shf, r4b, r, med, max_depth - 2, wl, pdf_nee, false);
                     if (!allZero(att3, EPS)) Lins = Lins + bsdfEval(b, itsb.wi, wl) * att3 * (pv * misPower(pdf_nee, bsdfPdf(b, itsb.wi, wl)));
-                    float q0 = r.next(), q1 = r.next(), q2 = r.next(), bpdf = 0.f;
+                    real q0 = r.next(), q1 = r.next(), q2 = r.next(), bpdf = 0._r;
                     f3 bv = bsdfSample(b, itsb.wi, q0, q1, q2, wl, bpdf);
                     if (!allZero(bv, EPS)) {
                         f3 thr1 = bv * pv;
@@ -565,7 +566,7 @@ template <int N> struct Tracer {
                         int med1 = isTransition(shp) ? targetMedium(shp, itsb.gn, wow) : med;
                         Hit itsc;
                         f3 att4 = lookForEmitter(itsb.p, wow, true, r, med1, max_depth - 2, itsc, pdf_nee);
-                        if (!allZero(att4, 1e-12f)) Lins = Lins + thr1 * att4 * misPower(bpdf, pdf_nee);
+                        if (!allZero(att4, 1e-12_r)) Lins = Lins + thr1 * att4 * misPower(bpdf, pdf_nee);
                         liStartAt(cont, itsb.p, wow, max_depth - 2, med1, false, itsc);
                         cw = thr1;
                         has_cont = true;
@@ -589,4 +590,5 @@ template <int N> struct Tracer {
     }
 };
 
+}  // namespace DT_PNS
 }  // namespace dt

@@ -64,6 +64,24 @@ def __getattr__(name):
 if "DTRT_NDER" in os.environ:
     set_nder(env_nder())
 
+_lib.dtrt_set_precision.argtypes = [C.c_int]
+_lib.dtrt_scene_precision.argtypes = [C.c_void_p]
+
+
+def set_precision(bits):
+    """Estimator precision (64 = double, the reference's; 32 = float) of scenes created
+    from now on. The default is env DTRT_PRECISION, else 64."""
+    if _lib.dtrt_set_precision(int(bits)) != 0:
+        raise ValueError(_lib.dtrt_last_error().decode())
+
+
+def precision():
+    return int(_lib.dtrt_precision())
+
+
+def scene_precision(scene):
+    return int(_lib.dtrt_scene_precision(scene._h))
+
 # ------------------------------------------------------------ test / bench hooks
 _P = C.c_void_p
 _lib.dtrt_test_kernel_times.argtypes = [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]

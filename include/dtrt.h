@@ -54,6 +54,16 @@ typedef struct dtrt_render_options {
 int dtrt_abi_version(void);
 int dtrt_set_nder(int nder);
 int dtrt_nder(void);
+/* Estimator precision of scenes created from now on, in bits: 64 (double, the
+ * reference's Float, include/fwd.h:10-13; the default) or 32 (float). Like nder the
+ * reference fixes it at compile time; here it is a process setting (env
+ * DTRT_PRECISION, else 64) read when a scene is created. Dual numbers, tracking,
+ * shading and accumulation use it; BVH traversal is fp32 in both, as Embree's is
+ * (src/scene.cpp:17-43), and outputs are float32 as the reference's. */
+int dtrt_set_precision(int bits);
+int dtrt_precision(void);
+/* The precision a scene was created with (32 or 64), -1 for a bad handle. */
+int dtrt_scene_precision(dtrt_object *scene);
 double dtrt_angle_eps(void);
 const char *dtrt_last_error(void);
 void dtrt_destroy(dtrt_object *obj);
