@@ -41,16 +41,31 @@ __constant__ FrameRot DT_FRAME_ROT;
 
 constexpr int BLOCK = TRACE_BLOCK;
 // Minimum resident waves per SIMD requested from the register allocator (HIP's second
-// __launch_bounds__ argument): bounds the VGPRs of the two path kernels.
+// __launch_bounds__ argument): bounds the VGPRs of the two path kernels. Per
+// precision: a double takes two registers.
 #ifndef CHILD_WAVES_PER_SIMD
-#define CHILD_WAVES_PER_SIMD 4
+#define CHILD_WAVES_PER_SIMD (kF64 ? CHILD_WAVES_F64 : 4)
+#endif
+#ifndef CHILD_WAVES_F64
+#define CHILD_WAVES_F64 2   // config[1] at nder 3: 1, 2, 4 waves -> 853, 571, 616 ms per render's child kernel
 #endif
 #ifndef AD_WAVES_PER_SIMD
-#define AD_WAVES_PER_SIMD 2
+#define AD_WAVES_PER_SIMD (kF64 ? AD_WAVES_F64 : 2)
 #endif
-// Wide duals (N >= 8 tangent lanes) take the whole 512-entry register file at one
-// wave per SIMD rather than spilling kilobytes per lane to scratch.
-template <int N> constexpr int adWaves() { return N >= 8 ? 1 : AD_WAVES_PER_SIMD; }
+#ifndef AD_WAVES_F64
+#define AD_WAVES_F64 1      // 1 wave: 498 ms, 2 waves: 553 ms per render's AD kernel (config[1], nder 3)
+#endif
+// Dual widths from which the VolPathTracerAD samples run in k_ad_refill (per-bounce
+// lane refill) at one wave per SIMD, with the whole 512-entry register file, rather
+// than spilling kilobytes per lane to scratch.
+#ifndef AD_REFILL_MIN_N
+#define AD_REFILL_MIN_N (kF64 ? AD_REFILL_MIN_N_F64 : 8)
+#endif
+#ifndef AD_REFILL_MIN_N_F64
+#define AD_REFILL_MIN_N_F64 8
+#endif
+template <int N> constexpr bool adRefill() { return N >= AD_REFILL_MIN_N; }
+template <int N> constexpr int adWaves() { return adRefill<N>() ? 1 : AD_WAVES_PER_SIMD; }
 // Grid of k_ad_refill: waves per CU sharing a launch's samples (measured on the
 // 100K-face dielectric scene at nder 8: 64 beats 32 and 128).
 #ifndef AD_REFILL_WAVES_PER_CU
@@ -1341,7 +1356,7 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
     else if (P.ad_kind == AD_DIRECT)
         hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf,
                            ovf_count, wk);
-    else if constexpr (adWaves<N>() == 1) {
+    else if constexpr (adRefill<N>()) {
         // one wave per block, AD_REFILL_WAVES_PER_CU of them per CU share the samples
         int dev = 0, ncu = 0;
         (void)hipGetDevice(&dev);
