@@ -8,11 +8,15 @@ samples per vertex and the primary-edge pass at 4096 samples per unit length.
 Inputs (scene, BVH, grid) are resident in HBM before timing starts; the image is
 written to device memory.
 
+The estimator runs in double precision, the reference's (include/fwd.h:10-13);
+`fp32_leg` repeats the workload with the float kernel set.
+
 Prints one JSON line (see the keys below). `roofline` is computed for the kernel
 with the largest share of GPU time from its work counters (box tests, triangle
 tests, density lookups, traced segments) and its HIP-event time on the scene's
 stream; `cpu_baseline` times the CPU oracle (a restatement of the reference
-integrator, OpenMP over all host threads) on a centred crop of the same workload.
+integrator, OpenMP over all host threads) on a centred crop of the same workload,
+and `gpu_same_crop` times the GPU on that same crop (`gpu_over_cpu` is their ratio).
 """
 import argparse
 import json
@@ -27,22 +31,24 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["ad_paths", "children", "primary_edges", "finalize"]
 
 
-def algorithmic_bytes(work, nder, spp, npix):
+def algorithmic_bytes(work, nder, spp, npix, scalar_bytes=8):
     """Algorithmic HBM bytes of each kernel from its work counters.
 
     Per event: 32 B per density lookup (8 fp32 voxels), 36 B per triangle test
     (3 x float3), 32 B per box test (one fp32 AABB + link), and per traced segment the
-    carried path state (org, dir, throughput, accumulator: 12 scalars) read + written:
-    96 * P bytes, with P = 1 + nder lanes on AD paths and P = 1 on primal sub-paths.
-    The AD kernel also writes its pixel: 12 * P bytes per pixel.
+    carried path state (org, dir, throughput, accumulator: 12 scalars of the estimator's
+    type, `scalar_bytes` each) read + written: 24 * scalar_bytes * P bytes, with
+    P = 1 + nder lanes on AD paths and P = 1 on primal sub-paths. The AD kernel also
+    writes its pixel (fp32): 12 * P bytes per pixel.
     """
     P = 1 + nder
+    seg_bytes = 24 * scalar_bytes
     out = {}
     for k, base, p in (("ad_paths", 0, P), ("children", 4, 1), ("primary_edges", 8, 1)):
         box, tri, lk, seg = [int(x) for x in work[base:base + 4]]
-        out[k] = 32 * lk + 36 * tri + 32 * box + 96 * p * seg
+        out[k] = 32 * lk + 36 * tri + 32 * box + seg_bytes * p * seg
     out["ad_paths"] += 12 * P * npix
-    out["finalize"] = 4 * 3 * npix * (2 * P + nder)
+    out["finalize"] = 3 * npix * (scalar_bytes * (2 * P - 1) + 4 * P)
     return out
 
 
@@ -66,10 +72,59 @@ def cpu_baseline(desc, target_s=15.0):
     t, n = run(side)
     side = int(min(160, max(8, side * (target_s / max(t, 1e-3)) ** 0.5)))
     t, n = run(side)
-    return {"value": n / t, "unit": "AD path samples/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (fp64 restatement of volpathAD/integratorAD, OpenMP) on a centred %dx%d crop of the "
-                      "same 512x512 scene at 128 spp, depth 10, nder 3, sppse 10, sppe 4096: %d samples in %.1f s"
-                      % (side, side, n, t)}
+    return {"value": n / t, "unit": "AD path samples/s", "cores": threads, "kind": "port", "crop": side,
+            "sample": "CPU oracle (fp64 restatement of volpathAD/integratorAD, binned-SAH BVH, OpenMP) on a centred "
+                      "%dx%d crop of the same 512x512 scene at 128 spp, depth 10, nder 3, sppse 10, sppe 4096: "
+                      "%d samples in %.1f s" % (side, side, n, t)}
+
+
+def gpu_crop_rate(desc, side, reps=5):
+    """The GPU on the CPU baseline's own centred crop (same spp, depth, nder, edge
+    sampling), at the headline precision: AD path samples/s over `reps` renders."""
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    d = scenes.config1(nder=desc.nder, res=desc.width, spp=desc.spp,
+                       crop=((desc.width - side) // 2, (desc.height - side) // 2, side, side))
+    sc = scenes.instantiate(G, d)
+    out = torch.zeros(d.out_shape(), dtype=torch.float32, device="cuda")
+    opts = G.RenderOptions(0, d.spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
+    G.render_device(0, sc, opts, out.data_ptr())
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(reps):
+        opts.seed = 1 + k
+        G.render_device(0, sc, opts, out.data_ptr())
+    torch.cuda.synchronize()
+    return side * side * d.spp * reps / (time.perf_counter() - t)
+
+
+def precision_leg(desc, bits, steps):
+    """The headline workload with the other estimator precision's kernel set."""
+    import torch
+    import dtrt_amd.dtrt as G
+    from dtrt_amd import scenes
+    old = G.precision()
+    G.set_precision(bits)
+    try:
+        sc = scenes.instantiate(G, desc)
+    finally:
+        G.set_precision(old)
+    out = torch.zeros(desc.out_shape(), dtype=torch.float32, device="cuda")
+    opts = G.RenderOptions(0, desc.spp, desc.max_bounces, desc.sppe, desc.sppse, True, -1, desc.ddist)
+    G.render_device(0, sc, opts, out.data_ptr())
+    torch.cuda.synchronize()
+    ms = 0
+    t = time.perf_counter()
+    for k in range(steps):
+        out.zero_()
+        opts.seed = k
+        G.render_device(0, sc, opts, out.data_ptr())
+        ms = ms + G.kernel_times(sc)[0]
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t) / steps
+    return {"dtype": "f%d" % bits, "value": desc.width * desc.height * desc.spp / el, "unit": "AD path samples/s",
+            "ms_per_step": el * 1e3, "kernel_ms_per_step": {k: float(ms[i]) / steps for i, k in enumerate(KERNELS)}}
 
 
 def adjoint_leg(spp=64, res=512, grid_res=128):
@@ -346,6 +401,7 @@ def main():
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--spp", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the float-precision leg of the headline workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward gradient timing")
     ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
@@ -375,6 +431,7 @@ def main():
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
 
+    G.set_precision(64)   # the reference's Float (include/fwd.h:10-13); every leg below runs at it
     desc = scenes.config1(nder=3, res=args.res, spp=args.spp)
     scene = scenes.instantiate(G, desc)
     out = torch.zeros(desc.out_shape(), dtype=torch.float32, device="cuda")
@@ -409,21 +466,23 @@ def main():
     npix = desc.width * desc.height
     samples = npix * desc.spp * args.steps
     value = samples / elapsed
-    alg = algorithmic_bytes(work, desc.nder, desc.spp, npix * args.steps)
+    alg = algorithmic_bytes(work, desc.nder, desc.spp, npix * args.steps, scalar_bytes=8)
     dom = int(np.argmax(ms))
     name = KERNELS[dom]
     achieved = alg[name] / (ms[dom] * 1e-3) / 1e9
-    # HBM bytes per launch: the FETCH_SIZE / WRITE_SIZE profile of this workload gives the
-    # kernel's (copy-calibrated) HBM bytes per algorithmic byte over its timed launches;
-    # times this run's algorithmic bytes per launch. Null when the workload differs.
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "ad_render_heterogeneous_512x512_128spp_nder3_hbm_traffic.json")
+    # HBM bytes per launch of the dominant kernel, as measured: the copy-calibrated
+    # rocprofv3 FETCH_SIZE and WRITE_SIZE counters of this command (bench.py, this
+    # workload and precision) averaged over the kernel's launches of the profiled run's
+    # timed render. Null when no profile of this workload, precision and kernel exists.
+    traffic, traffic_src = None, None
+    tfile = os.path.join(ROOT, "profiles", "ad_render_heterogeneous_512x512_128spp_nder3_f64_hbm_traffic.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        entry = t.get("%s<%d>" % (name, desc.nder))
-        ratio = entry.get("hbm_per_algorithmic_byte") if isinstance(entry, dict) else None
-        if (t["width"], t["height"], t["spp"], t["nder"]) == (desc.width, desc.height, desc.spp, desc.nder) and ratio:
-            traffic = ratio * alg[name] / max(1, launches[dom])
+        entry = t.get("kernels", {}).get("%s<%d>" % (name, desc.nder))
+        if (t["width"], t["height"], t["spp"], t["nder"], t["precision"]) == (desc.width, desc.height, desc.spp,
+                                                                           desc.nder, 64) and entry:
+            traffic = entry["hbm_bytes_per_launch"]
+            traffic_src = "measured: %s (%d launches, %s)" % (os.path.basename(tfile), entry["launches"], t["command"])
     rec = {
         "metric": "AD path samples/s at 1 GPU; adjoint vs forward grad time at equal variance",
         "value": value,
@@ -435,14 +494,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic (seeded fBm density grid, generated meshes)",
         "config": {"workload": "heterogeneous-medium AD render: null cube + %d^3 fBm grid, HG 0.3, quad light + "
                                "emissive env sphere" % 64, "width": desc.width, "height": desc.height,
                    "spp": desc.spp, "max_bounces": desc.max_bounces, "nder": desc.nder, "sppse": desc.sppse,
                    "sppe": desc.sppe, "primary_edge_samples_per_step": int(st["pe_samples"])},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "avg_launch_ms": ms[dom] / max(1, launches[dom]), "launches_timed": int(launches[dom]),
                      "algorithmic_bytes_per_launch": alg[name] / max(1, launches[dom])},
         "launches_per_step": {k: int(launches[i]) // args.steps for i, k in enumerate(KERNELS)},
@@ -454,6 +513,8 @@ def main():
         "autotune": tune,
     }
     del scene
+    if not args.no_fp32:
+        rec["fp32_leg"] = precision_leg(desc, 32, args.steps)
     if not args.no_views:
         rec["views"] = views_leg()
     if not args.no_adjoint:
@@ -468,7 +529,11 @@ def main():
         rec["inverse_step"] = inverse_leg(args.inverse_spp_divisor)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(desc)
-        rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
+        crop = rec["cpu_baseline"]["crop"]
+        rec["gpu_same_crop"] = {"crop": crop, "value": gpu_crop_rate(desc, crop), "unit": "AD path samples/s",
+                                "dtype": "f64"}
+        rec["gpu_over_cpu"] = rec["gpu_same_crop"]["value"] / rec["cpu_baseline"]["value"]
+        rec["gpu_full_frame_over_cpu_crop"] = value / rec["cpu_baseline"]["value"]
     print(json.dumps(rec), flush=True)
 
 

@@ -35,10 +35,12 @@ def test_cfg2_full_mesh_parity(gpu, oracle, cfg2, mode):
     d = cfg2
     assert sum(len(s.indices) for s in d.shapes) >= 100820 and d.media[0].density.shape == (128, 128, 128)
     assert d.max_bounces == 12 and d.nder == 8
-    a = renders(gpu, d, range(16), mode=mode, sppe=256)
-    b = renders(oracle, d, range(1000, 1012), mode=mode, sppe=1024, spp=4 * d.spp)
+    # the primary-edge pass alone is sparse (silhouette pixels only): more samples, 6x6 blocks
+    sppe, block = (2048, 6) if mode == 0 else (256, 4)
+    a = renders(gpu, d, range(16), mode=mode, sppe=sppe)
+    b = renders(oracle, d, range(1000, 1012), mode=mode, sppe=4 * sppe, spp=4 * d.spp)
     assert np.abs(b[:, 1:].mean(0)).max() > 0   # the term has tangents in the crop
-    z = block_z(a, b, 4)
+    z = block_z(a, b, block)
     assert_3sigma(z, what="cfg2 full mesh, mode %d" % mode, max_mean_z2=1.8)
 
 
