@@ -42,6 +42,7 @@ def _jobs():
     for ns, real in PRECISIONS:
         jobs.append((kern, "kernels_" + ns, ["-DDT_REAL=" + real, "-DDT_PNS=" + ns]))
     jobs.append((os.path.join(SRC, "dispatch.cpp"), "dispatch", []))
+    jobs.append((os.path.join(SRC, "scratch.hip"), "scratch", []))
     jobs.append((os.path.join(SRC, "host.cpp"), "host", []))
     return jobs
 

@@ -616,6 +616,24 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Bump allocation inside a caller-provided device area (the scratch-only render):
+// 256-byte aligned pieces, null when the area is used up.
+struct Bump {
+    char *p = nullptr;
+    uint64_t left = 0;
+    void *take(uint64_t n) {
+        n = (n + 255) / 256 * 256;
+        if (n > left) return nullptr;
+        void *r = p;
+        p += n;
+        left -= n;
+        return r;
+    }
+};
+// The scratch-only render's child queue must hold at least this much; below it the
+// render is refused (the exact overflow replay would still finish, in many rounds).
+constexpr uint64_t NOALLOC_MIN_QUEUE = uint64_t(16) << 20;
+
 struct Scene : Obj {
     Scene() : Obj(T_SCENE) {}
     ~Scene() override {
@@ -710,15 +728,28 @@ struct Scene : Obj {
         return renderViews(1, &c, &o, outs, out_is_device, ad_kind);
     }
     bool renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs, bool on_device,
-                     int ad_kind = AD_VOLPATH);
+                     int ad_kind = AD_VOLPATH, Bump *pool = nullptr);
     bool primaryEdges(const Camera &c, PECache &out);
+    void projectPrimaryEdges(const Camera &cam, std::vector<double> &cdf, std::vector<float> &packed, int &npe,
+                             double &pe_sum) const;
+    // dtrt_render_device_noalloc: one view into a device image, every device buffer of
+    // the render taken from the library's scratch area; no device allocation.
+    bool renderNoAlloc(const dtrt_render_options &o, float *out_device, int ad_kind) {
+        void *base = nullptr;
+        uint64_t bytes = 0;
+        HIPCHK(scratchArea(&base, &bytes));
+        Bump pool{static_cast<char *>(base), bytes};
+        const Camera *c = &cam;
+        float *outs[1] = {out_device};
+        return renderViews(1, &c, &o, outs, true, ad_kind, &pool);
+    }
     bool captureViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *images);
     int replayViews(const uint64_t *seeds);
     std::vector<void *> graphBuffers() const {
         return {viewbuf.p, acc.p, acc_pe.p, queue.p, qcount.p, stats.p, work.p, ovf[0].p, ovf[1].p};
     }
     bool setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
-                    std::vector<uint64_t> &pe_total);
+                    std::vector<uint64_t> &pe_total, Bump *pool = nullptr);
     bool renderAdjoint(const dtrt_render_options &o, int med, const float *dimg, float *grad) {
         const Camera *c = &cam;
         return renderAdjointViews(1, &c, &o, med, dimg, grad);
@@ -766,6 +797,7 @@ bool frameRotationReady() {
 bool Scene::build() {
     if (!frameRotationReady()) return false;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&caller_ev, hipEventDisableTiming));
     std::memset(&S, 0, sizeof(S));
     S.nder = nder;
     S.f64 = precisionNow() == 64;
@@ -915,6 +947,22 @@ bool Scene::initEdges(const float *weights) {
 bool Scene::primaryEdges(const Camera &cam, PECache &out) {
     auto it = pecache.find(cam.uid);
     if (it != pecache.end()) { out = it->second; return true; }
+    PECache c;
+    std::vector<double> cdf;
+    std::vector<float> packed;
+    projectPrimaryEdges(cam, cdf, packed, c.npe, c.pe_sum);
+    c.cdf = upload(cdf);
+    c.pe = packed.empty() ? nullptr : upload(packed);
+    if (!c.cdf || (!packed.empty() && !c.pe)) return fail("scene: device allocation failed");
+    pecache[cam.uid] = c;
+    out = c;
+    return true;
+}
+
+// The primary edges of `cam` on the host: the sampling CDF (npe + 1 entries) and the
+// packed records [npe][16 + 4 nder] the kernels read.
+void Scene::projectPrimaryEdges(const Camera &cam, std::vector<double> &cdf, std::vector<float> &packed, int &npe,
+                                double &pe_sum) const {
     std::vector<PEdge> pe;
     std::vector<double> pw;
     for (const Shape *sh : shapes) {
@@ -928,14 +976,13 @@ bool Scene::primaryEdges(const Camera &cam, PECache &out) {
             projectEdge(cam, a, b, nder, pe, pw);
         }
     }
-    PECache c;
-    c.npe = (int)pe.size();
-    std::vector<double> cdf(pe.size() + 1, 0.0);
+    npe = (int)pe.size();
+    cdf.assign(pe.size() + 1, 0.0);
     for (size_t i = 0; i < pw.size(); ++i) cdf[i + 1] = cdf[i] + pw[i];
-    c.pe_sum = cdf.back();
-    if (c.pe_sum > 0) { for (double &x : cdf) x /= c.pe_sum; cdf.back() = 1.0; }
+    pe_sum = cdf.back();
+    if (pe_sum > 0) { for (double &x : cdf) x /= pe_sum; cdf.back() = 1.0; }
     const int stride = 16 + 4 * nder;
-    std::vector<float> packed(pe.size() * stride, 0.f);
+    packed.assign(pe.size() * stride, 0.f);
     for (size_t i = 0; i < pe.size(); ++i) {
         float *p = &packed[i * stride];
         const PEdge &e = pe[i];
@@ -947,19 +994,13 @@ bool Scene::primaryEdges(const Camera &cam, PECache &out) {
             p[18 + 4 * j] = (float)e.s1.dx[j]; p[19 + 4 * j] = (float)e.s1.dy[j];
         }
     }
-    c.cdf = upload(cdf);
-    c.pe = packed.empty() ? nullptr : upload(packed);
-    if (!c.cdf || (!packed.empty() && !c.pe)) return fail("scene: device allocation failed");
-    pecache[cam.uid] = c;
-    out = c;
-    return true;
 }
 
 constexpr uint64_t MAX_CHUNK = uint64_t(1) << 22;    // AD samples per launch
 
 // Per-view device records: camera, sample and pixel offsets, seed, primary edges.
 bool Scene::setupViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, std::vector<DView> &hv,
-                       std::vector<uint64_t> &pe_total) {
+                       std::vector<uint64_t> &pe_total, Bump *pool) {
     hv.assign(nv, DView{});
     pe_total.assign(nv, 0);
     uint64_t s0 = 0;
@@ -979,8 +1020,20 @@ bool Scene::setupViews(int nv, const Camera *const *cams, const dtrt_render_opti
         d.spp = o.spp;
         d.sppe = o.sppe;
         PECache pe;
-        if (o.sppe > 0) {
+        if (o.sppe > 0 && pool && !pecache.count(c.uid)) {   // project into the pool, do not cache
+            std::vector<double> cdf;
+            std::vector<float> packed;
+            projectPrimaryEdges(c, cdf, packed, pe.npe, pe.pe_sum);
+            void *a = pool->take(cdf.size() * sizeof(double)), *b = pool->take(packed.size() * sizeof(float));
+            if (!a || !b) return fail("render: the primary edges do not fit the scratch area");
+            HIPCHK(hipMemcpy(a, cdf.data(), cdf.size() * sizeof(double), hipMemcpyHostToDevice));
+            if (!packed.empty()) HIPCHK(hipMemcpy(b, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice));
+            pe.cdf = static_cast<const double *>(a);
+            pe.pe = packed.empty() ? nullptr : static_cast<const float *>(b);
+        } else if (o.sppe > 0) {
             if (!primaryEdges(c, pe)) return false;
+        }
+        if (o.sppe > 0) {
             d.npe = pe.npe;
             d.pe_sum = pe.pe_sum;
             d.pe_cdf = pe.cdf;
@@ -991,9 +1044,16 @@ bool Scene::setupViews(int nv, const Camera *const *cams, const dtrt_render_opti
         pix0 += d.npix;
         if (pix0 > 0x7FFFFFFF / 4) return fail("render: too many pixels in one call");
     }
-    if (!ensure(viewbuf, (size_t)nv * sizeof(DView))) return false;
-    HIPCHK(hipMemcpyAsync(viewbuf.p, hv.data(), (size_t)nv * sizeof(DView), hipMemcpyHostToDevice, stream));
-    S.views = static_cast<const DView *>(viewbuf.p);
+    void *vt = nullptr;
+    if (pool) {
+        vt = pool->take((uint64_t)nv * sizeof(DView));
+        if (!vt) return fail("render: the view table does not fit the scratch area");
+    } else {
+        if (!ensure(viewbuf, (size_t)nv * sizeof(DView))) return false;
+        vt = viewbuf.p;
+    }
+    HIPCHK(hipMemcpyAsync(vt, hv.data(), (size_t)nv * sizeof(DView), hipMemcpyHostToDevice, stream));
+    S.views = static_cast<const DView *>(vt);
     S.nviews = nv;
     return true;
 }
@@ -1005,8 +1065,9 @@ constexpr uint64_t FIRST_CHUNK = uint64_t(1) << 16;  // before the children-per-
 // pixels occupy their own range of the accumulators. Every view's image equals what
 // a render of that view alone gives (same streams), up to float summation order.
 bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, float *const *outs,
-                        bool out_is_device, int ad_kind) {
+                        bool out_is_device, int ad_kind, Bump *pool) {
     if (nv <= 0) return fail("render: no views");
+    if (pool && !out_is_device) return fail("render: the scratch-only render writes device images only");
     if (!afterCaller()) return false;
     const dtrt_render_options &o0 = opts[0];
     for (int v = 1; v < nv; ++v)
@@ -1021,7 +1082,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
-    if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
+    if (!setupViews(nv, cams, opts, hv, pe_total, pool)) return false;
     int64_t npix_all = 0;
     uint64_t total = 0, pe_all = 0;
     for (int v = 0; v < nv; ++v) {
@@ -1035,19 +1096,54 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     // sized from the children-per-sample rate seen so far; a sample whose pushes
     // overflow is replayed exactly (see k_ad_paths). A primary-edge sample pushes at
     // most 2 records, so its chunks never overflow.
-    const uint64_t cap = std::max<uint64_t>(128, std::min<uint64_t>(queue_budget / P.rec_stride, 0xFFFFFFF0ull));
-    const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
-    const uint64_t max_chunk = std::min<uint64_t>(tune_max_chunk, std::max<uint64_t>(total, 64));
-    P.qcap = (unsigned)cap;
     const size_t es = accBytes();
-    if (!ensure(acc, img * es) || !ensure(acc_pe, (size_t)nder * npix * 3 * es) || !ensure(qcount, 64) ||
-        !ensure(stats, 64 * 4) || !ensure(work, 16 * 8) || !ensure(queue, (size_t)cap * P.rec_stride) ||
-        !ensure(ovf[0], (size_t)max_chunk * 16) || !ensure(ovf[1], (size_t)max_chunk * 16))
-        return false;
-    HIPCHK(hipMemsetAsync(acc.p, 0, img * es, stream));
-    HIPCHK(hipMemsetAsync(acc_pe.p, 0, (size_t)nder * npix * 3 * es, stream));
-    HIPCHK(hipMemsetAsync(stats.p, 0, 64 * 4, stream));
-    HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
+    void *A = nullptr, *APE = nullptr;
+    char *Q = nullptr;
+    unsigned *qc = nullptr;   // [0] queued records, [1] overflowed samples, [2] consumed records
+    unsigned *st = nullptr;   // [0..STAT_COUNT) counters, [32..) discarded (replays)
+    unsigned long long *wk = nullptr;
+    uint4 *OV[2] = {nullptr, nullptr};
+    uint64_t cap = 0, max_chunk = 0;
+    if (pool) {   // everything from the scratch area: fixed parts first, the queue gets the rest
+        max_chunk = std::min<uint64_t>(std::min<uint64_t>(tune_max_chunk, uint64_t(1) << 18), std::max<uint64_t>(total, 64));
+        A = pool->take(img * es);
+        APE = pool->take((uint64_t)nder * npix * 3 * es);
+        qc = static_cast<unsigned *>(pool->take(64));
+        st = static_cast<unsigned *>(pool->take(64 * 4));
+        wk = static_cast<unsigned long long *>(pool->take(16 * 8));
+        OV[0] = static_cast<uint4 *>(pool->take(max_chunk * 16));
+        OV[1] = static_cast<uint4 *>(pool->take(max_chunk * 16));
+        const uint64_t qbytes = pool->left / 256 * 256;
+        if (!A || !APE || !qc || !st || !wk || !OV[0] || !OV[1] || qbytes < NOALLOC_MIN_QUEUE)
+            return fail("render_device_noalloc: the image needs more than the " +
+                        std::to_string(DTRT_SCRATCH_BYTES >> 20) + " MiB scratch area holds (accumulators " +
+                        std::to_string((img + (size_t)nder * npix * 3) * es >> 20) + " MiB, plus a queue of at least " +
+                        std::to_string(NOALLOC_MIN_QUEUE >> 20) + " MiB)");
+        Q = static_cast<char *>(pool->take(qbytes));
+        cap = std::min<uint64_t>(qbytes / P.rec_stride, 0xFFFFFFF0ull);
+    } else {
+        cap = std::max<uint64_t>(128, std::min<uint64_t>(queue_budget / P.rec_stride, 0xFFFFFFF0ull));
+        max_chunk = std::min<uint64_t>(tune_max_chunk, std::max<uint64_t>(total, 64));
+        if (!ensure(acc, img * es) || !ensure(acc_pe, (size_t)nder * npix * 3 * es) || !ensure(qcount, 64) ||
+            !ensure(stats, 64 * 4) || !ensure(work, 16 * 8) || !ensure(queue, (size_t)cap * P.rec_stride) ||
+            !ensure(ovf[0], (size_t)max_chunk * 16) || !ensure(ovf[1], (size_t)max_chunk * 16))
+            return false;
+        A = acc.p;
+        APE = acc_pe.p;
+        Q = static_cast<char *>(queue.p);
+        qc = static_cast<unsigned *>(qcount.p);
+        st = static_cast<unsigned *>(stats.p);
+        wk = static_cast<unsigned long long *>(work.p);
+        OV[0] = static_cast<uint4 *>(ovf[0].p);
+        OV[1] = static_cast<uint4 *>(ovf[1].p);
+    }
+    const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
+    P.qcap = (unsigned)cap;
+    // clears by fill kernels (the scratch area keeps the last call's contents)
+    HIPCHK(launch_fill32(A, img * es / 4, 0u, stream));
+    HIPCHK(launch_fill32(APE, (size_t)nder * npix * 3 * es / 4, 0u, stream));
+    HIPCHK(launch_fill32(st, 64, 0u, stream));
+    HIPCHK(launch_fill32(wk, 32, 0u, stream));
     // per-view output images in device memory (staged through outbuf for host outputs)
     std::vector<float *> dout(nv);
     if (out_is_device) {
@@ -1062,7 +1158,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
             off += n;
         }
     }
-    if (ev.empty()) {
+    if (!pool && ev.empty()) {
         ev.resize(2);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     }
@@ -1072,7 +1168,12 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     const unsigned childGrid = (unsigned)std::max(1, ncu) * (unsigned)tune_child_blocks;
     std::memset(kernel_ms, 0, sizeof(kernel_ms));
     std::memset(kernel_launches, 0, sizeof(kernel_launches));
-    auto timed = [&](int slot, auto &&fn) -> bool {
+    auto timed = [&](int slot, auto &&fn) -> bool {   // per-kernel HIP-event times (not in the scratch-only render)
+        if (pool) {
+            HIPCHK(fn());
+            kernel_launches[slot]++;
+            return true;
+        }
         HIPCHK(hipEventRecord(ev[0], stream));
         HIPCHK(fn());
         HIPCHK(hipEventRecord(ev[1], stream));
@@ -1083,12 +1184,6 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
         kernel_launches[slot]++;
         return true;
     };
-    unsigned *qc = static_cast<unsigned *>(qcount.p);   // [0] queued records, [1] overflowed samples, [2] consumed records
-    unsigned *st = static_cast<unsigned *>(stats.p);    // [0..STAT_COUNT) counters, [32..) discarded (replays)
-    unsigned long long *wk = static_cast<unsigned long long *>(work.p);
-    void *A = acc.p, *APE = acc_pe.p;
-    char *Q = static_cast<char *>(queue.p);
-    uint4 *OV[2] = {static_cast<uint4 *>(ovf[0].p), static_cast<uint4 *>(ovf[1].p)};
     uint64_t children = 0, replayed = 0;
     auto counts = [&](unsigned h[2]) -> bool {
         HIPCHK(hipMemcpyAsync(h, qc, 8, hipMemcpyDeviceToHost, stream));
@@ -1102,7 +1197,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
         last_chunks.push_back(n);
         uint64_t delivered = 0;
         unsigned h[2];
-        HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
+        HIPCHK(launch_fill32(qc, 3, 0u, stream));
         if (!timed(0, [&] { return launch_ad_paths(S, P, s0, n, nullptr, A, Q, qc, st, OV[0], qc + 1, wk, stream); })) return false;
         if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
         if (!counts(h)) return false;
@@ -1110,7 +1205,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
         int in = 0;
         for (unsigned nov = h[1]; nov > 0; in ^= 1) {   // each round delivers >= 1 record: terminates
             replayed += nov;
-            HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
+            HIPCHK(launch_fill32(qc, 3, 0u, stream));
             if (!timed(0, [&] { return launch_ad_paths(S, P, 0, nov, OV[in], A, Q, qc, st + 32, OV[in ^ 1], qc + 1, wk, stream); }))
                 return false;
             if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
@@ -1128,7 +1223,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     for (int v = 0; v < nv; ++v)
         for (uint64_t s0 = 0; s0 < pe_total[v]; s0 += pe_chunk) {
             uint64_t n = std::min(pe_chunk, pe_total[v] - s0);
-            HIPCHK(hipMemsetAsync(qc, 0, 12, stream));
+            HIPCHK(launch_fill32(qc, 3, 0u, stream));
             if (!timed(2, [&] { return launch_primary(S, P, v, s0, n, pe_total[v], Q, qc, st, wk, stream); })) return false;
             if (!timed(1, [&] { return launch_children(S, P, Q, qc, A, APE, wk, childGrid, stream); })) return false;
             unsigned h[2];
@@ -1925,6 +2020,18 @@ int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *o
 }
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *image) {
     return renderCommon(integrator, scene, o, image, true);
+}
+int dtrt_render_device_noalloc(int integrator, dtrt_object *scene, const dtrt_render_options *o, float *image) {
+    if (integrator != 0 && integrator != 6 && integrator != 7) {
+        g_err = "render_device_noalloc: forward-mode integrators only: VolPathTracerAD (0), PathTracerAD (6), DirectAD (7)";
+        return -2;
+    }
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !o || !image) { if (s) g_err = "render_device_noalloc: null argument"; return -1; }
+    return guarded<int>(-1, [&]() -> int {
+        const int ad = integrator == 6 ? AD_PATH : integrator == 7 ? AD_DIRECT : AD_VOLPATH;
+        return s->renderNoAlloc(*o, image, ad) ? 0 : -1;
+    });
 }
 int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
     return guarded<int>(-1, [&]() -> int {

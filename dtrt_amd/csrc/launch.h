@@ -90,6 +90,11 @@ constexpr double PRIMARY_DELTA = 1e-4;
 
 enum PrimalKind { PRIMAL_VOLPATH = 1, PRIMAL_SIMPLE = 2, PRIMAL_PATH = 3 };
 
+// The allocation-free render's device scratch (scratch.hip): bytes per GPU, and the
+// device address of the current device's copy (looked up once per device).
+#define DTRT_SCRATCH_BYTES (uint64_t(256) << 20)
+hipError_t scratchArea(void **ptr, uint64_t *bytes);
+
 namespace p32 {
 DT_LAUNCH_API
 }

@@ -210,6 +210,20 @@ int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dt
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
                         const float *d_image_device, float *d_density_device);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same as dtrt_render_device for one view (integrator 0, 6 or 7), but
+ * every device buffer of the render (accumulators, child queue, overflow lists, view
+ * table, primary edges not yet cached for the camera) comes from one fixed scratch
+ * area of 256 MiB per GPU that the library holds from load until the process ends;
+ * the call makes no device allocation. Renders whose accumulators leave less than a
+ * 16 MiB child queue in that area are refused with an error (at nder 3 in double,
+ * 168 bytes of accumulators per pixel: above about 1.4 megapixels). The area keeps the last call's contents and is cleared by
+ * each call; calls that overlap on one GPU (several threads or scenes at once) race
+ * on it. Kernel times are not recorded (dtrt_test_kernel_times reads zeros). */
+int dtrt_render_device_noalloc(int integrator, dtrt_object *scene, const dtrt_render_options *options,
+                               float *image_device);
+
 /* No pybind counterpart (the reference's render reads host memory only). Names the
  * caller's HIP stream (hipStream_t; NULL = the legacy null stream, the default). Every
  * later call on `scene` that reads or writes caller device memory (dtrt_render_device,
