@@ -404,7 +404,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the float-precision leg of the headline workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-vs-forward gradient timing")
     ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view timing")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch settings at the scene's first use (the default keeps the built-in ones, "
+                         "which the profiles under profiles/ were taken with)")
     ap.add_argument("--no-primal", action="store_true", help="skip the primal-integrator timing")
     ap.add_argument("--no-examples", action="store_true", help="skip the glass_1 optimisation-step timing")
     ap.add_argument("--no-configs", action="store_true", help="skip the surface / dielectric scene timings")
@@ -443,7 +445,7 @@ def main():
         G.render_device(0, scene, opts, out.data_ptr())
 
     tune = None
-    if not args.no_autotune:   # the scene's first use: pick launch settings, then time with them
+    if args.autotune:   # the scene's first use: pick launch settings, then time with them
         tms, tset = G.autotune(scene, G.RenderOptions(999, desc.spp, desc.max_bounces, desc.sppe, desc.sppse, True, -1,
                                                       desc.ddist))
         tune = dict(tset, best_render_ms=tms)
@@ -479,8 +481,9 @@ def main():
     if os.path.exists(tfile):
         t = json.load(open(tfile))
         entry = t.get("kernels", {}).get("%s<%d>" % (name, desc.nder))
-        if (t["width"], t["height"], t["spp"], t["nder"], t["precision"]) == (desc.width, desc.height, desc.spp,
-                                                                           desc.nder, 64) and entry:
+        same = (t["width"], t["height"], t["spp"], t["nder"], t["precision"]) == (desc.width, desc.height, desc.spp,
+                                                                                 desc.nder, 64)
+        if same and entry and not args.autotune and entry["launches"] * args.steps == int(launches[dom]):
             traffic = entry["hbm_bytes_per_launch"]
             traffic_src = "measured: %s (%d launches, %s)" % (os.path.basename(tfile), entry["launches"], t["command"])
     rec = {

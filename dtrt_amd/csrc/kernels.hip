@@ -496,8 +496,9 @@ DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &
     const DCamera &cam = vw.cam;
     Dv<N> ro, rd;
     adCameraRay<N>(vw, r, pixel, ro, rd);
-    HitAD<N> its;
-    if (T.rayIntersectAD(ro, rd, false, its)) {
+    real tcam;
+    int hface = T.trace(ro.val(), rd.val(), 0._r, FINF, tcam);
+    if (hface >= 0) {
         int med = cam.med_id, depth = P.max_bounces, mode = P.mode;
         Ds<N> beta(f3{1, 1, 1});
         while (mode != 0) {
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
 // State of one AD path between bounces (the locals of adPathSample's loop).
 template <int N> struct AdState {
     Dv<N> ro, rd;
-    HitAD<N> its;
+    int hface;   // face the ray (ro, rd) hits; the bounce recomputes the hit record from it
     Ds<N> beta;
     int med, depth, mode;
 };
@@ -582,7 +583,9 @@ template <int N> struct AdState {
 template <int N>
 DI bool adBegin(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, AdState<N> &st) {
     adCameraRay<N>(vw, r, pixel, st.ro, st.rd);
-    if (!T.rayIntersectAD(st.ro, st.rd, false, st.its)) return false;
+    real tcam;
+    st.hface = T.trace(st.ro.val(), st.rd.val(), 0._r, FINF, tcam);
+    if (st.hface < 0) return false;
     st.med = vw.cam.med_id;
     st.depth = P.max_bounces;
     st.mode = P.mode;
@@ -598,7 +601,7 @@ DI bool adBounce(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, AdState
     const AdjCtx *ac = nullptr;
     const DScene &S = T.S;
     Dv<N> &ro = st.ro, &rd = st.rd;
-    HitAD<N> &its = st.its;
+    int &hface = st.hface;
     Ds<N> &beta = st.beta;
     int &med = st.med, &depth = st.depth, &mode = st.mode;
 #define AD_BOUNCE_END return false

@@ -11,10 +11,11 @@ namespace DT_PNS {
 
 DI real misPower(real a, real b) { a *= a; b *= b; return a / (a + b); }   // volpathAD.cpp:43-47
 
-// State of one primal path between bounces (see Tracer::liStep).
+// State of one primal path between bounces (see Tracer::liStep). The hit record of the
+// current ray is not carried: each step recomputes it from the face the ray hits.
 struct LiPath {
     f3 ro, rd, thr, ret;
-    Hit its;
+    int face;   // face the ray (ro, rd) hits, -1: none
     int depth, max_depth, med;
     bool incEmission;
 };
@@ -294,7 +295,13 @@ template <int N> struct Tracer {
         p.max_depth = max_depth;
         p.med = med;
         p.incEmission = incEmission;
-        p.its = first;
+        p.face = first.face;
+    }
+    // The hit record of the path's current ray (shape.cpp:166-208 on the recorded face).
+    DI Hit hitOf(const LiPath &p) const {
+        Hit h;
+        if (p.face >= 0) refineHit(S, p.face, p.ro, p.rd, h);
+        return h;
     }
     DI void liStart(LiPath &p, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {
         Hit h;
@@ -305,13 +312,14 @@ template <int N> struct Tracer {
     DI bool liStep(LiPath &p, Rng &r) {
         if (p.depth > p.max_depth) return false;
         const int max_inter = p.max_depth - p.depth - 1;
+        const Hit its = hitOf(p);
         real r1a = r.next(), r1b = r.next();
         real r2[4] = {r.next(), r.next(), r.next(), r.next()};
         real r3[3] = {r.next(), r.next(), r.next()};
         bool inside = false;
         if (p.med >= 0) {
             f3 ps;
-            inside = sampleDistance(S.media[p.med], p.ro, p.rd, p.its.t, r1a, r1b, r, ps, p.thr);
+            inside = sampleDistance(S.media[p.med], p.ro, p.rd, its.t, r1a, r1b, r, ps, p.thr);
             if (inside) p.ro = ps;
         }
         if (inside) {
@@ -327,35 +335,41 @@ template <int N> struct Tracer {
             }
             wo = phaseSample(ph, -p.rd, r3[0], r3[1]);
             p.rd = wo;
-            f3 att = lookForEmitter(p.ro, p.rd, false, r, p.med, max_inter, p.its, pdf_nee);
+            Hit nx;
+            f3 att = lookForEmitter(p.ro, p.rd, false, r, p.med, max_inter, nx, pdf_nee);
+            p.face = nx.face;
             if (!allZero(att, EPS)) p.ret = p.ret + p.thr * att * misPower(phaseEval(ph, -p.rd, wo), pdf_nee);
             p.incEmission = false;
         } else {
-            if (p.its.face < 0) return false;
-            const DShape &sh = shapeOf(S, p.its.face);
+            if (its.face < 0) return false;
+            const DShape &sh = shapeOf(S, its.face);
             if (sh.light_id >= 0 && p.incEmission)
-                p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], p.its.gn, p.its.sn, -p.rd);
+                p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], its.gn, its.sn, -p.rd);
             if (p.depth >= p.max_depth) return false;
             const DBsdf &b = bsdfOf(S, sh);
-            Frame shf(p.its.sn);
+            Frame shf(its.sn);
             f3 wo;
             real pdf_nee;
             if (b.type != BSDF_NULL) {
-                f3 value = neeSurface(p.its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
+                f3 value = neeSurface(its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
                 if (!allZero(value, EPS))
-                    p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, p.its.wi, wo));
+                    p.ret = p.ret + p.thr * value * bsdfEval(b, its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, its.wi, wo));
             }
             real bpdf = 0._r;
-            f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+            f3 bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
             if (allZero(bw, EPS)) return false;
             wo = shf.toWorld(wo);
-            p.ro = p.its.p;
+            p.ro = its.p;
             p.rd = wo;
-            if (isTransition(sh)) p.med = targetMedium(sh, p.its.gn, wo);
+            if (isTransition(sh)) p.med = targetMedium(sh, its.gn, wo);
             if (b.type == BSDF_NULL) {
-                rayIntersect(p.ro, p.rd, true, p.its);
+                Hit nx;
+                rayIntersect(p.ro, p.rd, true, nx);
+                p.face = nx.face;
             } else {
-                f3 att = lookForEmitter(p.ro, p.rd, true, r, p.med, max_inter, p.its, pdf_nee);
+                Hit nx;
+                f3 att = lookForEmitter(p.ro, p.rd, true, r, p.med, max_inter, nx, pdf_nee);
+                p.face = nx.face;
                 p.thr = p.thr * bw;
                 if (!allZero(att, 1e-12_r)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
                 p.incEmission = false;
@@ -371,13 +385,14 @@ template <int N> struct Tracer {
     DI bool liSimpleStep(LiPath &p, Rng &r) {
         if (p.depth > p.max_depth) return false;
         const int max_inter = p.max_depth - p.depth - 1;
+        const Hit its = hitOf(p);
         real r1a = r.next(), r1b = r.next();
         real r2[4] = {r.next(), r.next(), r.next(), r.next()};
         real r3[3] = {r.next(), r.next(), r.next()};
         bool inside = false;
         if (p.med >= 0) {
             f3 ps;
-            inside = sampleDistance(S.media[p.med], p.ro, p.rd, p.its.t, r1a, r1b, r, ps, p.thr);
+            inside = sampleDistance(S.media[p.med], p.ro, p.rd, its.t, r1a, r1b, r, ps, p.thr);
             if (inside) p.ro = ps;
         }
         if (inside) {
@@ -392,32 +407,36 @@ template <int N> struct Tracer {
                 if (pv != 0._r) p.ret = p.ret + p.thr * value * pv;
             }
             p.rd = phaseSample(ph, -p.rd, r3[0], r3[1]);   // weight eval/pdf = 1
-            rayIntersect(p.ro, p.rd, false, p.its);
+            Hit nx;
+            rayIntersect(p.ro, p.rd, false, nx);
+            p.face = nx.face;
             p.incEmission = false;
         } else {
-            if (p.its.face < 0) return false;
-            const DShape &sh = shapeOf(S, p.its.face);
+            if (its.face < 0) return false;
+            const DShape &sh = shapeOf(S, its.face);
             if (sh.light_id >= 0 && p.incEmission)
-                p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], p.its.gn, p.its.sn, -p.rd);
+                p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], its.gn, its.sn, -p.rd);
             if (p.depth >= p.max_depth) return false;
             const DBsdf &b = bsdfOf(S, sh);
-            Frame shf(p.its.sn);
+            Frame shf(its.sn);
             f3 wo;
             if (b.type != BSDF_NULL) {
                 real pdf_nee;
-                f3 value = neeSurface(p.its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
-                if (!allZero(value, 0._r)) p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo);
+                f3 value = neeSurface(its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
+                if (!allZero(value, 0._r)) p.ret = p.ret + p.thr * value * bsdfEval(b, its.wi, wo);
                 p.incEmission = false;
             }
             real bpdf = 0._r;
-            f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+            f3 bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
             if (allZero(bw, 0._r)) return false;
             p.thr = p.thr * bw;
             wo = shf.toWorld(wo);
-            p.ro = p.its.p;
+            p.ro = its.p;
             p.rd = wo;
-            if (isTransition(sh)) p.med = targetMedium(sh, p.its.gn, wo);
-            rayIntersect(p.ro, p.rd, true, p.its);
+            if (isTransition(sh)) p.med = targetMedium(sh, its.gn, wo);
+            Hit nx;
+            rayIntersect(p.ro, p.rd, true, nx);
+            p.face = nx.face;
         }
         p.depth++;
         return p.depth <= p.max_depth;
@@ -445,36 +464,39 @@ template <int N> struct Tracer {
 
     // path.cpp:11-61 — one bounce of PathTracer::radiance: surfaces only (media are
     // ignored), emission seen directly at the first hit, NEE and BSDF sampling with the
-    // power heuristic. p.its holds the current hit; the caller traces the camera ray
+    // power heuristic. p.face is the current ray's hit; the caller traces the camera ray
     // with tnear = ShadowEpsilon, as path.cpp:14 does.
     DI bool pathStep(LiPath &p, Rng &r) {
-        if (p.depth > p.max_depth || p.its.face < 0) return false;
-        const DShape &sh = shapeOf(S, p.its.face);
+        if (p.depth > p.max_depth || p.face < 0) return false;
+        const Hit its = hitOf(p);
+        const DShape &sh = shapeOf(S, its.face);
         if (sh.light_id >= 0 && p.depth == 0)
-            p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], p.its.gn, p.its.sn, -p.rd);
+            p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], its.gn, its.sn, -p.rd);
         if (p.depth >= p.max_depth) return false;
         const DBsdf &b = bsdfOf(S, sh);
-        Frame shf(p.its.sn);
+        Frame shf(its.sn);
         real r4[4] = {r.next(), r.next(), r.next(), r.next()};
         f3 wo;
         real pdf_nee;
-        f3 value = neeDirect(p.its, shf, r4, wo, pdf_nee);
+        f3 value = neeDirect(its, shf, r4, wo, pdf_nee);
         if (!allZero(value, 0._r))
-            p.ret = p.ret + p.thr * value * bsdfEval(b, p.its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, p.its.wi, wo));
+            p.ret = p.ret + p.thr * value * bsdfEval(b, its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, its.wi, wo));
         real r3[3] = {r.next(), r.next(), r.next()};
         real bpdf = 0._r;
-        f3 bw = bsdfSample(b, p.its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+        f3 bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
         if (allZero(bw, 0._r)) return false;
-        p.ro = p.its.p;
+        p.ro = its.p;
         p.rd = shf.toWorld(wo);
-        if (!rayIntersect(p.ro, p.rd, true, p.its)) return false;
+        Hit nx;
+        if (!rayIntersect(p.ro, p.rd, true, nx)) { p.face = -1; return false; }
+        p.face = nx.face;
         p.thr = p.thr * bw;
-        const DShape &hs = shapeOf(S, p.its.face);
+        const DShape &hs = shapeOf(S, nx.face);
         if (hs.light_id >= 0) {
-            f3 lc = emitterEval(S.emitters[hs.light_id], p.its.gn, p.its.sn, -p.rd);
+            f3 lc = emitterEval(S.emitters[hs.light_id], nx.gn, nx.sn, -p.rd);
             if (!allZero(lc, 0._r)) {
-                f3 dd = p.its.p - p.ro;
-                real G = p.its.wi.z / dot(dd, dd);
+                f3 dd = nx.p - p.ro;
+                real G = nx.wi.z / dot(dd, dd);
                 real pn = S.light_pmf[hs.light_id] / (S.light_areas[hs.light_id] * G);
                 p.ret = p.ret + p.thr * lc * misPower(bpdf, pn);
             }
