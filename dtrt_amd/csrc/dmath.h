@@ -30,8 +30,13 @@ constexpr bool kF64 = sizeof(real) == 8;
 // `1.5_r`: a literal of the estimator's type
 constexpr real operator""_r(long double v) { return (real)v; }
 constexpr real operator""_r(unsigned long long v) { return (real)v; }
-DI void rsincos(float a, float *s, float *c) { sincosf(a, s, c); }
-DI void rsincos(double a, double *s, double *c) { sincos(a, s, c); }
+// sin and cos of pi * a: the angles the warps draw (2 pi u) and the edge slerp's
+// angles, without the general argument reduction (whose double-precision path needs
+// many registers); mathematically the same as sincos(pi * a)
+DI void rsincospi(float a, float *s, float *c) { sincospif(a, s, c); }
+DI void rsincospi(double a, double *s, double *c) { sincospi(a, s, c); }
+DI float rsinpi(float a) { return sinpif(a); }
+DI double rsinpi(double a) { return sinpi(a); }
 
 // fwd.h:14, config.h:6-7.
 constexpr real EPS = 1e-7_r;
@@ -120,6 +125,8 @@ template <int N> HDI Df<N> dexp(const Df<N> &a) { real e = exp(a.v); Df<N> r(e);
 template <int N> HDI Df<N> dlog(const Df<N> &a) { real ia = 1.0_r / a.v; Df<N> r(log(a.v)); DF_LOOP r.d[i] = a.d[i] * ia; return r; }
 template <int N> HDI Df<N> dpow(const Df<N> &a, real p) { real t = pow(a.v, p - 1.0_r); Df<N> r(a.v * t); DF_LOOP r.d[i] = a.d[i] * p * t; return r; }
 template <int N> HDI Df<N> dpow(const Df<N> &a, const Df<N> &p) { return dexp(p * dlog(a)); }
+// a^1.5 = a * sqrt(a), with tangents 1.5 sqrt(a) a' (dpow(a, 1.5) without pow)
+template <int N> HDI Df<N> dpow15(const Df<N> &a) { real t = sqrt(a.v), h = 1.5_r * t; Df<N> r(a.v * t); DF_LOOP r.d[i] = a.d[i] * h; return r; }
 template <int N> HDI Df<N> dabs(const Df<N> &a) { return a.v > 0._r ? a : -a; }
 
 template <int N> struct Dv {
@@ -278,18 +285,18 @@ template <int N> struct FrameAD {   // frameAD.h:9-82 via coordinateSystemAD (ut
 // -------------------------------------------------------------------- warps
 // utils.cpp:18-52 (the reference's sin/cos naming swap in the concentric map is kept)
 DI f3 squareToCosineHemisphere(real sx, real sy) {
-    real r1 = 2.0_r * sx - 1.0_r, r2 = 2.0_r * sy - 1.0_r, phi, r;
+    real r1 = 2.0_r * sx - 1.0_r, r2 = 2.0_r * sy - 1.0_r, phi, r;   // phi in units of pi
     if (r1 == 0._r && r2 == 0._r) { r = phi = 0._r; }
-    else if (r1 * r1 > r2 * r2) { r = r1; phi = (PI / 4.0_r) * (r2 / r1); }
-    else { r = r2; phi = (PI / 2.0_r) - (r1 / r2) * (PI / 4.0_r); }
+    else if (r1 * r1 > r2 * r2) { r = r1; phi = 0.25_r * (r2 / r1); }
+    else { r = r2; phi = 0.5_r - (r1 / r2) * 0.25_r; }
     real sp, cp;
-    rsincos(phi, &sp, &cp);
+    rsincospi(phi, &sp, &cp);
     real px = r * sp, py = r * cp;
     return f3{px, py, sqrt(fmax(0._r, 1.0_r - px * px - py * py))};
 }
 DI f3 squareToUniformSphere(real sx, real sy) {
     real z = 1.0_r - 2.0_r * sy, r = sqrt(fmax(0._r, 1.0_r - z * z)), sp, cp;
-    rsincos(2.0_r * PI * sx, &sp, &cp);
+    rsincospi(2.0_r * sx, &sp, &cp);
     return f3{r * cp, r * sp, z};
 }
 HDI real luminance(f3 v) { return 0.212671_r * v.x + 0.715160_r * v.y + 0.072169_r * v.z; }

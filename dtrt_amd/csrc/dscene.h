@@ -520,7 +520,7 @@ template <int N> DI Df<N> phaseEvalAD(const DPhase &ph, const Dv<N> &wi, const f
     Df<N> g(ph.g);
     for (int j = 0; j < N; ++j) g.d[j] = ph.g_d[j];
     Df<N> tmp = 1.0_r + g * g + 2.0_r * g * ddot(wi, wo);
-    return INV_FOURPI * (1.0_r - g * g) / dpow(tmp, 1.5_r);
+    return INV_FOURPI * (1.0_r - g * g) / dpow15(tmp);
 }
 DI f3 phaseSample(const DPhase &ph, f3 wi, real u1, real u2) {
     if (ph.type == PHASE_ISO) return squareToUniformSphere(u1, u2);
@@ -531,7 +531,7 @@ DI f3 phaseSample(const DPhase &ph, f3 wi, real u1, real u2) {
         ct = (1.0_r + g * g - sq * sq) / (2.0_r * g);
     }
     real st = sqrt(fmax(0._r, 1.0_r - ct * ct)), sp, cp;
-    rsincos(2.0_r * PI * u2, &sp, &cp);
+    rsincospi(2.0_r * u2, &sp, &cp);
     return Frame(wi).toWorld(f3{st * cp, st * sp, ct});
 }
 template <int N> DI Df<N> phaseSampleWeightAD(const DPhase &ph, const Dv<N> &wi, f3 wo) {
@@ -564,15 +564,18 @@ DI real mfPdf(real alpha, f3 wi, f3 m) {
 }
 DI f3 mfSample(real alpha, f3 wi_, real s0, real s1) {
     f3 wi = normalize(f3{alpha * wi_.x, alpha * wi_.y, wi_.z});
-    real theta = 0._r, phi = 0._r;
-    if (wi.z < 0.99999_r) { theta = acos(wi.z); phi = atan2(wi.y, wi.x); }
-    real sP, cP;
-    rsincos(phi, &sP, &cP);
+    // theta, and phi as its sine and cosine (wi.y / r, wi.x / r: sincos of atan2(wi.y, wi.x))
+    real theta = 0._r, sP = 0._r, cP = 1._r;
+    if (wi.z < 0.99999_r) {
+        theta = acos(wi.z);
+        const real rxy = sqrt(wi.x * wi.x + wi.y * wi.y);
+        if (rxy > 0._r) { sP = wi.y / rxy; cP = wi.x / rxy; }
+    }
     real sx, sy;
     const real SQRT_PI_INV = 0.56418958354775628695_r;
     if (theta < 1e-4_r) {
         real r = sqrt(-log(1.0_r - s0)), sp, cp;
-        rsincos(2.0_r * PI * s1, &sp, &cp);
+        rsincospi(2.0_r * s1, &sp, &cp);
         sx = r * cp;
         sy = r * sp;
     } else {
@@ -701,7 +704,7 @@ DI f3 bsdfSample(const DBsdf &b, f3 wi, real r0, real r1, real r2, f3 &wo, real 
         case BSDF_PHONG: {
             if (r2 < b.specW) {
                 real sa = sqrt(fmax(0._r, 1.0_r - pow(r1, 2.0_r / (b.ex + 1.0_r)))), ca = pow(r1, 1.0_r / (b.ex + 1.0_r)), sp, cp;
-                rsincos(2.0_r * PI * r0, &sp, &cp);
+                rsincospi(2.0_r * r0, &sp, &cp);
                 wo = Frame(phongReflect(wi)).toWorld(f3{sa * cp, sa * sp, ca});
                 if (wo.z < EPS) return f3{0, 0, 0};
             } else {

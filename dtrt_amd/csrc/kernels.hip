@@ -214,8 +214,8 @@ DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const
         real l0 = length(e0.val()), l1 = length(e1.val());
         const real theta = edgeAngle(e0.val(), e1.val(), c01, l0, l1);
         if (theta < EPS || theta > PI - EPS) continue;
-        real sinT = sin(theta);
-        real w0 = sin(theta * (1.0_r - t)) / sinT / l0, w1 = sin(theta * t) / sinT / l1;
+        const real thp = theta * INV_PI, sinT = rsinpi(thp);   // sines through sinpi of theta / pi
+        real w0 = rsinpi(thp * (1.0_r - t)) / sinT / l0, w1 = rsinpi(thp * t) / sinT / l1;
         real ws = w0 + w1;
         w0 /= ws;
         w1 /= ws;
@@ -385,8 +385,8 @@ DI void adSurfaceSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rn
                     const real l0 = length(e0.val()), l1 = length(e1.val());
                     const real theta = edgeAngle(e0.val(), e1.val(), c01, l0, l1);
                     if (theta < EPS || theta > PI - EPS) continue;
-                    const real sinT = sin(theta);
-                    real w0 = sin(theta * (1.0_r - t)) / sinT / l0, w1 = sin(theta * t) / sinT / l1;
+                    const real thp = theta * INV_PI, sinT = rsinpi(thp);
+                    real w0 = rsinpi(thp * (1.0_r - t)) / sinT / l0, w1 = rsinpi(thp * t) / sinT / l1;
                     const real ws = w0 + w1;
                     w0 /= ws;
                     w1 /= ws;
