@@ -36,6 +36,12 @@ hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm) { 
 hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm) {
     return p32::launch_load_density(dst, src, n, maxbits, sm);
 }
+hipError_t launch_l2_grad(const float *img, const float *target, float *dimg, uint64_t n, double *loss, hipStream_t sm) {
+    return p32::launch_l2_grad(img, target, dimg, n, loss, sm);
+}
+hipError_t launch_contract(const float *img, const float *dimg, uint64_t n, int nder, double *grad, hipStream_t sm) {
+    return p32::launch_contract(img, dimg, n, nder, grad, sm);
+}
 hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,
                                unsigned long long *work, hipStream_t sm) {
     return DT_BY_PREC(S, launch_primal_paths, S, P, kind, s0, n, acc, work, sm);

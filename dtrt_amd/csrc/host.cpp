@@ -639,6 +639,8 @@ struct Scene : Obj {
     ~Scene() override {
         if (stream) (void)hipStreamSynchronize(stream);
         graph.reset();
+        stepg.reset();
+        if (stepbuf.p) (void)hipFree(stepbuf.p);
         for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
         for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf, &tunebuf})
             if (b->p) (void)hipFree(b->p);
@@ -755,7 +757,39 @@ struct Scene : Obj {
         return renderAdjointViews(1, &c, &o, med, dimg, grad);
     }
     bool renderAdjointViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, int med, const float *dimg,
-                            float *grad);
+                            float *grad, Bump *pool = nullptr, bool launch = true, RenderParams *Pout = nullptr,
+                            uint64_t *total_out = nullptr);
+    // One L2 inverse-rendering step (dtrt_scene_capture_step): forward views into
+    // `images`, d loss / d I = 2 (I - T), its contraction with the tangent images into
+    // pgrad[nder], the density adjoint into dgrid. Eager; captureStep records the same
+    // launches as a graph, replayStep relaunches it with new seeds.
+    struct StepArgs {
+        std::vector<Camera> cams;
+        std::vector<dtrt_render_options> fopts, aopts;
+        std::vector<float *> images;
+        std::vector<const float *> targets;
+        int med = -1;
+        float *dgrid = nullptr;
+        double *pgrad = nullptr, *loss = nullptr;
+    };
+    bool runStep(StepArgs &a);
+    bool stepLoss(StepArgs &a, const std::vector<DView> &hv, hipStream_t sm);
+    bool captureStep(StepArgs &a);
+    int replayStep(const uint64_t *seeds);
+    struct StepGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        StepArgs args;
+        std::vector<DView> hvf, hva;
+        std::vector<void *> bufs;
+        void reset() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            x = nullptr;
+            g = nullptr;
+        }
+    } stepg;
+    DevBuf stepbuf;   // the step's d loss / d I images and its adjoint view table
     bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator);
     bool autotune(const dtrt_render_options &o, int ad_kind, double *best_ms, int64_t *settings3);
     bool setDensity(int med, const float *density, bool on_device);
@@ -1380,6 +1414,209 @@ int Scene::replayViews(const uint64_t *seeds) {
     return 0;
 }
 
+// d loss / d I of every view into stepbuf (views concatenated like the adjoint's
+// loss-gradient input), the loss into a.loss, the lanes' gradient into a.pgrad.
+bool Scene::stepLoss(StepArgs &a, const std::vector<DView> &hv, hipStream_t sm) {
+    float *dimg = static_cast<float *>(stepbuf.p);
+    for (size_t v = 0; v < hv.size(); ++v) {
+        const uint64_t n = (uint64_t)hv[v].npix * 3;
+        float *dv = dimg + (size_t)hv[v].pix0 * 3;
+        HIPCHK(launch_l2_grad(a.images[v], a.targets[v], dv, n, a.loss, sm));
+        if (a.pgrad) HIPCHK(launch_contract(a.images[v], dv, n, nder, a.pgrad, sm));
+    }
+    return true;
+}
+
+static bool stepOptionsValid(const Scene::StepArgs &a) {
+    for (size_t v = 0; v < a.fopts.size(); ++v)
+        if (!a.images[v] || !a.targets[v] || a.fopts[v].spp <= 0 || a.aopts[v].spp <= 0)
+            return fail("step: every view needs an image, a target and positive spp");
+    return true;
+}
+
+bool Scene::runStep(StepArgs &a) {
+    const int nv = (int)a.cams.size();
+    if (!gridMedium(a.med) || !a.dgrid) return false;
+    if (!stepOptionsValid(a)) return false;
+    std::vector<const Camera *> cp;
+    for (auto &c : a.cams) cp.push_back(&c);
+    if (!afterCaller()) return false;
+    for (int v = 0; v < nv; ++v) {   // the forward render adds into zeroed images
+        const Camera &c = a.cams[v];
+        const uint64_t np = c.cropped() ? (uint64_t)c.cw * c.ch : (uint64_t)c.w * c.h;
+        HIPCHK(launch_fill32(a.images[v], (uint64_t)(nder + 1) * np * 3, 0u, stream));
+    }
+    if (!renderViews(nv, cp.data(), a.fopts.data(), a.images.data(), true)) return false;
+    std::vector<DView> hv;
+    std::vector<uint64_t> pe_total;
+    if (!setupViews(nv, cp.data(), a.fopts.data(), hv, pe_total)) return false;
+    int64_t npix = 0;
+    for (auto &d : hv) npix += d.npix;
+    if (!ensure(stepbuf, (size_t)npix * 3 * 4 + (size_t)nv * sizeof(DView) + 512)) return false;
+    const DMedium &m = hmed[a.med];
+    HIPCHK(launch_fill32(a.dgrid, (uint64_t)m.res[0] * m.res[1] * m.res[2], 0u, stream));
+    if (a.pgrad) HIPCHK(launch_fill32(a.pgrad, 2 * (uint64_t)nder, 0u, stream));
+    if (a.loss) HIPCHK(launch_fill32(a.loss, 2, 0u, stream));
+    if (!stepLoss(a, hv, stream)) return false;
+    return renderAdjointViews(nv, cp.data(), a.aopts.data(), a.med, static_cast<const float *>(stepbuf.p), a.dgrid);
+}
+
+// The eager step fixes the forward chunk schedule; the graph then holds: clears, the
+// forward chunks and primary-edge passes, finalize, the loss gradient and lane
+// contraction, the grid clear and the adjoint chunks. The forward views keep the view
+// table in viewbuf, the adjoint's lies in stepbuf, so replays rewrite both.
+bool Scene::captureStep(StepArgs &a) {
+    stepg.reset();
+    if (!runStep(a)) return false;
+    if (last_replayed) return fail("capture_step: the schedule overflowed the child queue; render in smaller batches");
+    const int nv = (int)a.cams.size();
+    std::vector<const Camera *> cp;
+    for (auto &c : a.cams) cp.push_back(&c);
+    RenderParams P{};
+    const dtrt_render_options &o0 = a.fopts[0];
+    P.ad_kind = AD_VOLPATH;
+    P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
+    P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
+    P.rec_stride = childRecordStride(nder, S.f64 != 0);
+    std::vector<DView> hvf, hva;
+    std::vector<uint64_t> pe_total, pe_unused;
+    if (!setupViews(nv, cp.data(), a.fopts.data(), hvf, pe_total)) return false;
+    const DScene Sf = S;
+    int64_t npix_all = 0;
+    for (int v = 0; v < nv; ++v) npix_all += hvf[v].npix;
+    P.npix = (int)npix_all;
+    char *base = static_cast<char *>(stepbuf.p);
+    Bump apool{base + ((size_t)npix_all * 3 * 4 + 255) / 256 * 256, (uint64_t)nv * sizeof(DView) + 256};
+    RenderParams PA{};
+    uint64_t atotal = 0;
+    if (!renderAdjointViews(nv, cp.data(), a.aopts.data(), a.med, static_cast<const float *>(stepbuf.p), a.dgrid, &apool,
+                            false, &PA, &atotal))
+        return false;
+    const DScene Sa = S;
+    hva.assign(nv, DView{});
+    HIPCHK(hipMemcpyAsync(hva.data(), Sa.views, (size_t)nv * sizeof(DView), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    S = Sf;
+    const size_t npix = P.npix, img = (size_t)(nder + 1) * npix * 3;
+    const uint64_t cap = std::max<uint64_t>(128, std::min<uint64_t>(queue_budget / P.rec_stride, 0xFFFFFFF0ull));
+    const uint64_t pe_chunk = std::max<uint64_t>(64, std::min<uint64_t>(cap / 2 / 64 * 64, uint64_t(1) << 23));
+    P.qcap = (unsigned)cap;
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned childGrid = (unsigned)std::max(1, ncu) * (unsigned)tune_child_blocks;
+    unsigned *qc = static_cast<unsigned *>(qcount.p);
+    unsigned *st = static_cast<unsigned *>(stats.p);
+    unsigned long long *wk = static_cast<unsigned long long *>(work.p);
+    void *A = acc.p, *APE = acc_pe.p;
+    char *Q = static_cast<char *>(queue.p);
+    uint4 *OV = static_cast<uint4 *>(ovf[0].p);
+    const DMedium &m = hmed[a.med];
+    const uint64_t ngrid = (uint64_t)m.res[0] * m.res[1] * m.res[2];
+    const uint64_t achunk = uint64_t(1) << 22;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    auto issue = [&]() -> hipError_t {
+        hipError_t e;
+#define GCHK(x) if ((e = (x)) != hipSuccess) return e
+        GCHK(launch_fill32(A, img * accBytes() / 4, 0u, stream));
+        GCHK(launch_fill32(APE, (size_t)nder * npix * 3 * accBytes() / 4, 0u, stream));
+        GCHK(launch_fill32(st, 64, 0u, stream));
+        GCHK(launch_fill32(wk, 32, 0u, stream));
+        for (int v = 0; v < nv; ++v) GCHK(launch_fill32(a.images[v], (size_t)(nder + 1) * hvf[v].npix * 3, 0u, stream));
+        uint64_t s0 = 0;
+        for (uint64_t n : last_chunks) {
+            GCHK(launch_fill32(qc, 3, 0u, stream));
+            GCHK(launch_ad_paths(Sf, P, s0, n, nullptr, A, Q, qc, st, OV, qc + 1, wk, stream));
+            GCHK(launch_add_u32(qc + 3, qc + 1, stream));
+            GCHK(launch_children(Sf, P, Q, qc, A, APE, wk, childGrid, stream));
+            s0 += n;
+        }
+        for (int v = 0; v < nv; ++v)
+            for (uint64_t p0 = 0; p0 < pe_total[v]; p0 += pe_chunk) {
+                GCHK(launch_fill32(qc, 3, 0u, stream));
+                GCHK(launch_primary(Sf, P, v, p0, std::min(pe_chunk, pe_total[v] - p0), pe_total[v], Q, qc, st, wk, stream));
+                GCHK(launch_children(Sf, P, Q, qc, A, APE, wk, childGrid, stream));
+            }
+        for (int v = 0; v < nv; ++v)
+            GCHK(launch_finalize(Sf, P, hvf[v].npix, hvf[v].pix0, hvf[v].spp, pe_total[v] > 0, A, APE, a.images[v], st, stream));
+        if (a.pgrad) GCHK(launch_fill32(a.pgrad, 2 * (uint64_t)nder, 0u, stream));
+        if (a.loss) GCHK(launch_fill32(a.loss, 2, 0u, stream));
+        float *dimg = static_cast<float *>(stepbuf.p);
+        for (int v = 0; v < nv; ++v) {
+            const uint64_t n = (uint64_t)hvf[v].npix * 3;
+            GCHK(launch_l2_grad(a.images[v], a.targets[v], dimg + (size_t)hvf[v].pix0 * 3, n, a.loss, stream));
+            if (a.pgrad) GCHK(launch_contract(a.images[v], dimg + (size_t)hvf[v].pix0 * 3, n, nder, a.pgrad, stream));
+        }
+        GCHK(launch_fill32(a.dgrid, ngrid, 0u, stream));
+        for (uint64_t a0 = 0; a0 < atotal; a0 += achunk)
+            GCHK(launch_adjoint(Sa, PA, a0, std::min(achunk, atotal - a0), dimg, a.dgrid, a.med, st + 32, wk, stream));
+#undef GCHK
+        return hipSuccess;
+    };
+    hipError_t e = issue();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(std::string("capture_step: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    stepg.g = g;
+    HIPCHK(hipGraphInstantiate(&stepg.x, g, nullptr, nullptr, 0));
+    stepg.args = a;
+    stepg.hvf = hvf;
+    stepg.hva = hva;
+    stepg.bufs = graphBuffers();
+    stepg.bufs.push_back(stepbuf.p);
+    return true;
+}
+
+// Replay the captured step with new seeds: seeds[2v] forward, seeds[2v + 1] adjoint
+// of view v. 0 after a graph launch, 1 if the forward schedule overflowed the queue
+// and the step ran eagerly instead (exact either way), -1 on error.
+int Scene::replayStep(const uint64_t *seeds) {
+    if (!stepg.x) { g_err = "replay_step: no captured step"; return -1; }
+    std::vector<void *> now = graphBuffers();
+    now.push_back(stepbuf.p);
+    if (now != stepg.bufs) { g_err = "replay_step: an eager render reallocated the buffers; capture again"; return -1; }
+    if (!afterCaller()) return -1;
+    const int nv = (int)stepg.hvf.size();
+    for (int v = 0; v < nv; ++v) {
+        stepg.hvf[v].seed = stepg.args.fopts[v].seed = seeds[2 * v];
+        stepg.hva[v].seed = stepg.args.aopts[v].seed = seeds[2 * v + 1];
+    }
+    char *base = static_cast<char *>(stepbuf.p);
+    int64_t npix_all = 0;
+    for (auto &d : stepg.hvf) npix_all += d.npix;
+    void *avt = base + ((size_t)npix_all * 3 * 4 + 255) / 256 * 256;
+    unsigned *qc = static_cast<unsigned *>(qcount.p);
+    if (ev.empty()) {
+        ev.resize(2);
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    if (hipMemcpyAsync(viewbuf.p, stepg.hvf.data(), (size_t)nv * sizeof(DView), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(avt, stepg.hva.data(), (size_t)nv * sizeof(DView), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemsetAsync(qc + 3, 0, 4, stream) != hipSuccess || hipEventRecord(ev[0], stream) != hipSuccess ||
+        hipGraphLaunch(stepg.x, stream) != hipSuccess || hipEventRecord(ev[1], stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        g_err = "replay_step: graph launch failed";
+        return -1;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    unsigned flag = 0;
+    if (hipMemcpy(&flag, qc + 3, 4, hipMemcpyDeviceToHost) != hipSuccess) { g_err = "replay_step: readback failed"; return -1; }
+    if (flag) {   // some forward sample overflowed the queue: run these seeds eagerly (exact)
+        StepArgs a = stepg.args;
+        if (!runStep(a)) return -1;
+        return 1;
+    }
+    std::memset(kernel_ms, 0, sizeof(kernel_ms));
+    std::memset(kernel_launches, 0, sizeof(kernel_launches));
+    kernel_ms[0] = ms;
+    kernel_launches[0] = 1;
+    return 0;
+}
+
 // Replace the density grid of medium `med` (same resolution): negative values are
 // clamped to 0 as the VOL reader does, and the majorant is recomputed.
 bool Scene::setDensity(int med, const float *density, bool on_device) {
@@ -1424,9 +1661,10 @@ bool Scene::setVoxelLanes(int med, const int *vox, int n) {
 // Views adjoint: dimg holds the views' loss gradients back to back ([h][w][3] each, in
 // view order); every view's samples replay the streams of its forward render.
 bool Scene::renderAdjointViews(int nv, const Camera *const *cams, const dtrt_render_options *opts, int med,
-                               const float *dimg, float *grad) {
+                               const float *dimg, float *grad, Bump *pool, bool launch, RenderParams *Pout,
+                               uint64_t *total_out) {
     if (!gridMedium(med) || !dimg || !grad || nv <= 0) return false;
-    if (!afterCaller()) return false;
+    if (launch && !afterCaller()) return false;
     const dtrt_render_options &o = opts[0];
     RenderParams P{};
     P.max_bounces = o.max_bounces; P.sppse = 0;
@@ -1440,10 +1678,13 @@ bool Scene::renderAdjointViews(int nv, const Camera *const *cams, const dtrt_ren
     }
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
-    if (!setupViews(nv, cams, ov.data(), hv, pe_total)) return false;
+    if (!setupViews(nv, cams, ov.data(), hv, pe_total, pool)) return false;
     uint64_t total = 0;
     P.npix = 0;
     for (int v = 0; v < nv; ++v) { P.npix += hv[v].npix; total += (uint64_t)hv[v].npix * hv[v].spp; }
+    if (Pout) *Pout = P;
+    if (total_out) *total_out = total;
+    if (!launch) return true;
     if (!ensure(stats, 64 * 4) || !ensure(work, 16 * 8)) return false;
     HIPCHK(hipMemsetAsync(work.p, 0, 16 * 8, stream));
     if (ev.empty()) {
@@ -1988,6 +2229,60 @@ int dtrt_scene_capture_views(dtrt_object *scene, int nviews, dtrt_object *const 
         g_err = e.what();
         return -1;
     }
+}
+
+static bool stepArgs(Scene *s, int nviews, dtrt_object *const *cameras, const dtrt_render_options *options,
+                     float *const *images, const float *const *targets, int medium, const dtrt_render_options *adj_options,
+                     float *d_density, double *param_grad, double *loss, Scene::StepArgs &a) {
+    if (nviews <= 0 || !options || !adj_options || !images || !targets || !d_density)
+        return fail("step: bad arguments");
+    for (int v = 0; v < nviews; ++v) {
+        const Camera *c = (cameras && cameras[v]) ? as<Camera>(cameras[v], T_CAMERA) : &s->cam;
+        if (!c) return false;
+        a.cams.push_back(*c);
+    }
+    a.fopts.assign(options, options + nviews);
+    a.aopts.assign(adj_options, adj_options + nviews);
+    a.images.assign(images, images + nviews);
+    a.targets.assign(targets, targets + nviews);
+    a.med = medium;
+    a.dgrid = d_density;
+    a.pgrad = param_grad;
+    a.loss = loss;
+    return true;
+}
+
+int dtrt_scene_step(dtrt_object *scene, int nviews, dtrt_object *const *cameras, const dtrt_render_options *options,
+                    float *const *images, const float *const *targets, int medium,
+                    const dtrt_render_options *adj_options, float *d_density, double *param_grad, double *loss) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    return guarded<int>(-1, [&]() -> int {
+        Scene::StepArgs a;
+        if (!stepArgs(s, nviews, cameras, options, images, targets, medium, adj_options, d_density, param_grad, loss, a))
+            return -1;
+        return s->runStep(a) ? 0 : -1;
+    });
+}
+
+int dtrt_scene_capture_step(dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                            const dtrt_render_options *options, float *const *images, const float *const *targets,
+                            int medium, const dtrt_render_options *adj_options, float *d_density, double *param_grad,
+                            double *loss) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    return guarded<int>(-1, [&]() -> int {
+        Scene::StepArgs a;
+        if (!stepArgs(s, nviews, cameras, options, images, targets, medium, adj_options, d_density, param_grad, loss, a))
+            return -1;
+        return s->captureStep(a) ? 0 : -1;
+    });
+}
+
+int dtrt_scene_replay_step(dtrt_object *scene, const uint64_t *seeds) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !seeds) { if (s) g_err = "replay_step: bad arguments"; return -1; }
+    return guarded<int>(-1, [&]() -> int { return s->replayStep(seeds); });
 }
 
 int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {

@@ -956,6 +956,42 @@ __global__ void k_add_u32(unsigned *dst, const unsigned *src) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *dst += *src;
 }
 
+// One view of an L2 image loss sum (I - T)^2 over the value channel: dimg = 2 (I - T),
+// the loss added into *loss (one double atomic per wave).
+__global__ void k_l2_grad(const float *img, const float *target, float *dimg, uint64_t n, double *loss) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (i < n) {
+        const float e = img[i] - target[i];
+        dimg[i] = 2.f * e;
+        e2 = (double)e * e;
+    }
+    for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off);
+    if ((threadIdx.x & 63) == 0 && e2 != 0.0) unsafeAtomicAdd(loss, e2);
+}
+
+// grad[k] += sum_i dimg[i] * tangent_k[i] for the view image img = [(nder + 1)][n]
+// (value, then one tangent image per lane): the loss gradient of each forward-mode
+// parameter, grid-stride over the n entries, one double atomic per wave and lane.
+__global__ void k_contract(const float *img, const float *dimg, uint64_t n, int nder, double *grad) {
+    double acc[DTRT_NDER_MAX];
+#pragma unroll
+    for (int k = 0; k < DTRT_NDER_MAX; ++k) acc[k] = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const double g = dimg[i];
+#pragma unroll
+        for (int k = 0; k < DTRT_NDER_MAX; ++k)
+            if (k < nder) acc[k] += g * (double)img[(uint64_t)(k + 1) * n + i];
+    }
+#pragma unroll
+    for (int k = 0; k < DTRT_NDER_MAX; ++k) {
+        if (k >= nder) break;
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & 63) == 0 && v != 0.0) unsafeAtomicAdd(grad + k, v);
+    }
+}
+
 // dst = max(src, 0) (NaN -> 0), and the maximum into *maxbits (non-negative floats
 // order like their bit patterns). dst may alias src.
 __global__ void k_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits) {
@@ -1452,6 +1488,17 @@ hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm) {
 }
 hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm) {
     hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, sm, dst, src);
+    return hipGetLastError();
+}
+hipError_t launch_l2_grad(const float *img, const float *target, float *dimg, uint64_t n, double *loss, hipStream_t sm) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_l2_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sm, img, target, dimg, n, loss);
+    return hipGetLastError();
+}
+hipError_t launch_contract(const float *img, const float *dimg, uint64_t n, int nder, double *grad, hipStream_t sm) {
+    if (n == 0 || nder <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)std::min<uint64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_contract, dim3(nb), dim3(256), 0, sm, img, dimg, n, std::min(nder, DTRT_NDER_MAX), grad);
     return hipGetLastError();
 }
 hipError_t launch_load_density(float *dst, const float *src, uint64_t n, unsigned *maxbits, hipStream_t sm) {

@@ -197,6 +197,51 @@ def replay_views(scene, seeds):
     return rc
 
 
+_lib.dtrt_scene_step.argtypes = [_P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
+                                 C.POINTER(_P), C.POINTER(_P), C.c_int,
+                                 C.POINTER(_m.fns["render"].argtypes[2]._type_), _P, _P, _P]
+_lib.dtrt_scene_capture_step.argtypes = _lib.dtrt_scene_step.argtypes
+_lib.dtrt_scene_replay_step.argtypes = [_P, C.POINTER(C.c_uint64)]
+
+
+def _step_args(scene, cameras, options, images, targets, medium, adj_options, d_density, param_grad, loss):
+    n = len(options)
+    cams = (_P * n)(*[None if c is None else c._h for c in (cameras or [None] * n)])
+    opts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in options])
+    aopts = (_m.fns["render"].argtypes[2]._type_ * n)(*[o._struct() for o in adj_options])
+    ims = (_P * n)(*[im.data_ptr() for im in images])
+    tgs = (_P * n)(*[t.data_ptr() for t in targets])
+    return (scene._h, n, cams, opts, ims, tgs, int(medium), aopts, d_density.data_ptr(),
+            None if param_grad is None else param_grad.data_ptr(), None if loss is None else loss.data_ptr())
+
+
+def step(scene, cameras, options, images, targets, medium, adj_options, d_density, param_grad=None, loss=None):
+    """``dtrt_scene_step``: one L2 inverse-rendering step over the views (device tensors:
+    images [(nder+1), h, w, 3], targets [h, w, 3], d_density [rz, ry, rx] float32,
+    param_grad [nder] and loss [1] float64)."""
+    _after_torch(scene)
+    _chk(_lib.dtrt_scene_step(*_step_args(scene, cameras, options, images, targets, medium, adj_options, d_density,
+                                          param_grad, loss)), "step")
+
+
+def capture_step(scene, cameras, options, images, targets, medium, adj_options, d_density, param_grad=None, loss=None):
+    """``dtrt_scene_capture_step``: run the step once, then capture it as a HIP graph."""
+    _after_torch(scene)
+    _chk(_lib.dtrt_scene_capture_step(*_step_args(scene, cameras, options, images, targets, medium, adj_options,
+                                                  d_density, param_grad, loss)), "capture_step")
+
+
+def replay_step(scene, seeds):
+    """``dtrt_scene_replay_step``: seeds[2v] forward, seeds[2v+1] adjoint; 0 = graph replay,
+    1 = eager fallback (queue overflow)."""
+    s = (C.c_uint64 * len(seeds))(*[int(x) for x in seeds])
+    _after_torch(scene)
+    rc = _lib.dtrt_scene_replay_step(scene._h, s)
+    if rc < 0:
+        raise RuntimeError("replay_step: " + _lib.dtrt_last_error().decode())
+    return rc
+
+
 def set_density(scene, medium, density):
     """``dtrt_scene_set_density``: a torch tensor (host or device) or numpy array [rz][ry][rx]."""
     import numpy as np

@@ -210,6 +210,32 @@ int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dt
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
                         const float *d_image_device, float *d_density_device);
 
+/* No pybind counterpart (the batched inverse-rendering step; the reference's examples
+ * run it as a Python loop of render, loss and backward calls, e.g.
+ * examples/glass_1/optimize.py:51-68). One step of an L2 image loss
+ *   L = sum over views, pixels and colour channels of (I - T)^2
+ * over nviews views (cameras[v], or the scene's camera where NULL): the forward-mode
+ * render of every view into images[v] ([(nder+1)][h][w][3], device), the loss into
+ * *loss (double, device, may be NULL), the loss gradient of every tangent lane,
+ * sum 2 (I - T) dI/dtheta_k, into param_grad[nder] (double, device, may be NULL), and
+ * the path-replay density adjoint of medium `medium` with adj_options[v] (interior
+ * EMode terms) into d_density (float, device; overwritten, not accumulated). targets[v]
+ * are [h][w][3] device images. dtrt_scene_step runs it eagerly; capture_step runs it
+ * once and records the same launch sequence as one HIP graph (the forward chunk
+ * schedule is that run's); replay_step relaunches the graph with new seeds
+ * (seeds[2v] forward, seeds[2v+1] adjoint) and returns 0, or 1 when a forward sample
+ * overflowed the fixed schedule's queue and the step ran eagerly instead (exact
+ * either way), -1 on error. An eager render on the scene in between may reallocate
+ * the buffers the graph uses; replay then fails and asks for a new capture. */
+int dtrt_scene_step(dtrt_object *scene, int nviews, dtrt_object *const *cameras, const dtrt_render_options *options,
+                    float *const *images, const float *const *targets, int medium,
+                    const dtrt_render_options *adj_options, float *d_density, double *param_grad, double *loss);
+int dtrt_scene_capture_step(dtrt_object *scene, int nviews, dtrt_object *const *cameras,
+                            const dtrt_render_options *options, float *const *images, const float *const *targets,
+                            int medium, const dtrt_render_options *adj_options, float *d_density, double *param_grad,
+                            double *loss);
+int dtrt_scene_replay_step(dtrt_object *scene, const uint64_t *seeds);
+
 /* No counterpart in the reference: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). Same as dtrt_render_device for one view (integrator 0, 6 or 7), but
