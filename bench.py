@@ -347,13 +347,14 @@ def configs_leg():
     return rec
 
 
-def inverse_leg(spp_divisor=1):
-    """One batched inverse-rendering step: 32 orbit views
-    (res 128^2..768^2, spp 16..256 drawn with PCG32 seed 5, each divided by
-    spp_divisor) of a 1,003,520-face displaced sphere in a 256^3 fBm medium. Forward:
-    one dtrt_render_views pass with 8 shape-deformation tangent lanes (all terms).
-    Backward: one dtrt_render_views_adjoint pass for all 16.8M node densities, with
-    the loss sum(I^2) over every view (d loss / d I = 2 I)."""
+def inverse_leg(spp_divisor=16):
+    """One batched inverse-rendering step on 32 orbit views (res 128^2..768^2, spp
+    16..256 drawn with PCG32 seed 5, each divided by spp_divisor) of a 1,003,520-face
+    displaced sphere in a 256^3 fBm medium, through dtrt_scene_step: the forward render
+    of all views with 8 shape-deformation tangent lanes (all terms), the L2 loss against
+    target images, its gradient for the 8 shape lanes, and the density adjoint over all
+    16.8M grid nodes. Timed eagerly, then as one captured HIP graph replayed with new
+    seeds; the forward pass is also timed batched against one call per view."""
     import numpy as np
     import torch
     import dtrt_amd.dtrt as G
@@ -367,30 +368,45 @@ def inverse_leg(spp_divisor=1):
     cams = [G.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
             for c2w, w, h, _ in views]
     spps = [max(1, spp // spp_divisor) for _, _, _, spp in views]
-    opts = [G.RenderOptions(500 + i, spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist) for i, spp in enumerate(spps)]
-    outs = [torch.zeros((d.nder + 1, h, w, 3), device="cuda") for _, w, h, _ in views]
+    fo = [G.RenderOptions(500 + i, spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist) for i, spp in enumerate(spps)]
+    ao = [G.RenderOptions(900 + i, spp, d.max_bounces, 0, 0, True, 1 | 2 | 8, 0.0) for i, spp in enumerate(spps)]
+    ims = [torch.zeros((d.nder + 1, h, w, 3), device="cuda") for _, w, h, _ in views]
+    rng = np.random.default_rng(6)
+    tgts = [torch.tensor(rng.random((h, w, 3)) * 0.2, dtype=torch.float32, device="cuda") for _, w, h, _ in views]
+    grad = torch.zeros(d.media[0].density.shape, dtype=torch.float32, device="cuda")
+    pgrad = torch.zeros(d.nder, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
     samples = sum(w * h * spp for (_, w, h, _), spp in zip(views, spps))
-    G.render_views(sc, cams[:1], opts[:1], outs[:1])   # builds device buffers, primary edges of view 0
+    G.render_views(sc, cams[:1], fo[:1], [torch.zeros_like(ims[0])])   # device buffers, primary edges of view 0
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    G.render_views(sc, cams, opts, outs)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    dimgs = [2.0 * o[0] for o in outs]
-    grad = torch.zeros(d.media[0].density.shape, dtype=torch.float32, device="cuda")
-    aopts = [G.RenderOptions(900 + i, spp, d.max_bounces, d.sppe, 0, True, 1 | 2 | 8, 0.0) for i, spp in enumerate(spps)]
-    t0 = time.perf_counter()
-    G.render_views_adjoint(sc, cams, aopts, 0, dimgs, grad)
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
+
+    def clock(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, r
+
+    t_capture, _ = clock(lambda: G.capture_step(sc, cams, fo, ims, tgts, 0, ao, grad, pgrad, loss))
+    t_graph, rc = clock(lambda: G.replay_step(sc, [700 + i for i in range(2 * len(views))]))
+    t_eager, _ = clock(lambda: G.step(sc, cams, fo, ims, tgts, 0, ao, grad, pgrad, loss))
+    t_batch, _ = clock(lambda: G.render_views(sc, cams, fo, ims))
+
+    def singles():
+        for i in range(len(views)):
+            G.render_views(sc, [cams[i]], [fo[i]], [ims[i]])
+    t_single, _ = clock(singles)
     return {"workload": "batched inverse-rendering step: 32 orbit views (res 128^2..768^2, spp 16..256 / %d, PCG32 seed 5) "
-                        "of a 1,003,520-face "
-                        "displaced sphere in a 256^3 fBm medium; forward nder 8 (shape lanes), all terms; adjoint over "
-                        "16,777,216 node densities, loss sum(I^2)" % spp_divisor,
+                        "of a 1,003,520-face displaced sphere in a 256^3 fBm medium; forward nder 8 (shape lanes), all "
+                        "terms; L2 loss vs target images; gradient of the 8 lanes; density adjoint over 16,777,216 nodes"
+                        % spp_divisor,
             "spp_divisor": spp_divisor, "samples": samples, "scene_generation_s": t_gen, "setup_s": t_setup,
-            "forward_views_s": t_fwd, "forward_samples_per_s": samples / t_fwd, "adjoint_views_s": t_adj,
-            "step_s": t_fwd + t_adj, "grad_nonzero_nodes": int((grad != 0).sum().item())}
+            "eager_step_s": t_eager, "graph_step_s": t_graph, "graph_fell_back_to_eager": bool(rc == 1),
+            "graph_speedup": t_eager / t_graph, "capture_s_including_one_eager_step": t_capture,
+            "step_samples_per_s": samples / t_graph, "forward_batched_s": t_batch, "forward_one_call_per_view_s": t_single,
+            "batching_speedup": t_single / t_batch, "loss": float(loss.item()),
+            "shape_lane_grad": [float(x) for x in pgrad.cpu()], "grad_nonzero_nodes": int((grad != 0).sum().item())}
 
 
 def main():
@@ -410,7 +426,7 @@ def main():
     ap.add_argument("--no-primal", action="store_true", help="skip the primal-integrator timing")
     ap.add_argument("--no-examples", action="store_true", help="skip the glass_1 optimisation-step timing")
     ap.add_argument("--no-configs", action="store_true", help="skip the surface / dielectric scene timings")
-    ap.add_argument("--inverse-spp-divisor", type=int, default=8,
+    ap.add_argument("--inverse-spp-divisor", type=int, default=16,
                     help="divide the batched inverse step's per-view spp by this (0: skip the leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (reference kernel for HBM counters)")

@@ -42,6 +42,10 @@ hipError_t launch_l2_grad(const float *img, const float *target, float *dimg, ui
 hipError_t launch_contract(const float *img, const float *dimg, uint64_t n, int nder, double *grad, hipStream_t sm) {
     return p32::launch_contract(img, dimg, n, nder, grad, sm);
 }
+hipError_t launch_tri_records(const DScene &S, hipStream_t sm) { return p32::launch_tri_records(S, sm); }
+hipError_t launch_refit(const DScene &S, const int *list, int n, hipStream_t sm) {
+    return p32::launch_refit(S, list, n, sm);
+}
 hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,
                                unsigned long long *work, hipStream_t sm) {
     return DT_BY_PREC(S, launch_primal_paths, S, P, kind, s0, n, acc, work, sm);

@@ -129,6 +129,21 @@ struct Medium : Obj {
 };
 
 // shape.cpp:109-126 — face normals and their tangents (double)
+// shape.cpp:37-51 — vertex tangents [nder][nv][3]; with normals, the normal tangents
+// follow as one [nv][3] block that every lane reads (shape.cpp:45, no lane offset)
+void setShapeVelocities(Shape &s, const float *vel) {
+    const int nd = s.nder, nv = s.nv;
+    for (int j = 0; j < nd; ++j)
+        for (int i = 0; i < nv; ++i) {
+            const float *p = vel + ((size_t)j * nv + i) * 3;
+            s.vd[(size_t)i * nd + j] = D3{p[0], p[1], p[2]};
+            if (s.hasNormals()) {
+                const float *q = vel + (size_t)nv * 3 * nd + (size_t)i * 3;
+                s.nd[(size_t)i * nd + j] = D3{q[0], q[1], q[2]};
+            }
+        }
+}
+
 void computeFaceNormals(Shape &s) {
     s.fn.assign(s.nf, D3{0, 0, 0});
     s.fnd.assign((size_t)s.nf * s.nder, D3{0, 0, 0});
@@ -641,6 +656,8 @@ struct Scene : Obj {
         graph.reset();
         stepg.reset();
         if (stepbuf.p) (void)hipFree(stepbuf.p);
+        for (void *p : pe_bufs) (void)hipFree(p);
+        for (auto *b : {&edgebuf, &ecdfbuf, &lightbuf, &refitbuf, &shapecdfbuf}) if (b->p) (void)hipFree(b->p);
         for (auto &b : bufs) if (b.p) (void)hipFree(b.p);
         for (auto *b : {&acc, &acc_pe, &queue, &qcount, &stats, &work, &outbuf, &ovf[0], &ovf[1], &viewbuf, &tunebuf})
             if (b->p) (void)hipFree(b->p);
@@ -660,6 +677,7 @@ struct Scene : Obj {
     DevBuf acc, acc_pe, queue, qcount, stats, work, outbuf, ovf[2], viewbuf;
     std::vector<DMedium> hmed;   // host copy of the device media records
     struct PECache { int npe = 0; double pe_sum = 0; const double *cdf = nullptr; const float *pe = nullptr; };
+    std::vector<void *> pe_bufs;   // device buffers of the primary-edge cache (freed when geometry changes)
     std::vector<uint64_t> last_chunks;   // AD chunk sizes of the last eager renderViews
     uint64_t last_replayed = 0;
     // A captured launch sequence of renderViews (fixed chunk schedule), replayed with new seeds.
@@ -790,6 +808,16 @@ struct Scene : Obj {
         }
     } stepg;
     DevBuf stepbuf;   // the step's d loss / d I images and its adjoint view table
+    // dtrt_scene_update_shape: new vertex positions (and optionally normals and tangents)
+    // of one shape; face normals, edges and light CDFs are recomputed on the host, the
+    // triangle records and BVH boxes are rebuilt on the GPU over the existing tree.
+    bool updateShape(int si, const float *v, const float *n, const float *vel);
+    bool uploadEdges();    // edges, edge CDFs and per-shape edge ranges of all shapes
+    bool uploadLights();   // light CDFs, pmf and areas (scene.cpp:45-98)
+    void dropPrimaryEdges();
+    DevBuf edgebuf, ecdfbuf, lightbuf, refitbuf, shapecdfbuf;
+    std::vector<int> refit_level_end;   // node ranges of refitbuf per depth, deepest first
+    std::vector<float> last_edge_weights;   // the weights of the last initEdges call (empty: none)
     bool renderPrimal(const dtrt_render_options &o, float *out, bool out_is_device, int integrator);
     bool autotune(const dtrt_render_options &o, int ad_kind, double *best_ms, int64_t *settings3);
     bool setDensity(int med, const float *density, bool on_device);
@@ -922,7 +950,14 @@ bool Scene::build() {
     }
     S.media = upload(dm);
     hmed = dm;
-    // lights (scene.cpp:45-98)
+    if (!uploadLights()) return false;
+    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
+    return initEdges(nullptr);
+}
+
+// scene.cpp:45-98 — light pmf proportional to area * luminance * pi, per light a
+// triangle CDF proportional to area; one device buffer, rewritten when geometry changes.
+bool Scene::uploadLights() {
     int nl = (int)emitters.size();
     std::vector<double> lcdf(nl), pool;
     std::vector<float> lpmf(nl), lareas(nl);
@@ -949,17 +984,153 @@ bool Scene::build() {
     if (nl > 0 && !(tot > 0)) return fail("scene: total light importance must be positive");
     double run = 0;
     for (int l = 0; l < nl; ++l) { pmf[l] /= tot; lpmf[l] = (float)pmf[l]; run += pmf[l]; lcdf[l] = run; }
-    S.light_cdf = upload(lcdf);
-    S.light_pmf = upload(lpmf);
-    S.light_areas = upload(lareas);
-    S.area_cdf_begin = upload(abeg);
-    S.area_cdf_pool = upload(pool);
-    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
-    return initEdges(nullptr);
+    // one buffer: [light cdf | area cdf pool] doubles, then [pmf | areas] floats, then begins
+    const size_t nd = lcdf.size() + pool.size(), bytes = nd * 8 + (size_t)nl * 8 + (size_t)nl * 4 + 64;
+    if (!ensure(lightbuf, bytes)) return false;
+    char *b = static_cast<char *>(lightbuf.p);
+    std::vector<char> host(bytes, 0);
+    std::memcpy(host.data(), lcdf.data(), lcdf.size() * 8);
+    std::memcpy(host.data() + lcdf.size() * 8, pool.data(), pool.size() * 8);
+    std::memcpy(host.data() + nd * 8, lpmf.data(), (size_t)nl * 4);
+    std::memcpy(host.data() + nd * 8 + (size_t)nl * 4, lareas.data(), (size_t)nl * 4);
+    std::memcpy(host.data() + nd * 8 + (size_t)nl * 8, abeg.data(), (size_t)nl * 4);
+    HIPCHK(hipStreamSynchronize(stream));   // no kernel of this scene is reading the old values
+    HIPCHK(hipMemcpy(b, host.data(), bytes, hipMemcpyHostToDevice));
+    S.light_cdf = reinterpret_cast<const double *>(b);
+    S.area_cdf_pool = reinterpret_cast<const double *>(b + lcdf.size() * 8);
+    S.light_pmf = reinterpret_cast<const float *>(b + nd * 8);
+    S.light_areas = reinterpret_cast<const float *>(b + nd * 8 + (size_t)nl * 4);
+    S.area_cdf_begin = reinterpret_cast<const int *>(b + nd * 8 + (size_t)nl * 8);
+    return true;
+}
+
+// Edges of all shapes (global vertex and face ids), their per-shape CDFs (ne + 1
+// entries at ebase + shape index) and each shape's edge range in the shape records.
+bool Scene::uploadEdges() {
+    std::vector<int4> edges;
+    std::vector<double> ecdf;
+    std::vector<DShape> dsh(shapes.size());
+    HIPCHK(hipMemcpy(dsh.data(), S.shapes, dsh.size() * sizeof(DShape), hipMemcpyDeviceToHost));
+    int ebase = 0;
+    for (size_t si = 0; si < shapes.size(); ++si) {
+        const Shape &s = *shapes[si];
+        dsh[si].ebase = ebase;
+        dsh[si].ne = (int)s.edges.size();
+        for (const EdgeH &e : s.edges)
+            edges.push_back(int4{e.v0 + dsh[si].vbase, e.v1 + dsh[si].vbase, e.f0 + dsh[si].fbase,
+                                 e.f1 >= 0 ? e.f1 + dsh[si].fbase : -1});
+        ecdf.insert(ecdf.end(), s.edge_cdf.begin(), s.edge_cdf.end());
+        ebase += (int)s.edges.size();
+    }
+    if (!ensure(edgebuf, std::max<size_t>(1, edges.size()) * sizeof(int4)) ||
+        !ensure(ecdfbuf, std::max<size_t>(1, ecdf.size()) * sizeof(double)))
+        return false;
+    HIPCHK(hipStreamSynchronize(stream));
+    if (!edges.empty()) HIPCHK(hipMemcpy(edgebuf.p, edges.data(), edges.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ecdfbuf.p, ecdf.data(), ecdf.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(const_cast<DShape *>(S.shapes), dsh.data(), dsh.size() * sizeof(DShape), hipMemcpyHostToDevice));
+    S.edges = static_cast<const int4 *>(edgebuf.p);
+    S.edge_cdf = static_cast<const double *>(ecdfbuf.p);
+    return true;
+}
+
+void Scene::dropPrimaryEdges() {
+    (void)hipStreamSynchronize(stream);
+    for (void *p : pe_bufs) (void)hipFree(p);
+    pe_bufs.clear();
+    pecache.clear();
+}
+
+bool Scene::updateShape(int si, const float *v, const float *n, const float *vel) {
+    if (si < 0 || si >= (int)shapes.size() || !v) return fail("update_shape: bad shape index or null vertices");
+    Shape &sh = *shapes[si];
+    if (n && !sh.hasNormals()) return fail("update_shape: the shape was created without normals");
+    // host geometry, as the Shape constructor builds it (shape.cpp:10-126, 254-300)
+    for (int i = 0; i < sh.nv; ++i) sh.v[i] = D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    if (n)
+        for (int i = 0; i < sh.nv; ++i) {
+            D3 q{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+            sh.n[i] = q * (1.0 / norm3(q));
+        }
+    if (vel) setShapeVelocities(sh, vel);
+    computeFaceNormals(sh);
+    if (!constructEdges(sh)) return false;
+    // device copies of this shape's vertex, normal and face records
+    DShape d;
+    HIPCHK(hipMemcpy(&d, S.shapes + si, sizeof(DShape), hipMemcpyDeviceToHost));
+    std::vector<float> V, Vd, Nrm, Nd, FN, FNd;
+    for (int i = 0; i < sh.nv; ++i) {
+        V.insert(V.end(), {(float)sh.v[i].x, (float)sh.v[i].y, (float)sh.v[i].z});
+        D3 nn = sh.hasNormals() ? sh.n[i] : D3{0, 0, 0};
+        Nrm.insert(Nrm.end(), {(float)nn.x, (float)nn.y, (float)nn.z});
+        for (int j = 0; j < nder; ++j) {
+            const D3 &g = sh.vd[(size_t)i * nder + j];
+            Vd.insert(Vd.end(), {(float)g.x, (float)g.y, (float)g.z});
+            D3 h = sh.hasNormals() ? sh.nd[(size_t)i * nder + j] : D3{0, 0, 0};
+            Nd.insert(Nd.end(), {(float)h.x, (float)h.y, (float)h.z});
+        }
+    }
+    for (int k = 0; k < sh.nf; ++k) {
+        FN.insert(FN.end(), {(float)sh.fn[k].x, (float)sh.fn[k].y, (float)sh.fn[k].z});
+        for (int j = 0; j < nder; ++j) {
+            const D3 &g = sh.fnd[(size_t)k * nder + j];
+            FNd.insert(FNd.end(), {(float)g.x, (float)g.y, (float)g.z});
+        }
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    auto put = [&](const float *dst, const std::vector<float> &src, size_t first) -> hipError_t {
+        return hipMemcpy(const_cast<float *>(dst) + first, src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice);
+    };
+    HIPCHK(put(S.V, V, (size_t)d.vbase * 3));
+    HIPCHK(put(S.Vd, Vd, (size_t)d.vbase * nder * 3));
+    HIPCHK(put(S.Nrm, Nrm, (size_t)d.vbase * 3));
+    HIPCHK(put(S.Nd, Nd, (size_t)d.vbase * nder * 3));
+    HIPCHK(put(S.FN, FN, (size_t)d.fbase * 3));
+    HIPCHK(put(S.FNd, FNd, (size_t)d.fbase * nder * 3));
+    if (!uploadEdges() || !initEdges(last_edge_weights.empty() ? nullptr : last_edge_weights.data()) || !uploadLights())
+        return false;
+    // triangle records in leaf order, then node boxes level by level from the deepest
+    if (refit_level_end.empty()) {
+        std::vector<int> depth(S.nnodes, -1), order;
+        std::vector<float4> nodes((size_t)S.nnodes * 4);
+        HIPCHK(hipMemcpy(nodes.data(), S.nodes, nodes.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        std::vector<std::vector<int>> levels;
+        std::vector<int> q{0};
+        depth[0] = 0;
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int nd_ = q[h];
+            if ((int)levels.size() <= depth[nd_]) levels.resize(depth[nd_] + 1);
+            levels[depth[nd_]].push_back(nd_);
+            for (int k = 0; k < 2; ++k) {
+                int link;
+                std::memcpy(&link, &nodes[4 * (size_t)nd_ + k].w, 4);
+                if (link >= 0) { depth[link] = depth[nd_] + 1; q.push_back(link); }
+            }
+        }
+        for (int l = (int)levels.size() - 1; l >= 0; --l) {
+            order.insert(order.end(), levels[l].begin(), levels[l].end());
+            refit_level_end.push_back((int)order.size());
+        }
+        if (!ensure(refitbuf, order.size() * sizeof(int))) return false;
+        HIPCHK(hipMemcpy(refitbuf.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIPCHK(launch_tri_records(S, stream));
+    int begin = 0;
+    for (int end : refit_level_end) {
+        HIPCHK(launch_refit(S, static_cast<const int *>(refitbuf.p) + begin, end - begin, stream));
+        begin = end;
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    // what was built from the old geometry: projected primary edges and captured graphs
+    dropPrimaryEdges();
+    graph.reset();
+    stepg.reset();
+    return true;
 }
 
 // bruteforce.cpp:4-20 and edge_manager.cpp:6-16
 bool Scene::initEdges(const float *weights) {
+    if (weights && weights != last_edge_weights.data()) last_edge_weights.assign(weights, weights + shapes.size());
     std::vector<double> w(shapes.size() + 1, 0.0);
     double s = 0;
     for (size_t i = 0; i < shapes.size(); ++i) {
@@ -971,8 +1142,10 @@ bool Scene::initEdges(const float *weights) {
     }
     S.edges_valid = s > 1e-7;
     if (S.edges_valid) { for (double &x : w) x /= s; w.back() = 1.0; }
-    S.shape_edge_cdf = upload(w);
-    for (const auto &b : bufs) if (!b.p) return fail("scene: device allocation failed");
+    if (!ensure(shapecdfbuf, w.size() * sizeof(double))) return false;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(shapecdfbuf.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    S.shape_edge_cdf = static_cast<const double *>(shapecdfbuf.p);
     return true;
 }
 
@@ -985,8 +1158,15 @@ bool Scene::primaryEdges(const Camera &cam, PECache &out) {
     std::vector<double> cdf;
     std::vector<float> packed;
     projectPrimaryEdges(cam, cdf, packed, c.npe, c.pe_sum);
-    c.cdf = upload(cdf);
-    c.pe = packed.empty() ? nullptr : upload(packed);
+    auto put = [&](const void *src, size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) return nullptr;
+        pe_bufs.push_back(p);
+        if (bytes && hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return p;
+    };
+    c.cdf = static_cast<const double *>(put(cdf.data(), cdf.size() * sizeof(double)));
+    c.pe = packed.empty() ? nullptr : static_cast<const float *>(put(packed.data(), packed.size() * sizeof(float)));
     if (!c.cdf || (!packed.empty() && !c.pe)) return fail("scene: device allocation failed");
     pecache[cam.uid] = c;
     out = c;
@@ -2017,16 +2197,7 @@ dtrt_object *dtrt_shape_create(const float *v, const int *idx, const float *uvs,
         }
         s->vd.assign((size_t)nv * nd, D3{0, 0, 0});
         s->nd.assign(n ? (size_t)nv * nd : 0, D3{0, 0, 0});
-        if (vel)
-            for (int j = 0; j < nd; ++j)
-                for (int i = 0; i < nv; ++i) {
-                    const float *p = vel + ((size_t)j * nv + i) * 3;
-                    s->vd[(size_t)i * nd + j] = D3{p[0], p[1], p[2]};
-                    if (n) {   // shape.cpp:45 reads the same normal block for every lane
-                        const float *q = vel + (size_t)nv * 3 * nd + (size_t)i * 3;
-                        s->nd[(size_t)i * nd + j] = D3{q[0], q[1], q[2]};
-                    }
-                }
+        if (vel) setShapeVelocities(*s, vel);
         computeFaceNormals(*s);
         if (!constructEdges(*s)) return nullptr;
         return handle(s.release());
@@ -2283,6 +2454,13 @@ int dtrt_scene_replay_step(dtrt_object *scene, const uint64_t *seeds) {
     Scene *s = as<Scene>(scene, T_SCENE);
     if (!s || !seeds) { if (s) g_err = "replay_step: bad arguments"; return -1; }
     return guarded<int>(-1, [&]() -> int { return s->replayStep(seeds); });
+}
+
+int dtrt_scene_update_shape(dtrt_object *scene, int shape, const float *vertices, const float *normals,
+                            const float *velocities) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s) return -1;
+    return guarded<int>(-1, [&]() -> int { return s->updateShape(shape, vertices, normals, velocities) ? 0 : -1; });
 }
 
 int dtrt_scene_replay_views(dtrt_object *scene, const uint64_t *seeds) {

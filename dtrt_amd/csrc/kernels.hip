@@ -15,6 +15,7 @@
 #include "launch.h"
 
 #include <algorithm>
+#include <cfloat>
 
 // The file is compiled once per dual width (-DDT_N=1,2,3,4,8,16: that width's AD,
 // child, primary-edge, adjoint and finalize kernels and their launchers) and once
@@ -956,6 +957,53 @@ __global__ void k_add_u32(unsigned *dst, const unsigned *src) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *dst += *src;
 }
 
+// Triangle records in leaf order from the current vertex positions: for leaf-order
+// slot i of face f (kept in v0.w), v0 | f, e1 = v1 - v0, e2 = v2 - v0 (the BVH build's
+// records, rebuilt after a shape's vertices change).
+__global__ void k_tri_records(float4 *tris, const int *F, const float *V, int ntris) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntris) return;
+    float4 *t = tris + 3 * (size_t)i;
+    const float4 old = t[0];
+    const int f = __float_as_int(old.w);
+    const int *fi = F + 3 * (size_t)f;
+    const float *a = V + 3 * (size_t)fi[0], *b = V + 3 * (size_t)fi[1], *c = V + 3 * (size_t)fi[2];
+    t[0] = make_float4(a[0], a[1], a[2], old.w);
+    t[1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
+    t[2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+}
+
+// Refit of one tree level: each listed node's two child boxes from their contents, a
+// leaf child from its triangles' vertices, an inner child from its own two (already
+// refitted) child boxes. Links and the tree's shape stay; levels run deepest first.
+__global__ void k_refit(float4 *nodes, const float4 *tris, const int *F, const float *V, const int *list, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 *nd = nodes + 4 * (size_t)list[i];
+    for (int k = 0; k < 2; ++k) {
+        const int link = __float_as_int(k == 0 ? nd[0].w : nd[1].w);
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (link >= 0) {
+            const float4 *c = nodes + 4 * (size_t)link;
+            lo[0] = fminf(c[0].x, c[2].x); lo[1] = fminf(c[0].y, c[2].y); lo[2] = fminf(c[0].z, c[2].z);
+            hi[0] = fmaxf(c[1].x, c[3].x); hi[1] = fmaxf(c[1].y, c[3].y); hi[2] = fmaxf(c[1].z, c[3].z);
+        } else {
+            const int enc = ~link, first = enc >> 4, cnt = enc & 15;
+            if (cnt == 0) continue;   // the empty slot of a single-leaf root keeps its box
+            for (int t = first; t < first + cnt; ++t) {
+                const int *fi = F + 3 * (size_t)__float_as_int(tris[3 * (size_t)t].w);
+                for (int q = 0; q < 3; ++q) {
+                    const float *p = V + 3 * (size_t)fi[q];
+                    for (int a = 0; a < 3; ++a) { lo[a] = fminf(lo[a], p[a]); hi[a] = fmaxf(hi[a], p[a]); }
+                }
+            }
+        }
+        float4 &L = nd[2 * k], &H = nd[2 * k + 1];
+        L = make_float4(lo[0], lo[1], lo[2], L.w);
+        H = make_float4(hi[0], hi[1], hi[2], H.w);
+    }
+}
+
 // One view of an L2 image loss sum (I - T)^2 over the value channel: dimg = 2 (I - T),
 // the loss added into *loss (one double atomic per wave).
 __global__ void k_l2_grad(const float *img, const float *target, float *dimg, uint64_t n, double *loss) {
@@ -1488,6 +1536,18 @@ hipError_t launch_fill32(void *p, uint64_t n, uint32_t v, hipStream_t sm) {
 }
 hipError_t launch_add_u32(unsigned *dst, const unsigned *src, hipStream_t sm) {
     hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, sm, dst, src);
+    return hipGetLastError();
+}
+hipError_t launch_tri_records(const DScene &S, hipStream_t sm) {
+    if (S.ntris == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tri_records, dim3((S.ntris + 255) / 256), dim3(256), 0, sm, const_cast<float4 *>(S.tris), S.F, S.V,
+                       S.ntris);
+    return hipGetLastError();
+}
+hipError_t launch_refit(const DScene &S, const int *list, int n, hipStream_t sm) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, sm, const_cast<float4 *>(S.nodes), S.tris, S.F, S.V,
+                       list, n);
     return hipGetLastError();
 }
 hipError_t launch_l2_grad(const float *img, const float *target, float *dimg, uint64_t n, double *loss, hipStream_t sm) {

@@ -57,6 +57,8 @@ constexpr double PRIMARY_DELTA = 1e-4;
 // launch_l2_grad: one view of the L2 loss sum (I - T)^2: dimg = 2 (I - T) over n floats,
 // the loss added into *loss. launch_contract: grad[k] += sum dimg * (tangent image k)
 // for a view image [(nder + 1)][n] (forward-mode lanes contracted with d loss / d I).
+// launch_tri_records: the leaf-order triangle records from S.V; launch_refit: the boxes of
+// the n nodes listed (one tree level, children refitted first).
 // launch_primal_paths: samples s0 .. s0+n-1 of view 0, radiance sums into acc[pixel][3].
 // launch_particles: ParticleTracer particles s0 .. s0+n-1 splatted into acc[pixel][3]
 // of view 0's window.
@@ -81,6 +83,8 @@ constexpr double PRIMARY_DELTA = 1e-4;
     hipError_t launch_l2_grad(const float *img, const float *target, float *dimg, uint64_t n, double *loss,             \
                               hipStream_t sm);                                                                           \
     hipError_t launch_contract(const float *img, const float *dimg, uint64_t n, int nder, double *grad, hipStream_t sm); \
+    hipError_t launch_tri_records(const DScene &S, hipStream_t sm);                                                     \
+    hipError_t launch_refit(const DScene &S, const int *list, int n, hipStream_t sm);                                   \
     hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,  \
                                    unsigned long long *work, hipStream_t sm);                                            \
     hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, void *acc,             \

@@ -242,6 +242,21 @@ def replay_step(scene, seeds):
     return rc
 
 
+_lib.dtrt_scene_update_shape.argtypes = [_P, C.c_int, _P, _P, _P]
+
+
+def update_shape(scene, shape, vertices, normals=None, velocities=None):
+    """``dtrt_scene_update_shape``: new vertex positions [nv, 3] (and optionally normals and
+    tangents in the Shape constructor's layouts) for shape `shape`, with a GPU BVH refit."""
+    import numpy as np
+
+    def host(x):
+        return None if x is None else np.ascontiguousarray(x, dtype=np.float32)
+    v, n, vel = host(vertices), host(normals), host(velocities)
+    _chk(_lib.dtrt_scene_update_shape(scene._h, int(shape), v.ctypes.data, None if n is None else n.ctypes.data,
+                                      None if vel is None else vel.ctypes.data), "update_shape")
+
+
 def set_density(scene, medium, density):
     """``dtrt_scene_set_density``: a torch tensor (host or device) or numpy array [rz][ry][rx]."""
     import numpy as np

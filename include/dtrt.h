@@ -210,6 +210,19 @@ int dtrt_render_views_adjoint(int integrator, dtrt_object *scene, int nviews, dt
 int dtrt_render_adjoint(int integrator, dtrt_object *scene, const dtrt_render_options *options, int medium,
                         const float *d_image_device, float *d_density_device);
 
+/* No pybind counterpart: the reference builds a new Scene (and Embree BVH, scene.cpp:
+ * 17-43) for every optimisation step. New vertex positions for shape `shape` of the
+ * scene (host, [nv][3], the constructor's layout), optionally new normals ([nv][3], only
+ * for shapes created with normals) and new tangents (the constructor's velocity layout),
+ * NULL keeping the old ones. Face normals, edges (crease/boundary classification and
+ * lengths), edge CDFs with the last initEdges weights and light CDFs are recomputed on
+ * the host; the triangle records and every BVH box are rebuilt on the GPU over the
+ * existing tree (a refit: same topology, so traversal stays exact but may slow if the
+ * shape moves far). Also updates the Shape object. Primary-edge projections are
+ * dropped and captured graphs (views, steps) released: capture again after. */
+int dtrt_scene_update_shape(dtrt_object *scene, int shape, const float *vertices, const float *normals,
+                            const float *velocities);
+
 /* No pybind counterpart (the batched inverse-rendering step; the reference's examples
  * run it as a Python loop of render, loss and backward calls, e.g.
  * examples/glass_1/optimize.py:51-68). One step of an L2 image loss
