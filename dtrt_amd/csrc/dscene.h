@@ -30,11 +30,12 @@ DI f3 xform33(const float *m, f3 p) {
     return f3{m[0] * p.x + m[1] * p.y + m[2] * p.z, m[4] * p.x + m[5] * p.y + m[6] * p.z, m[8] * p.x + m[9] * p.y + m[10] * p.z};
 }
 
-// Per-thread traversal stack in LDS, laid out [depth][lane] so that the lanes of a
-// wave touch consecutive banks. Every kernel that traces rays runs blocks of
-// exactly TRACE_BLOCK threads.
-constexpr int TRACE_BLOCK = 64;
-__shared__ int s_bvh_stack[DTRT_BVH_STACK * TRACE_BLOCK];
+// Per-thread traversal stack in LDS, laid out [depth][thread of the block] so that the
+// lanes of a wave touch consecutive banks. Its size follows the work-group size: every
+// kernel that traces rays is launched with traceLds(block) bytes of dynamic LDS.
+constexpr int TRACE_BLOCK = 64;   // the default work-group size (one wave)
+constexpr size_t traceLds(unsigned block) { return (size_t)DTRT_BVH_STACK * block * sizeof(int); }
+extern __shared__ int s_bvh_stack[];
 
 // ---------------------------------------------------------------- BVH (fp32)
 struct fv3 {
@@ -71,6 +72,7 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
     TraceHit res{-1, tfar, 0u, 0u};
     if (nnodes == 0) return res;
     int *stk = s_bvh_stack + threadIdx.x;
+    const unsigned stride = blockDim.x;
     const float BEPS = 1e-6f;
     const fv3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float best = tfar;
@@ -89,7 +91,7 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
             int cl = __float_as_int(a.w), cr = __float_as_int(b.w);
             if (hl && hr) {
                 if (tr < tl) { int t = cl; cl = cr; cr = t; }
-                stk[TRACE_BLOCK * sp++] = cr;
+                stk[stride * sp++] = cr;
                 node = cl;
                 continue;
             } else if (hl) {
@@ -122,7 +124,7 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
             }
         }
         if (sp == 0) break;
-        node = stk[TRACE_BLOCK * --sp];
+        node = stk[stride * --sp];
     }
     res.face = face;
     res.t = best;

@@ -722,6 +722,7 @@ struct Scene : Obj {
     int tune_child_blocks = 32;
     double tune_fill = 0.75;
     uint64_t tune_max_chunk = uint64_t(1) << 22;
+    int tune_ad_block = 64, tune_child_block = 64;   // work-group sizes of the AD-path and child kernels
     DevBuf tunebuf;
 
     template <class T> const T *upload(const std::vector<T> &v) {
@@ -1294,6 +1295,8 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
+    P.ad_block = tune_ad_block;
+    P.child_block = tune_child_block;
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total, pool)) return false;
@@ -1483,6 +1486,8 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
     P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
+    P.ad_block = tune_ad_block;
+    P.child_block = tune_child_block;
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
@@ -1658,6 +1663,8 @@ bool Scene::captureStep(StepArgs &a) {
     P.max_bounces = o0.max_bounces; P.sppse = std::max(0, o0.sppse);
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
+    P.ad_block = tune_ad_block;
+    P.child_block = tune_child_block;
     std::vector<DView> hvf, hva;
     std::vector<uint64_t> pe_total, pe_unused;
     if (!setupViews(nv, cp.data(), a.fopts.data(), hvf, pe_total)) return false;
@@ -1969,6 +1976,23 @@ bool Scene::autotune(const dtrt_render_options &o, int ad_kind, double *best_ms,
     const int blocks[] = {16, 24, 32, 48};
     const double fills[] = {0.5, 0.75, 0.9};
     const uint64_t chunks[] = {uint64_t(1) << 21, uint64_t(1) << 22, uint64_t(1) << 23};
+    const size_t budgets[] = {size_t(1) << 30, size_t(2) << 30, size_t(4) << 30, size_t(6) << 30};
+    const int groups[] = {64, 128, 256};
+    auto tryKnob = [&](auto &knob, auto value) -> bool {   // keep `value` if it renders faster
+        if (knob == value) return true;
+        const auto keep = knob;
+        knob = value;
+        double ms;
+        if (!run(ms)) return false;
+        if (ms < best) best = ms; else knob = keep;
+        return true;
+    };
+    for (size_t q : budgets)
+        if (!tryKnob(queue_budget, q)) return false;
+    for (int g : groups)
+        if (!tryKnob(tune_ad_block, g)) return false;
+    for (int g : groups)
+        if (!tryKnob(tune_child_block, g)) return false;
     for (int b : blocks) {
         if (b == tune_child_blocks) continue;
         const int keep = tune_child_blocks;
@@ -2518,6 +2542,34 @@ int dtrt_render_stats(dtrt_object *scene, int64_t *out) {
 int dtrt_scene_precision(dtrt_object *scene) {
     Scene *s = as<Scene>(scene, T_SCENE);
     return s ? (s->S.f64 ? 64 : 32) : -1;
+}
+int dtrt_scene_launch_settings(dtrt_object *scene, int64_t *out6) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !out6) return -1;
+    out6[0] = s->tune_child_blocks;
+    out6[1] = (int64_t)std::lround(s->tune_fill * 100.0);
+    out6[2] = (int64_t)s->tune_max_chunk;
+    out6[3] = (int64_t)s->queue_budget;
+    out6[4] = s->tune_ad_block;
+    out6[5] = s->tune_child_block;
+    return 0;
+}
+int dtrt_scene_set_launch_settings(dtrt_object *scene, const int64_t *in6) {
+    Scene *s = as<Scene>(scene, T_SCENE);
+    if (!s || !in6) return -1;
+    auto group = [](int64_t g) { return g == 64 || g == 128 || g == 256; };
+    if (in6[0] < 1 || in6[0] > 1024 || in6[1] < 1 || in6[1] > 100 || in6[2] < 64 || in6[3] < (int64_t(1) << 20) ||
+        !group(in6[4]) || !group(in6[5])) {
+        g_err = "set_launch_settings: out of range (work-group sizes must be 64, 128 or 256)";
+        return -1;
+    }
+    s->tune_child_blocks = (int)in6[0];
+    s->tune_fill = in6[1] / 100.0;
+    s->tune_max_chunk = (uint64_t)in6[2];
+    s->queue_budget = (size_t)in6[3];
+    s->tune_ad_block = (int)in6[4];
+    s->tune_child_block = (int)in6[5];
+    return 0;
 }
 int dtrt_scene_set_stream(dtrt_object *scene, void *stream) {
     Scene *s = as<Scene>(scene, T_SCENE);

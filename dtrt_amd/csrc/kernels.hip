@@ -41,6 +41,8 @@ DI real edgeAngle(f3 e0, f3 e1, f3 c01, real l0, real l1) {
 __constant__ FrameRot DT_FRAME_ROT;
 
 constexpr int BLOCK = TRACE_BLOCK;
+constexpr int MAX_BLOCK = 256;   // largest work-group size of the AD-path and child kernels
+HDI unsigned blockOr64(int b) { return b == 128 || b == 256 ? (unsigned)b : 64u; }
 // Minimum resident waves per SIMD requested from the register allocator (HIP's second
 // __launch_bounds__ argument): bounds the VGPRs of the two path kernels. Per
 // precision: a double takes two registers.
@@ -514,12 +516,12 @@ DI void adPathSample(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &
 // {sid lo, sid hi, children delivered, 0} of samples whose pushes overflowed.
 // Samples that overflow (again) are appended to ovf.
 template <int N, int I>   // I: AdKind
-__global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
+__global__ __launch_bounds__(MAX_BLOCK, adWaves<N>()) void k_ad_paths(DScene S, RenderParams P, uint64_t s0, uint64_t nsamp,
                                                     const uint4 *replay, real *acc, char *q, unsigned *qcount,
                                                     unsigned *stats, uint4 *ovf, unsigned *ovf_count,
                                                     unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
-    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    const uint64_t li = (uint64_t)blk * blockDim.x + threadIdx.x;
     Tracer<N> T(S);
     Ds<N> L;
     const bool active = li < nsamp;
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_paths(DScene S, Rend
 #pragma unroll
         for (int k = 0; k < 3 * (N + 1); ++k)
             for (int off = 32; off > 0; off >>= 1) sum[k] += __shfl_xor(sum[k], off);
-        if (threadIdx.x == 0 && active)
+        if ((threadIdx.x & 63) == 0 && active)
             for (int k = 0; k < 3 * (N + 1); ++k)
                 if (sum[k] != 0._r) unsafeAtomicAdd(acc + ((size_t)(k / 3) * nd + gpix) * 3 + (k % 3), sum[k]);
     } else if (put) {
@@ -1109,7 +1111,7 @@ DI void scatterFinished(const RenderParams &P, const char *q, real *acc, real *a
 // iteration; a lane whose path has ended takes the next queued record (one atomic
 // per wave on qnext), so the lanes of a wave stay busy until the queue is drained.
 template <int N>
-__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
+__global__ __launch_bounds__(MAX_BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
                                                     unsigned *qnext, real *acc, real *acc_pe, unsigned long long *work) {
     Tracer<N> T(S);
     const unsigned n = min(*qcount, P.qcap);
@@ -1173,7 +1175,7 @@ __global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene
 // Persistent drain of CHILD_PATH records (the surface-only integrators' sub-paths,
 // path.cpp:11-61 from the record's origin and direction), as k_children.
 template <int N>
-__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_children_path(DScene S, RenderParams P, const char *q,
+__global__ __launch_bounds__(MAX_BLOCK, CHILD_WAVES_PER_SIMD) void k_children_path(DScene S, RenderParams P, const char *q,
                                                                             const unsigned *qcount, unsigned *qnext,
                                                                             real *acc, real *acc_pe,
                                                                             unsigned long long *work) {
@@ -1436,40 +1438,44 @@ template <int N> hipError_t WidthOps<N>::setFrameRotation(const double *m9) {
 template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,
                                       real *acc, char *q, unsigned *qc, unsigned *st, uint4 *ovf, unsigned *ovf_count,
                                       unsigned long long *wk, hipStream_t sm) {
-    unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
+    const unsigned blk = blockOr64(P.ad_block);
+    unsigned nb = (unsigned)((n + blk - 1) / blk);
     if (P.ad_kind == AD_PATH)
-        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf,
-                           ovf_count, wk);
+        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(blk), traceLds(blk), sm, S, P, s0, n, replay, acc, q, qc,
+                           st, ovf, ovf_count, wk);
     else if (P.ad_kind == AD_DIRECT)
-        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st, ovf,
-                           ovf_count, wk);
+        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(blk), traceLds(blk), sm, S, P, s0, n, replay, acc, q,
+                           qc, st, ovf, ovf_count, wk);
     else if constexpr (adRefill<N>()) {
         // one wave per block, AD_REFILL_WAVES_PER_CU of them per CU share the samples
         int dev = 0, ncu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         const uint64_t waves = (uint64_t)std::max(ncu, 1) * AD_REFILL_WAVES_PER_CU;
-        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(nb, waves));
+        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, waves));
         const uint64_t per_wave = (n + grid - 1) / grid;
-        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), 0, sm, S, P, s0, n, per_wave, replay, acc, q,
-                           qc, st, ovf, ovf_count, wk);
+        // one wave per block: each owns a contiguous range of samples
+        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, per_wave,
+                           replay, acc, q, qc, st, ovf, ovf_count, wk);
     } else {
-        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, replay, acc, q, qc, st,
-                           ovf, ovf_count, wk);
+        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(blk), traceLds(blk), sm, S, P, s0, n, replay, acc, q,
+                           qc, st, ovf, ovf_count, wk);
     }
 }
 template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
                                             real *acc, real *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
+    const unsigned blk = blockOr64(P.child_block);   // grid: blocks of `blk` threads
     if (P.ad_kind == AD_VOLPATH)
-        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
+        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(blk), traceLds(blk), sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
     else
-        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(BLOCK), 0, sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
+        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(blk), traceLds(blk), sm, S, P, q, qc, qc + 2, acc, acc_pe,
+                           wk);
 }
 template <int N> void WidthOps<N>::launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,
                                            uint64_t total, char *q, unsigned *qc, unsigned *st, unsigned long long *wk,
                                            hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, view, s0, n, total, q, qc, st, wk);
+    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, view, s0, n, total, q, qc, st, wk);
 }
 template <int N> void WidthOps<N>::launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const real *acc,
                                             const real *acc_pe, float *out, unsigned *st, hipStream_t sm) {
@@ -1483,7 +1489,7 @@ template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderPa
                                                  const float *dimg, float *grad, int pmed, unsigned *st,
                                                  unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), 0, sm, S, P, s0, n, dimg, grad, pmed, st, wk);
+    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, dimg, grad, pmed, st, wk);
 }
 template struct WidthOps<DT_N>;
 
@@ -1574,7 +1580,7 @@ hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float 
     DT_DISPATCH(launchGrid, S, med, pts, n, out, sm)
 }
 hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, float *t, hipStream_t sm) {
-    hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, face, t);
+    hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), traceLds(BLOCK), sm, S, rays, n, face, t);
     return hipGetLastError();
 }
 hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,
@@ -1583,11 +1589,14 @@ hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind,
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
     real *A = static_cast<real *>(acc);
     if (kind == PRIMAL_SIMPLE)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, A, work);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n,
+                           A, work);
     else if (kind == PRIMAL_PATH)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, A, work);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, A,
+                           work);
     else if (kind == PRIMAL_VOLPATH)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, A, work);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n,
+                           A, work);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1596,7 +1605,8 @@ hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0,
                             unsigned long long *work, hipStream_t sm) {
     const uint64_t nb = (n + BLOCK - 1) / BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(BLOCK), 0, sm, S, P, s0, n, static_cast<real *>(acc), work);
+    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n,
+                       static_cast<real *>(acc), work);
     return hipGetLastError();
 }
 hipError_t launch_scale(const DScene &S, const void *acc, float *out, uint64_t n, double scale, hipStream_t sm) {
@@ -1607,7 +1617,8 @@ hipError_t launch_scale(const DScene &S, const void *acc, float *out, uint64_t n
     return hipGetLastError();
 }
 hipError_t launch_li(const DScene &S, const float *rays, int n, int depth, int med, uint64_t seed, float *out, hipStream_t sm) {
-    hipLaunchKernelGGL(k_li, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, sm, S, rays, n, depth, med, seed, out);
+    hipLaunchKernelGGL(k_li, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), traceLds(BLOCK), sm, S, rays, n, depth, med, seed,
+                       out);
     return hipGetLastError();
 }
 hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out, hipStream_t sm) {

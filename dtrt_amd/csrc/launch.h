@@ -18,6 +18,7 @@ struct RenderParams {
     int npix;                // pixels of all views (accumulator stride)
     int rec_stride;          // bytes per child record (ChildHdr + 12*nder, 16-byte aligned)
     unsigned qcap;           // child records the queue can hold
+    int ad_block, child_block;   // work-group sizes of the AD-path and child kernels (64, 128, 256; 0 = 64)
 };
 
 // Counters written by the kernels (warnings the reference prints, as counts).

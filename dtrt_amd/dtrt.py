@@ -138,12 +138,33 @@ _lib.dtrt_scene_autotune.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].arg
 
 def autotune(scene, options, integrator=0):
     """``dtrt_scene_autotune``: picks and keeps the fastest launch settings for renders
-    like ``options``; returns (best ms, {"child_blocks_per_cu", "fill_percent", "max_chunk"})."""
+    like ``options``; returns (best ms, launch_settings(scene))."""
     st = options._struct()
     ms = C.c_double()
     s3 = (C.c_int64 * 3)()
     _chk(_lib.dtrt_scene_autotune(integrator, scene._h, C.byref(st), C.byref(ms), s3), "autotune")
-    return ms.value, {"child_blocks_per_cu": int(s3[0]), "fill_percent": int(s3[1]), "max_chunk": int(s3[2])}
+    return ms.value, launch_settings(scene)
+
+
+_lib.dtrt_scene_launch_settings.argtypes = [_P, C.POINTER(C.c_int64)]
+_lib.dtrt_scene_set_launch_settings.argtypes = [_P, C.POINTER(C.c_int64)]
+_SETTINGS = ("child_blocks_per_cu", "fill_percent", "max_chunk", "queue_budget_bytes", "ad_workgroup", "child_workgroup")
+
+
+def set_launch_settings(scene, **kw):
+    """``dtrt_scene_set_launch_settings``: change some of the scene's launch settings."""
+    cur = launch_settings(scene)
+    cur.update(kw)
+    s6 = (C.c_int64 * 6)(*[int(cur[k]) for k in _SETTINGS])
+    _chk(_lib.dtrt_scene_set_launch_settings(scene._h, s6), "set_launch_settings")
+
+
+def launch_settings(scene):
+    """``dtrt_scene_launch_settings``: the scene's current launch settings."""
+    s6 = (C.c_int64 * 6)()
+    _chk(_lib.dtrt_scene_launch_settings(scene._h, s6), "launch_settings")
+    return {"child_blocks_per_cu": int(s6[0]), "fill_percent": int(s6[1]), "max_chunk": int(s6[2]),
+            "queue_budget_bytes": int(s6[3]), "ad_workgroup": int(s6[4]), "child_workgroup": int(s6[5])}
 
 
 def render_views_adjoint(scene, cameras, options, medium, d_images, d_density, integrator=0):

@@ -149,14 +149,22 @@ int dtrt_render(int integrator, dtrt_object *scene, const dtrt_render_options *o
 int dtrt_render_device(int integrator, dtrt_object *scene, const dtrt_render_options *options, float *image_device);
 
 /* Autotuner (no counterpart in the reference): renders `options` with the scene's
- * camera several times, by coordinate descent over the launch settings (persistent
- * child blocks per CU, the queue share an AD chunk aims at, the largest AD chunk), and
+ * camera several times, by coordinate descent over the launch settings (child queue
+ * budget, work-group sizes of the AD-path and child kernels, persistent child blocks
+ * per CU, the queue share an AD chunk aims at, the largest AD chunk), and
  * keeps the fastest in the scene for later renders, view batches and graph captures.
  * integrator: 0, 6 or 7. best_ms (may be NULL): the fastest render; settings3 (may be
  * NULL): {blocks per CU, fill target in percent, max AD chunk}. Images and stats of
  * the last call are overwritten. */
 int dtrt_scene_autotune(int integrator, dtrt_object *scene, const dtrt_render_options *options, double *best_ms,
                         int64_t *settings3);
+/* The scene's launch settings, as the autotuner left them: out6 = {persistent child
+ * blocks per CU, AD chunk fill target in percent, largest AD chunk in samples, child
+ * queue budget in bytes, AD-path work-group size, child work-group size}. */
+int dtrt_scene_launch_settings(dtrt_object *scene, int64_t *out6);
+/* Sets them (same order), e.g. to reuse a tuned configuration; work-group sizes 64,
+ * 128 or 256. Settings change speed only: images are the same up to summation order. */
+int dtrt_scene_set_launch_settings(dtrt_object *scene, const int64_t *in6);
 
 /* Several views of one scene in one pass. View v renders cameras[v] (NULL entry or
  * NULL array: the scene's camera) with options[v] into images[v], laid out as for

@@ -194,7 +194,24 @@ def test_autotune_keeps_results_and_picks_listed_settings(gpu):
     assert ms > 0
     assert st["child_blocks_per_cu"] in (16, 24, 32, 48) and st["fill_percent"] in (50, 75, 90)
     assert st["max_chunk"] in (1 << 21, 1 << 22, 1 << 23)
+    assert st["queue_budget_bytes"] in (1 << 30, 2 << 30, 4 << 30, 6 << 30)
+    assert st["ad_workgroup"] in (64, 128, 256) and st["child_workgroup"] in (64, 128, 256)
     after = scenes.render(gpu, d, seed=9, scene=sc).astype(np.float64)
     scale = np.abs(before).reshape(before.shape[0], -1).max(axis=1)
     for lane in range(before.shape[0]):
         assert np.allclose(after[lane], before[lane], rtol=1e-3, atol=1e-4 * scale[lane]), lane
+
+
+@pytest.mark.parametrize("groups", [(128, 256), (256, 128), (256, 64)])
+def test_workgroup_sizes_render_the_same_image(gpu, groups):
+    """The AD-path and child kernels at other work-group sizes (the traversal stack in
+    LDS follows the block size) and a small queue budget: same seed, same image."""
+    d = scenes.config1(nder=3, res=48, spp=32, grid_res=16)
+    sc = scenes.instantiate(gpu, d)
+    ref = scenes.render(gpu, d, seed=5, scene=sc).astype(np.float64)
+    gpu.set_launch_settings(sc, ad_workgroup=groups[0], child_workgroup=groups[1], queue_budget_bytes=1 << 24)
+    got = scenes.render(gpu, d, seed=5, scene=sc).astype(np.float64)
+    assert gpu.render_stats(sc)["replayed"] > 0   # the small queue overflowed and was replayed
+    scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    for lane in range(ref.shape[0]):
+        assert np.allclose(got[lane], ref[lane], rtol=1e-3, atol=1e-4 * scale[lane]), lane
