@@ -74,5 +74,8 @@ hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float 
 hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm) {
     return p32::launch_philox(ctr, key, out, sm);
 }
+hipError_t kernel_info(const DScene &S, KernelInfo *out, int cap, int *count) {
+    return DT_BY_PREC(S, kernel_info, S, out, cap, count);
+}
 
 }  // namespace dt

@@ -857,6 +857,66 @@ bool frameRotationReady() {
     return true;
 }
 
+// Scratch limits of the kernels a scene launches. A kernel's private segment (register
+// spills, private arrays) lives in per-lane scratch that the runtime provisions for every
+// wave the device holds at once. The limits are the largest figures among the kernels the
+// GPU parity suite runs and checks, rounded up: 10,360 bytes per lane and 679 MB over the
+// device (the double-precision nder-16 refill kernel: 10,360 B x 64 lanes x 4 waves per
+// CU x 256 CUs). A width-16 build of the single-precision AD-path kernel held to two
+// waves per SIMD, whose spills needed 7.9 KB per lane (1.04 GB over the device), rendered
+// empty images: every path ended after two segments. Its cause is not known; per-lane
+// size alone is not it (the 10,360-byte kernel is correct), the device-wide figure is not
+// ruled out. So a kernel beyond either limit is refused, not launched.
+constexpr uint64_t kMaxPrivateBytesPerLane = 12u << 10;
+constexpr uint64_t kMaxScratchBytesPerDevice = 768ull << 20;
+
+// Device-wide scratch of kernel k at full occupancy on a device of `cus` compute units.
+uint64_t deviceScratch(const KernelInfo &k, int cus) {
+    return (uint64_t)k.private_bytes * 64 * k.waves_per_cu * (uint64_t)std::max(cus, 1);
+}
+
+// The limits in force (lowered only by dtrt_test_set_scratch_limits) and the device,
+// width and precision combinations already checked against them.
+struct ScratchLimits {
+    std::mutex mu;
+    uint64_t lane = kMaxPrivateBytesPerLane, device = kMaxScratchBytesPerDevice;
+    std::vector<std::array<int, 3>> done;
+};
+ScratchLimits &scratchLimits() {
+    static ScratchLimits L;
+    return L;
+}
+
+// Checks, once per device, estimator precision and width, that every kernel a scene of
+// that kind launches stays within the scratch limits.
+bool kernelsWithinScratchLimits(const DScene &S) {
+    ScratchLimits &L = scratchLimits();
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail("no HIP device");
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const std::array<int, 3> key{dev, S.nder, S.f64};
+    std::lock_guard<std::mutex> lk(L.mu);
+    if (std::find(L.done.begin(), L.done.end(), key) != L.done.end()) return true;
+    KernelInfo info[32];
+    int n = 0;
+    HIPCHK(kernel_info(S, info, 32, &n));
+    for (int i = 0; i < std::min(n, 32); ++i) {
+        const KernelInfo &k = info[i];
+        const uint64_t dev_bytes = deviceScratch(k, cus);
+        if (k.private_bytes > L.lane || dev_bytes > L.device) {
+            char msg[256];
+            std::snprintf(msg, sizeof(msg),
+                          "%s (%s, nder %d) needs %u bytes of scratch per lane, %.1f MB over the device: above "
+                          "the checked limits of %llu bytes and %.1f MB",
+                          k.name, S.f64 ? "double" : "float", S.nder, k.private_bytes, dev_bytes / 1e6,
+                          (unsigned long long)L.lane, L.device / 1e6);
+            return fail(msg);
+        }
+    }
+    L.done.push_back(key);
+    return true;
+}
+
 bool Scene::build() {
     if (!frameRotationReady()) return false;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -864,6 +924,7 @@ bool Scene::build() {
     std::memset(&S, 0, sizeof(S));
     S.nder = nder;
     S.f64 = precisionNow() == 64;
+    if (!kernelsWithinScratchLimits(S)) return false;
     S.nshapes = (int)shapes.size();
     S.nlights = (int)emitters.size();
     S.nmedia = (int)media.size();
@@ -2720,6 +2781,59 @@ int dtrt_test_medium_data(dtrt_object *medium, int which, float *out, int64_t ca
                             (double)m->d.maxDensity, (double)m->d.scale, 0.0};
     std::memcpy(meta8, meta, sizeof(meta));
     if (out) std::memcpy(out, v.data(), (size_t)std::min<int64_t>(cap, (int64_t)v.size()) * sizeof(float));
+    return 0;
+}
+int dtrt_test_kernel_resources(int nder, int bits, char *names, int name_len, uint64_t *vals, int cap) {
+    return guarded<int>(-1, [&]() -> int {
+        if (!(nder == 1 || nder == 2 || nder == 3 || nder == 4 || nder == 8 || nder == 16) || (bits != 32 && bits != 64) ||
+            cap < 0 || (cap > 0 && (!vals || !names || name_len < 1))) { g_err = "kernel_resources: bad arguments"; return -1; }
+        DScene S;
+        std::memset(&S, 0, sizeof(S));
+        S.nder = nder;
+        S.f64 = bits == 64;
+        int dev = 0, cus = 0, n = 0;
+        KernelInfo info[32];
+        auto query = [&]() -> bool {
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            HIPCHK(kernel_info(S, info, 32, &n));
+            return true;
+        };
+        if (!query()) return -1;
+        for (int i = 0; i < std::min({n, cap, 32}); ++i) {
+            std::snprintf(names + (size_t)i * name_len, (size_t)name_len, "%s", info[i].name);
+            const uint64_t v[5] = {info[i].private_bytes, info[i].vgprs, info[i].block, info[i].waves_per_cu,
+                                   deviceScratch(info[i], cus)};
+            std::memcpy(vals + 5 * (size_t)i, v, sizeof(v));
+        }
+        return n;
+    });
+}
+int dtrt_test_scratch_limits(uint64_t *out6) {
+    return guarded<int>(-1, [&]() -> int {
+        if (!out6) return -1;
+        {
+            ScratchLimits &L = scratchLimits();
+            std::lock_guard<std::mutex> lk(L.mu);
+            out6[0] = L.lane;
+            out6[1] = L.device;
+        }
+        const hipLimit_t lim[4] = {hipLimitStackSize, hipExtLimitScratchMin, hipExtLimitScratchMax,
+                                   hipExtLimitScratchCurrent};
+        for (int i = 0; i < 4; ++i) {
+            size_t v = 0;
+            out6[2 + i] = hipDeviceGetLimit(&v, lim[i]) == hipSuccess ? v : 0;
+        }
+        (void)hipGetLastError();   // an unsupported limit leaves no error behind
+        return 0;
+    });
+}
+int dtrt_test_set_scratch_limits(uint64_t lane_bytes, uint64_t device_bytes) {
+    ScratchLimits &L = scratchLimits();
+    std::lock_guard<std::mutex> lk(L.mu);
+    L.lane = lane_bytes ? std::min<uint64_t>(lane_bytes, kMaxPrivateBytesPerLane) : kMaxPrivateBytesPerLane;
+    L.device = device_bytes ? std::min<uint64_t>(device_bytes, kMaxScratchBytesPerDevice) : kMaxScratchBytesPerDevice;
+    L.done.clear();
     return 0;
 }
 int dtrt_test_shape_edges(dtrt_object *shape, int *out_v0v1f0f1, double *out_len, int cap) {

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdio>
 
 // The file is compiled once per dual width (-DDT_N=1,2,3,4,8,16: that width's AD,
 // child, primary-edge, adjoint and finalize kernels and their launchers) and once
@@ -1423,7 +1424,30 @@ template <int N> struct WidthOps {
     static void launchGrid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);
     static void launchAdjoint(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const float *dimg,
                               float *grad, int pmed, unsigned *st, unsigned long long *wk, hipStream_t sm);
+    static hipError_t kernelInfo(KernelInfo *out, int cap, int *count);
 };
+
+// Appends the resources of kernel `fn`, launched with `block` threads and `lds` bytes of
+// dynamic LDS, to out[*count] (nothing beyond cap).
+template <class F>
+static hipError_t addKernelInfo(F fn, const char *name, int block, size_t lds, KernelInfo *out, int cap, int *count) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void *>(fn));
+    if (e != hipSuccess) return e;
+    int blocks = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, block, lds);
+    if (e != hipSuccess) return e;
+    if (*count < cap) {
+        KernelInfo &k = out[*count];
+        std::snprintf(k.name, sizeof(k.name), "%s", name);
+        k.private_bytes = (uint32_t)a.localSizeBytes;
+        k.vgprs = (uint32_t)a.numRegs;
+        k.block = (uint32_t)block;
+        k.waves_per_cu = (uint32_t)(blocks * ((block + 63) / 64));
+    }
+    ++*count;
+    return hipSuccess;
+}
 
 template <class Sym> static hipError_t setRotation(const Sym &sym, const double *m9) {
     FrameRot r;
@@ -1490,6 +1514,30 @@ template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderPa
                                                  unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, dimg, grad, pmed, st, wk);
+}
+template <int N> hipError_t WidthOps<N>::kernelInfo(KernelInfo *out, int cap, int *count) {
+    char nm[40];
+    hipError_t e = hipSuccess;
+#define DT_INFO(FN, LABEL, BLK, LDS)                                             \
+    if (e == hipSuccess) {                                                       \
+        std::snprintf(nm, sizeof(nm), LABEL "<%d>", N);                          \
+        e = addKernelInfo(FN, nm, BLK, LDS, out, cap, count);                     \
+    }
+    if constexpr (adRefill<N>()) {
+        DT_INFO(k_ad_refill<N>, "k_ad_refill", BLOCK, traceLds(BLOCK))
+    } else {
+        DT_INFO((k_ad_paths<N, AD_VOLPATH>), "k_ad_paths", MAX_BLOCK, traceLds(MAX_BLOCK))
+    }
+    DT_INFO((k_ad_paths<N, AD_PATH>), "k_ad_paths/path", MAX_BLOCK, traceLds(MAX_BLOCK))
+    DT_INFO((k_ad_paths<N, AD_DIRECT>), "k_ad_paths/direct", MAX_BLOCK, traceLds(MAX_BLOCK))
+    DT_INFO(k_children<N>, "k_children", MAX_BLOCK, traceLds(MAX_BLOCK))
+    DT_INFO(k_children_path<N>, "k_children_path", MAX_BLOCK, traceLds(MAX_BLOCK))
+    DT_INFO(k_primary<N>, "k_primary", BLOCK, traceLds(BLOCK))
+    DT_INFO(k_adjoint<N>, "k_adjoint", BLOCK, traceLds(BLOCK))
+    DT_INFO(k_grid<N>, "k_grid", BLOCK, 0)
+    DT_INFO(k_finalize<N>, "k_finalize", 256, 0)
+#undef DT_INFO
+    return e;
 }
 template struct WidthOps<DT_N>;
 
@@ -1628,6 +1676,23 @@ hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, i
 hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm) {
     hipLaunchKernelGGL(k_philox, dim3(1), dim3(1), 0, sm, ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
     return hipGetLastError();
+}
+hipError_t kernel_info(const DScene &S, KernelInfo *out, int cap, int *count) {
+    *count = 0;
+    hipError_t e;
+    switch (S.nder) {
+#define DT_INFO_N(n) \
+        case n: e = WidthOps<n>::kernelInfo(out, cap, count); break;
+        DT_FOR_N(DT_INFO_N)
+#undef DT_INFO_N
+        default: return hipErrorInvalidValue;
+    }
+    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_VOLPATH>, "k_primal_paths/volpath", BLOCK, traceLds(BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_SIMPLE>, "k_primal_paths/simple", BLOCK, traceLds(BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_PATH>, "k_primal_paths/path", BLOCK, traceLds(BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_particles, "k_particles", BLOCK, traceLds(BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_refit, "k_refit", 256, 0, out, cap, count);
+    return e;
 }
 #endif  // DT_N
 

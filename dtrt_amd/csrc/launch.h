@@ -35,6 +35,17 @@ enum Stat {
 // an fp32 direction apart up to ~1000-px-wide images.
 constexpr double PRIMARY_DELTA = 1e-4;
 
+// Compiled resources of one kernel (hipFuncGetAttributes) at the largest work-group size
+// it is launched with, and the waves of it one CU holds at that size
+// (hipOccupancyMaxActiveBlocksPerMultiprocessor).
+struct KernelInfo {
+    char name[40];
+    uint32_t private_bytes;   // private segment per lane: register spills and private arrays
+    uint32_t vgprs;
+    uint32_t block;
+    uint32_t waves_per_cu;
+};
+
 // The launchers exist once per estimator precision: in namespace p32 (float) and p64
 // (double), each defined by the kernel objects of that precision, and once in
 // namespace dt itself, where they dispatch on S.f64 (dispatch.cpp). Accumulators
@@ -64,6 +75,8 @@ constexpr double PRIMARY_DELTA = 1e-4;
 // launch_particles: ParticleTracer particles s0 .. s0+n-1 splatted into acc[pixel][3]
 // of view 0's window.
 // launch_scale: out[i] = acc[i] * scale, i < n.
+// kernel_info: the kernels a render at S.nder launches, then the width-independent ones;
+// *count of them written into out[0 .. cap).
 #define DT_LAUNCH_API                                                                                                   \
     hipError_t setFrameRotation(const double *m9);                                                                       \
     hipError_t launch_ad_paths(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, const uint4 *replay,    \
@@ -97,7 +110,8 @@ constexpr double PRIMARY_DELTA = 1e-4;
     hipError_t launch_medium(const DScene &S, int med, int kind, const float *seg, int n, uint64_t seed, float *out,     \
                              hipStream_t sm);                                                                            \
     hipError_t launch_grid(const DScene &S, int med, const float *pts, int n, float *out, hipStream_t sm);              \
-    hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm);
+    hipError_t launch_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, hipStream_t sm);                \
+    hipError_t kernel_info(const DScene &S, KernelInfo *out, int cap, int *count);
 
 enum PrimalKind { PRIMAL_VOLPATH = 1, PRIMAL_SIMPLE = 2, PRIMAL_PATH = 3 };
 

@@ -39,6 +39,20 @@ int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8);
  * meta8: {rx, ry, rz, channels, floats stored, max density, scale, 0} (doubles).
  * Returns 0, or -1 for a handle that is not a medium. */
 int dtrt_test_medium_data(dtrt_object *medium, int which, float *out, int64_t cap, double *meta8);
+/* Compiled resources of the kernels a scene of dual width nder (1, 2, 3, 4, 8, 16)
+ * and estimator precision `bits` (32, 64) launches: up to cap kernels, each a
+ * NUL-terminated name in names[i * name_len ..] and five values in vals[5 i ..]:
+ * private (scratch) bytes per lane, VGPRs, work-group size, waves per CU at that size,
+ * scratch bytes over the device at that occupancy. Returns the number of kernels
+ * (which may exceed cap), or -1. */
+int dtrt_test_kernel_resources(int nder, int bits, char *names, int name_len, uint64_t *vals, int cap);
+/* out6: [0] scratch bytes per lane and [1] over the device above which a scene's
+ * kernels are refused; the runtime's [2] stack-size limit per thread and [3..5] scratch
+ * limit minimum, maximum and current value (hipDeviceGetLimit; 0 where unsupported). */
+int dtrt_test_scratch_limits(uint64_t *out6);
+/* Lowers the scratch limits scenes are checked against (0 restores a default; values
+ * above the defaults are clamped to them), so that the refusal can be tested. */
+int dtrt_test_set_scratch_limits(uint64_t lane_bytes, uint64_t device_bytes);
 /* Edge list of a shape (v0, v1, f0, f1 local ids; f1 = -1 on boundaries) and lengths;
  * returns the number of edges. */
 int dtrt_test_shape_edges(dtrt_object *shape, int *out_v0v1f0f1, double *out_len, int cap);
