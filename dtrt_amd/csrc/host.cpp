@@ -20,6 +20,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace dt;
@@ -2467,6 +2468,47 @@ int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_objec
         g_err = e.what();
         return -1;
     }
+}
+
+int dtrt_render_scenes(int integrator, int nscenes, dtrt_object *const *scenes, const int *nviews,
+                       dtrt_object *const *cameras, const dtrt_render_options *options, float *const *images,
+                       int images_on_device) {
+    if (integrator != 0) { g_err = "render_scenes: only VolPathTracerAD (integrator 0) is provided"; return -2; }
+    if (nscenes <= 0 || !scenes || !nviews || !options || !images) { g_err = "render_scenes: bad arguments"; return -1; }
+    std::vector<size_t> first((size_t)nscenes + 1, 0);
+    for (int i = 0; i < nscenes; ++i) {
+        if (!as<Scene>(scenes[i], T_SCENE)) return -1;
+        if (nviews[i] <= 0) { g_err = "render_scenes: every scene needs at least one view"; return -1; }
+        for (int j = 0; j < i; ++j)
+            if (scenes[j] == scenes[i]) { g_err = "render_scenes: a scene is listed twice"; return -1; }
+        first[i + 1] = first[i] + (size_t)nviews[i];
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { g_err = "render_scenes: no HIP device"; return -1; }
+    // Each scene renders on its own stream from its own host thread (a render waits on
+    // its child counts between launches), so the scenes' kernels run side by side.
+    std::vector<int> rc((size_t)nscenes, 0);
+    std::vector<std::string> err((size_t)nscenes);
+    auto one = [&](int i) {
+        if (hipSetDevice(dev) != hipSuccess) { rc[i] = -1; err[i] = "hipSetDevice failed"; return; }
+        rc[i] = dtrt_render_views(integrator, scenes[i], nviews[i], cameras ? cameras + first[i] : nullptr,
+                                  options + first[i], images + first[i], images_on_device);
+        if (rc[i] != 0) err[i] = g_err;
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int i = 1; i < nscenes; ++i) pool.emplace_back(one, i);
+    } catch (const std::exception &e) {   // could not start a thread: render the rest here
+        for (int i = (int)pool.size() + 1; i < nscenes; ++i) one(i);
+    }
+    one(0);
+    for (std::thread &t : pool) t.join();
+    for (int i = 0; i < nscenes; ++i)
+        if (rc[i] != 0) {
+            g_err = "render_scenes: scene " + std::to_string(i) + ": " + err[i];
+            return rc[i];
+        }
+    return 0;
 }
 
 int dtrt_scene_capture_views(dtrt_object *scene, int nviews, dtrt_object *const *cameras, const dtrt_render_options *options,

@@ -94,6 +94,8 @@ _lib.dtrt_scene_capture_views.argtypes = [_P, C.c_int, C.POINTER(_P), C.POINTER(
 _lib.dtrt_scene_replay_views.argtypes = [_P, C.POINTER(C.c_uint64)]
 _lib.dtrt_render_views.argtypes = [C.c_int, _P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
                                    C.POINTER(_P), C.c_int]
+_lib.dtrt_render_scenes.argtypes = [C.c_int, C.c_int, C.POINTER(_P), C.POINTER(C.c_int), C.POINTER(_P),
+                                    C.POINTER(_m.fns["render"].argtypes[2]._type_), C.POINTER(_P), C.c_int]
 _lib.dtrt_render_adjoint.argtypes = [C.c_int, _P, C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
 _lib.dtrt_render_views_adjoint.argtypes = [C.c_int, _P, C.c_int, C.POINTER(_P),
                                            C.POINTER(_m.fns["render"].argtypes[2]._type_), C.c_int, _P, _P]
@@ -196,6 +198,25 @@ def render_views(scene, cameras, options, images, integrator=0):
     if on_dev:
         _after_torch(scene)
     _chk(_lib.dtrt_render_views(integrator, scene._h, n, cams, opts, ptrs, int(on_dev)), "render_views")
+
+
+def render_scenes(scenes, cameras, options, images, integrator=0):
+    """``dtrt_render_scenes``: the views of several scenes in one call, each scene on its
+    own stream. cameras, options, images: one list per scene, as for render_views."""
+    counts = [len(o) for o in options]
+    flat_c = [c for i, n in enumerate(counts) for c in (cameras[i] if cameras and cameras[i] else [None] * n)]
+    flat_o = [o for ol in options for o in ol]
+    flat_i = [im for il in images for im in il]
+    on_dev = hasattr(flat_i[0], "data_ptr")
+    cams = (_P * len(flat_o))(*[None if c is None else c._h for c in flat_c])
+    opts = (_m.fns["render"].argtypes[2]._type_ * len(flat_o))(*[o._struct() for o in flat_o])
+    ptrs = (_P * len(flat_o))(*[im.data_ptr() if on_dev else im.ctypes.data for im in flat_i])
+    if on_dev:
+        for sc in scenes:
+            _after_torch(sc)
+    hs = (_P * len(scenes))(*[sc._h for sc in scenes])
+    nv = (C.c_int * len(scenes))(*counts)
+    _chk(_lib.dtrt_render_scenes(integrator, len(scenes), hs, nv, cams, opts, ptrs, int(on_dev)), "render_scenes")
 
 
 def capture_views(scene, cameras, options, images):

@@ -178,6 +178,16 @@ int dtrt_scene_set_launch_settings(dtrt_object *scene, const int64_t *in6);
 int dtrt_render_views(int integrator, dtrt_object *scene, int nviews, dtrt_object *const *cameras,
                       const dtrt_render_options *options, float *const *images, int images_on_device);
 
+/* Views of several scenes in one call: scene i renders its nviews[i] views (the next
+ * nviews[i] entries of cameras, options and images, as for dtrt_render_views) on its
+ * own HIP stream, driven by its own host thread, so the scenes' launches overlap on
+ * the GPU. Each image equals that scene's dtrt_render_views result. A scene may be
+ * listed once. Returns 0, or the first failing scene's status with its error.
+ * No counterpart in the reference (one scene per render call). */
+int dtrt_render_scenes(int integrator, int nscenes, dtrt_object *const *scenes, const int *nviews,
+                       dtrt_object *const *cameras, const dtrt_render_options *options, float *const *images,
+                       int images_on_device);
+
 /* Render views once with dtrt_render_views (device images), then capture that launch
  * sequence (its chunk schedule fixed) as a HIP graph owned by the scene, replacing any
  * earlier one. dtrt_scene_replay_views re-renders the same views into the same images

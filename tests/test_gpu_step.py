@@ -80,3 +80,58 @@ def test_step_graph_replays_equal_eager_steps(gpu):
         assert np.allclose(got[2].cpu().numpy(), pg2.cpu().numpy(), rtol=1e-4, atol=1e-6 * float(pg2.abs().max()))
         w = g2.cpu().numpy()
         assert np.allclose(got[1].cpu().numpy(), w, rtol=1e-3, atol=1e-5 * np.abs(w).max())
+
+
+@pytest.mark.parametrize("lane", [0, 1])
+def test_shape_lane_gradient_matches_crn_finite_differences(gpu, lane):
+    """The step's shape-parameter gradient is its loss gradient contracted with the
+    tangent lanes (test_step_equals_its_parts). Here the lane itself is checked: for a
+    fixed weight image w (the L2 gradient 2 (I - T) of a 2048-spp render against a
+    target moved by 0.05 along the lane's field), sum(w * lane) agrees with central
+    differences of sum(w * I), the vertices moved by +-h along the field through
+    dtrt_scene_update_shape and rendered with the same seeds (common random numbers),
+    within the joint sampling error of eight seeds per side. Lane 0 translates the
+    sphere along x, lane 1 is a deformation field. With fixed weights the check carries
+    no bias from the image's own noise.
+
+    The sphere is black, an occluder inside the medium: its silhouette (primary edges),
+    its shadows in the medium (volume edges) and the medium boundary it forms (Lins) carry
+    the whole derivative. A lit sphere is not used: the estimator, as the reference's,
+    has no boundary term for radiance steps inside a mesh's outline (shading-normal
+    terminators where the view direction crosses the interpolated normal, facet edges of
+    flat shading), so its shape derivative differs from finite differences there."""
+    import torch
+    d = scenes.config2(nder=8, res=48, spp=256, freq=10, grid_res=16)
+    d.bsdfs[2] = ("diffuse", (0.0, 0.0, 0.0))
+    d.shapes[0].velocities[0] = np.array([1.0, 0.0, 0.0], np.float32)
+    sc = scenes.instantiate(gpu, d)
+    cam = [gpu.Camera(d.width, d.height, d.cam_to_world.astype(np.float32), d.cam_to_ndc.astype(np.float32),
+                      d.clip_near, -1, None)]
+    v0 = d.shapes[0].vertices.astype(np.float64)
+    vel = d.shapes[0].velocities[lane].astype(np.float64)
+
+    def render(seed, spp):
+        img = torch.zeros((d.nder + 1, d.height, d.width, 3), device="cuda")
+        gpu.render_views(sc, cam, [gpu.RenderOptions(seed, spp, 6, d.sppe, d.sppse, True, -1, d.ddist)], [img])
+        return img.double()
+
+    gpu.update_shape(sc, 0, v0 + 0.05 * vel)
+    tgt = render(99, 2048)[0]
+    gpu.update_shape(sc, 0, v0)
+    w = 2 * (render(98, 2048)[0] - tgt)
+    seeds = [7 + 10 * i for i in range(8)]
+    ad = np.array([float((w * render(s, d.spp)[1 + lane]).sum()) for s in seeds])
+    h = 0.02
+    fds = []
+    for s in seeds:
+        gpu.update_shape(sc, 0, v0 + h * vel)
+        up = float((w * render(s, d.spp)[0]).sum())
+        gpu.update_shape(sc, 0, v0 - h * vel)
+        dn = float((w * render(s, d.spp)[0]).sum())
+        fds.append((up - dn) / (2 * h))
+    gpu.update_shape(sc, 0, v0)
+    fd, se_fd = np.mean(fds), np.std(fds, ddof=1) / np.sqrt(len(fds))
+    g, se_g = ad.mean(), ad.std(ddof=1) / np.sqrt(len(ad))
+    print("lane", lane, "AD", g, "+-", se_g, "FD", fd, "+-", se_fd)
+    assert g < -4 * se_g, (g, se_g)
+    assert abs(fd - g) < 4 * np.hypot(se_fd, se_g) + 0.05 * abs(g), (fd, se_fd, g, se_g)

@@ -88,3 +88,31 @@ def test_graph_capture_refuses_overflowing_schedule(gpu):
     with pytest.raises(RuntimeError, match="overflowed"):
         gpu.capture_views(sc, None, [gpu.RenderOptions(1, 16, 5, 0, 4, True)], [torch.zeros((2, 32, 32, 3), device="cuda")])
     gpu.set_queue_budget(sc, 0)
+
+
+def test_scenes_render_together_as_alone(gpu):
+    """dtrt_render_scenes over three different scenes (surfaces only, a medium, a
+    dielectric in a medium), each on its own stream and host thread, gives every view
+    the image of that scene's render_views alone; a scene listed twice is refused."""
+    import torch
+    ds = [scenes.config0(nder=4, res=32, spp=16), scenes.config1(nder=4, res=40, spp=8, grid_res=16),
+          scenes.config2(nder=4, res=32, spp=8, freq=8, grid_res=16)]
+    scs = [scenes.instantiate(gpu, d) for d in ds]
+    cams = [[None], [None, _camera(gpu, ds[1], (1.5, 1.2, 3.0), (48, 40))], [None]]
+    opts = [[gpu.RenderOptions(1, 16, 5, 256, 4, True)],
+            [gpu.RenderOptions(2, 8, 6, 256, 4, True), gpu.RenderOptions(3, 12, 6, 256, 4, True)],
+            [gpu.RenderOptions(4, 8, 6, 256, 4, True)]]
+    shapes = [[(5, 32, 32, 3)], [(5, 40, 40, 3), (5, 40, 48, 3)], [(5, 32, 32, 3)]]
+    outs = [[torch.zeros(s, device="cuda") for s in sl] for sl in shapes]
+    gpu.render_scenes(scs, cams, opts, outs)
+    for i in range(3):
+        alone = [torch.zeros(s, device="cuda") for s in shapes[i]]
+        gpu.render_views(scs[i], cams[i], opts[i], alone)
+        for a, b in zip(outs[i], alone):
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            assert np.abs(b[0]).sum() > 0
+            for lane in range(5):
+                scale = np.abs(b[lane]).max() + 1e-12
+                assert np.allclose(a[lane], b[lane], rtol=1e-4, atol=1e-5 * scale), (i, lane)
+    with pytest.raises(RuntimeError, match="twice"):
+        gpu.render_scenes([scs[0], scs[0]], [cams[0], cams[0]], [opts[0], opts[0]], [outs[0], outs[0]])
