@@ -240,8 +240,31 @@ def views_leg(nviews=32):
     fallbacks = sum(G.replay_views(sc, [300 + 8 * k + i for i in range(len(small))]) for k in range(reps))
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t) / reps
+    # several small scenes (own density grids) in one call, each on its own stream
+    nsc = 8
+    mds = [scenes.config1(nder=3, res=128, spp=16, seed=20 + i) for i in range(nsc)]
+    mscs = [scenes.instantiate(G, m) for m in mds]
+    mopts = [[G.RenderOptions(400 + i, m.spp, m.max_bounces, m.sppe, m.sppse, True, -1, m.ddist)] for i, m in enumerate(mds)]
+    mouts = [[torch.zeros(m.out_shape(), device="cuda")] for m in mds]
+    G.render_scenes(mscs, None, mopts, mouts)   # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(reps):
+        for i in range(nsc):
+            G.render_views(mscs[i], None, mopts[i], mouts[i])
+    torch.cuda.synchronize()
+    t_ms_seq = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for k in range(reps):
+        G.render_scenes(mscs, None, mopts, mouts)
+    torch.cuda.synchronize()
+    t_ms_tog = (time.perf_counter() - t) / reps
     return {"workload": "%d orbit views of the heterogeneous-medium scene, res %d^2..%d^2, spp 16/32/64 (PCG32 seed 5), "
                         "nder 3, depth 10, sppse 10, sppe 4096" % (nviews, min(v[1] for v in views), max(v[1] for v in views)),
+            "multi_scene": {"scenes": nsc, "workload": "heterogeneous-medium scene with 8 different 64^3 grids (seeds 20..27), "
+                                                       "one 128^2 view at 16 spp each",
+                            "one_call_per_scene_ms": t_ms_seq * 1e3, "one_call_ms": t_ms_tog * 1e3,
+                            "speedup": t_ms_seq / t_ms_tog},
             "samples": samples, "batched_s": t_batch, "one_call_per_view_s": t_seq,
             "batched_samples_per_s": samples / t_batch, "batching_speedup": t_seq / t_batch,
             "small_batch": {"views": len(small), "res": 128, "spp": 16, "eager_ms": t_eager * 1e3,
@@ -397,11 +420,15 @@ def inverse_leg(spp_divisor=16):
         for i in range(len(views)):
             G.render_views(sc, [cams[i]], [fo[i]], [ims[i]])
     t_single, _ = clock(singles)
+    # a shape step: new vertex positions for the 1M-face sphere, BVH refitted on the GPU
+    v_new = d.shapes[0].vertices + 1e-3 * d.shapes[0].velocities[0]
+    t_update, _ = clock(lambda: G.update_shape(sc, 0, v_new))
     return {"workload": "batched inverse-rendering step: 32 orbit views (res 128^2..768^2, spp 16..256 / %d, PCG32 seed 5) "
                         "of a 1,003,520-face displaced sphere in a 256^3 fBm medium; forward nder 8 (shape lanes), all "
                         "terms; L2 loss vs target images; gradient of the 8 lanes; density adjoint over 16,777,216 nodes"
                         % spp_divisor,
             "spp_divisor": spp_divisor, "samples": samples, "scene_generation_s": t_gen, "setup_s": t_setup,
+            "vertex_update_s": t_update,
             "eager_step_s": t_eager, "graph_step_s": t_graph, "graph_fell_back_to_eager": bool(rc == 1),
             "graph_speedup": t_eager / t_graph, "capture_s_including_one_eager_step": t_capture,
             "step_samples_per_s": samples / t_graph, "forward_batched_s": t_batch, "forward_one_call_per_view_s": t_single,
