@@ -9,6 +9,8 @@
 #include "dmath.h"
 #include "scene_types.h"
 
+#include <climits>
+
 namespace dt {
 namespace DT_PNS {
 
@@ -59,7 +61,10 @@ DI bool boxHit(const float4 &lo, const float4 &hi, fv3 o, fv3 inv, float tnear, 
 // t, and the box and triangle tests done. Inlined at every call site: as an
 // out-of-line call, the path kernels save their live path state to scratch around
 // each traversal (measured: ~2.5% slower per render, with several GB of spill
-// traffic per launch).
+// traffic per launch). Nested loops (inner nodes, then the leaf reached): a wave runs
+// its lanes' inner-node steps together and then tests the leaves they reached
+// together, instead of running both branches in every step that some lane spends at
+// a leaf; nodes are visited in the same order, so hits are the same.
 struct TraceHit {
     int face;
     float t;
@@ -80,8 +85,11 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
     int sp = 0;
     int node = 0;
     uint32_t nbox = 0, ntri = 0;
-    while (true) {
-        if (node >= 0) {
+    constexpr int DONE = INT_MIN;   // no leaf encodes to it: leaves hold >= 1 triangle of < 2^27
+    while (node != DONE) {
+        // inner nodes until this lane reaches a leaf or runs out of nodes; the wave then
+        // tests the leaves it reached together
+        while (node >= 0) {
             const float4 *n = nodes + 4 * (size_t)node;
             float4 a = n[0], b = n[1], c = n[2], e = n[3];
             float tl, tr;
@@ -93,38 +101,36 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
                 if (tr < tl) { int t = cl; cl = cr; cr = t; }
                 stk[stride * sp++] = cr;
                 node = cl;
-                continue;
             } else if (hl) {
                 node = cl;
-                continue;
             } else if (hr) {
                 node = cr;
-                continue;
-            }
-        } else {
-            int enc = ~node;
-            int first = enc >> 4, cnt = enc & 15;
-            ntri += cnt;
-            for (int k = first; k < first + cnt; ++k) {
-                const float4 *tp = tris + 3 * (size_t)k;
-                float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
-                fv3 E1{e1.x, e1.y, e1.z}, E2{e2.x, e2.y, e2.z};
-                fv3 pv = fcross(d, E2);
-                float det = fdot(E1, pv);
-                if (det == 0.0f) continue;
-                float idet = 1.0f / det;
-                fv3 s = fsub(o, fv3{v0.x, v0.y, v0.z});
-                float u = fdot(s, pv) * idet;
-                if (u < -BEPS || u > 1.0f + BEPS) continue;
-                fv3 q = fcross(s, E1);
-                float v = fdot(d, q) * idet;
-                if (v < -BEPS || u + v > 1.0f + BEPS) continue;
-                float t = fdot(E2, q) * idet;
-                if (t >= tnear && t <= best) { best = t; face = __float_as_int(v0.w); }
+            } else {
+                node = sp == 0 ? DONE : stk[stride * --sp];
             }
         }
-        if (sp == 0) break;
-        node = stk[stride * --sp];
+        if (node == DONE) break;
+        int enc = ~node;
+        int first = enc >> 4, cnt = enc & 15;
+        ntri += cnt;
+        for (int k = first; k < first + cnt; ++k) {
+            const float4 *tp = tris + 3 * (size_t)k;
+            float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
+            fv3 E1{e1.x, e1.y, e1.z}, E2{e2.x, e2.y, e2.z};
+            fv3 pv = fcross(d, E2);
+            float det = fdot(E1, pv);
+            if (det == 0.0f) continue;
+            float idet = 1.0f / det;
+            fv3 s = fsub(o, fv3{v0.x, v0.y, v0.z});
+            float u = fdot(s, pv) * idet;
+            if (u < -BEPS || u > 1.0f + BEPS) continue;
+            fv3 q = fcross(s, E1);
+            float v = fdot(d, q) * idet;
+            if (v < -BEPS || u + v > 1.0f + BEPS) continue;
+            float t = fdot(E2, q) * idet;
+            if (t >= tnear && t <= best) { best = t; face = __float_as_int(v0.w); }
+        }
+        node = sp == 0 ? DONE : stk[stride * --sp];
     }
     res.face = face;
     res.t = best;

@@ -78,17 +78,24 @@ template <int N> struct Tracer {
         return T;
     }
 
-    // scene.cpp:617-701
-    DI f3 lookForEmitter(f3 o, f3 d, bool onSurface, Rng &r, int med, int max_inter, Hit &its, real &pdf_nee) {
+    // scene.cpp:617-701. One trace site: the first hit and each hit past a null
+    // interface come from the same traversal. first_only: stop at the first hit (the
+    // continuation through a null surface, scene.cpp:538-575).
+    DI f3 lookForEmitter(f3 o, f3 d, bool onSurface, Rng &r, int med, int max_inter, Hit &its, real &pdf_nee,
+                         bool first_only = false) {
         real tmin = onSurface ? SHADOW_EPS : 0._r, th;
-        f3 scat = o;
-        int f = trace(o, d, tmin, FINF, th);
-        if (f < 0) { its.face = -1; its.t = FINF; return f3{0, 0, 0}; }
-        refineHit(S, f, o, d, its);
-        Hit nh = its;
+        const f3 scat = o;
         real T = 1._r;
         int inter = 0;
+        its.face = -1;
+        its.t = FINF;
         while (true) {
+            const int f = trace(o, d, tmin, FINF, th);
+            if (f < 0) return f3{0, 0, 0};
+            Hit nh;
+            refineHit(S, f, o, d, nh);
+            if (inter == 0) its = nh;
+            if (first_only) return f3{0, 0, 0};
             const DShape &sh = shapeOf(S, f);
             real tmax = nh.t;
             if (med >= 0) T *= mediumTransmittance(S.media[med], o, d, tmin, tmax, r);
@@ -103,24 +110,29 @@ template <int N> struct Tracer {
             o = o + d * tmax;
             tmin = SHADOW_EPS;
             inter++;
-            f = trace(o, d, SHADOW_EPS, FINF, th);
-            if (f < 0) return f3{0, 0, 0};
-            refineHit(S, f, o, d, nh);
         }
     }
 
-    // scene.cpp:703-793 — with tangents; reports the far hit's face and AD distance
-    DI Ds<N> lookForEmitterAD(Dv<N> o, const Dv<N> &d, bool onSurface, Rng &r, int med, int max_inter, HitAD<N> &its,
-                              real &pdf_nee, int *far_face, real *far_t, const ScatterCtx *scx = nullptr) {
+    // scene.cpp:703-793 — with tangents; reports the face of the first hit and, when a
+    // light ends the walk, the light's face and the walked AD distance. One trace site:
+    // the first hit and each hit past a null interface come from the same traversal.
+    // first_only: stop at the first hit (the continuation through a null surface,
+    // scene.cpp:577-615).
+    DI Ds<N> lookForEmitterAD(Dv<N> o, const Dv<N> &d, bool onSurface, Rng &r, int med, int max_inter, int &first_face,
+                              real &pdf_nee, int *far_face, real *far_t, const ScatterCtx *scx = nullptr,
+                              bool first_only = false) {
         real tmin = onSurface ? SHADOW_EPS : 0._r, th;
-        f3 scat = o.val(), dv = d.val();
-        int f = trace(scat, dv, tmin, FINF, th);
-        if (f < 0) { its.face = -1; its.t = Df<N>(FINF); return Ds<N>(); }
-        refineHitAD<N>(S, f, o, d, its);
-        HitAD<N> nh = its;
+        const f3 scat = o.val(), dv = d.val();
         Df<N> T(1._r), dist(0._r);
         int inter = 0;
+        first_face = -1;
         while (true) {
+            const int f = trace(o.val(), dv, tmin, FINF, th);
+            if (f < 0) return Ds<N>();
+            if (inter == 0) first_face = f;
+            if (first_only) return Ds<N>();
+            HitAD<N> nh;
+            refineHitAD<N>(S, f, o, d, nh);
             const DShape &sh = shapeOf(S, f);
             Df<N> tmax = nh.t;
             dist += nh.t;
@@ -137,9 +149,6 @@ template <int N> struct Tracer {
             o = o + d * tmax;
             tmin = SHADOW_EPS;
             inter++;
-            f = trace(o.val(), dv, SHADOW_EPS, FINF, th);
-            if (f < 0) return Ds<N>();
-            refineHitAD<N>(S, f, o, d, nh);
         }
     }
 
@@ -156,6 +165,55 @@ template <int N> struct Tracer {
         nrm = normalize(cross(e1, e2));
         pos = v0 + e1 * b1 + e2 * b2;
         return l;
+    }
+
+    // A next-event shadow ray, set up but not yet traced (scene.cpp:399-491), so that
+    // the scattering events of a bounce share one transmittance walk.
+    struct Shadow {
+        f3 o, d, ln;
+        real dist, pdf;
+        int light, med, max_inter;
+        bool on_surface, live;
+    };
+    // scene.cpp:399-416: the light sample of a point in a medium
+    DI void neePointSetup(f3 ps, const real *r4, int med, int max_inter, Shadow &q) {
+        q.live = false;
+        if (S.nlights == 0) return;
+        f3 lp, ln;
+        const int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - ps;
+        const real pdf = S.light_pmf[l] / S.light_areas[l];
+        if (dot(dir, ln) < 0._r && pdf != 0._r) {
+            const real dist = length(dir);
+            q.o = ps; q.d = dir / dist; q.ln = ln; q.dist = dist; q.pdf = pdf;
+            q.light = l; q.med = med; q.max_inter = max_inter; q.on_surface = false; q.live = true;
+        }
+    }
+    // scene.cpp:456-480: the light sample of a surface point
+    DI void neeSurfaceSetup(const Hit &its, const real *r4, int med, int max_inter, Shadow &q) {
+        q.live = false;
+        if (S.nlights == 0) return;
+        f3 lp, ln;
+        const int l = sampleLight(r4, lp, ln);
+        f3 dir = lp - its.p;
+        const real pdf = S.light_pmf[l] / S.light_areas[l];
+        const DShape &lsh = S.shapes[S.emitters[l].shape_id];
+        const int lbt = bsdfOf(S, lsh).type;
+        const bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
+        if ((transm || dot(dir, its.sn) > 0._r) && dot(dir, ln) < 0._r && pdf != 0._r && max_inter >= 0) {
+            const DShape &sh = shapeOf(S, its.face);
+            if (isTransition(sh)) med = targetMedium(sh, its.gn, dir);
+            const real dist = length(dir);
+            q.o = its.p; q.d = dir / dist; q.ln = ln; q.dist = dist; q.pdf = pdf;
+            q.light = l; q.med = med; q.max_inter = max_inter; q.on_surface = true; q.live = true;
+        }
+    }
+    // scene.cpp:417-425, 481-487: the emitted radiance over the pdf, given the walk's
+    // transmittance T (0: blocked); pdf becomes the solid-angle pdf.
+    DI f3 neeFinish(const Shadow &q, real T, real &pdf) {
+        if (!q.live || T == 0._r) return f3{0, 0, 0};
+        pdf = q.pdf * (q.dist * q.dist / dot(q.ln, -q.d));
+        return emitterEvalNorm(S.emitters[q.light], q.ln, -q.d) * (T / pdf);
     }
 
     // scene.cpp:399-426 (point in a medium; wo in world space)
@@ -205,53 +263,56 @@ template <int N> struct Tracer {
         }
         return f3{0, 0, 0};
     }
-    // scene.cpp:428-454
-    DI Ds<N> neePointAD(const Dv<N> &ps, const real *r4, Rng &r, int med, int max_inter, f3 &wo, real &pdf,
-                        const ScatterCtx *scx = nullptr) {
-        if (S.nlights == 0) return Ds<N>();
+    // A next-event shadow ray with tangents, set up but not yet traced
+    // (scene.cpp:428-454, 493-524): the origin carries its tangents, the direction to
+    // the light sample none.
+    struct ShadowAD {
+        Dv<N> o;
+        f3 d;
+        real dist;
+        int light, med, max_inter;
+        bool on_surface, live;
+    };
+    // scene.cpp:428-440: the light sample of a point in a medium
+    DI void neePointADSetup(const Dv<N> &ps, const real *r4, int med, int max_inter, ShadowAD &q) {
+        q.live = false;
+        if (S.nlights == 0) return;
         f3 lp, ln;
-        int l = sampleLight(r4, lp, ln);
+        const int l = sampleLight(r4, lp, ln);
         f3 dir = lp - ps.val();
         if (dot(dir, ln) < 0._r) {
-            real dist = length(dir);
-            dir = dir / dist;
-            HitAD<N> inear;
-            int ff = -1;
-            real ft = 0._r;
-            Ds<N> ret = lookForEmitterAD(ps, Dv<N>(dir), false, r, med, max_inter, inear, pdf, &ff, &ft, scx);
-            if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
-                fabs(ft - dist) < SHADOW_EPS) {
-                wo = dir;
-                return ret * (1.0_r / pdf);
-            }
+            const real dist = length(dir);
+            q.o = ps; q.d = dir / dist; q.dist = dist;
+            q.light = l; q.med = med; q.max_inter = max_inter; q.on_surface = false; q.live = true;
         }
-        return Ds<N>();
     }
-    // scene.cpp:493-524
-    DI Ds<N> neeSurfaceAD(const HitAD<N> &its, const FrameAD<N> &shf, const real *r4, Rng &r, int med, int max_inter,
-                          Dv<N> &wo, real &pdf, const ScatterCtx *scx = nullptr) {
-        if (S.nlights == 0) return Ds<N>();
+    // scene.cpp:493-511: the light sample of a surface point
+    DI void neeSurfaceADSetup(const HitAD<N> &its, const real *r4, int med, int max_inter, ShadowAD &q) {
+        q.live = false;
+        if (S.nlights == 0) return;
         f3 lp, ln;
-        int l = sampleLight(r4, lp, ln);
+        const int l = sampleLight(r4, lp, ln);
         f3 dir = lp - its.p.val();
         const DShape &lsh = S.shapes[S.emitters[l].shape_id];
-        int lbt = bsdfOf(S, lsh).type;
-        bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
+        const int lbt = bsdfOf(S, lsh).type;
+        const bool transm = lbt == BSDF_NULL || lbt == BSDF_ROUGHDIELECTRIC;
         if ((transm || dot(dir, its.sn.val()) > 0._r) && dot(dir, ln) < 0._r) {
             const DShape &sh = shapeOf(S, its.face);
             if (isTransition(sh)) med = targetMedium(sh, its.gn.val(), dir);
-            real dist = length(dir);
-            dir = dir / dist;
-            HitAD<N> inear;
-            int ff = -1;
-            real ft = 0._r;
-            Ds<N> ret = lookForEmitterAD(its.p, Dv<N>(dir), true, r, med, max_inter, inear, pdf, &ff, &ft, scx);
-            if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[l].shape_id &&
-                fabs(ft - dist) < SHADOW_EPS) {
-                wo = shf.toLocal(Dv<N>(dir));
-                return ret * (1.0_r / pdf);
-            }
+            const real dist = length(dir);
+            q.o = its.p; q.d = dir / dist; q.dist = dist;
+            q.light = l; q.med = med; q.max_inter = max_inter; q.on_surface = true; q.live = true;
         }
+    }
+    // scene.cpp:441-453, 512-523: walk the shadow ray; the sampled light's radiance over
+    // the solid-angle pdf if the walk ends on it at the sampled distance, else zero.
+    DI Ds<N> neeWalkAD(const ShadowAD &q, Rng &r, real &pdf, const ScatterCtx *scx = nullptr) {
+        int first = -1, ff = -1;
+        real ft = 0._r;
+        Ds<N> ret = lookForEmitterAD(q.o, Dv<N>(q.d), q.on_surface, r, q.med, q.max_inter, first, pdf, &ff, &ft, scx);
+        if (!ret.isZero(EPS) && pdf > EPS && ff >= 0 && __ldg(S.face_shape + ff) == S.emitters[q.light].shape_id &&
+            fabs(ft - q.dist) < SHADOW_EPS)
+            return ret * (1.0_r / pdf);
         return Ds<N>();
     }
 
@@ -309,6 +370,8 @@ template <int N> struct Tracer {
         liStartAt(p, ro, rd, max_depth, med, incEmission, h);
     }
 
+    // The volume and surface branches share one shadow walk and one continuation walk,
+    // so the lanes of a wave on either kind of event traverse the BVH together.
     DI bool liStep(LiPath &p, Rng &r) {
         if (p.depth > p.max_depth) return false;
         const int max_inter = p.max_depth - p.depth - 1;
@@ -322,58 +385,64 @@ template <int N> struct Tracer {
             inside = sampleDistance(S.media[p.med], p.ro, p.rd, its.t, r1a, r1b, r, ps, p.thr);
             if (inside) p.ro = ps;
         }
+        Shadow q;
+        q.live = false;
+        int bt = BSDF_NULL;
         if (inside) {
             if (p.depth >= p.max_depth) return false;
             if (allZero(p.thr, EPS)) return false;
-            const DPhase &ph = S.phases[S.media[p.med].phase_id];
-            f3 wo;
-            real pdf_nee;
-            f3 value = neePoint(p.ro, r2, r, p.med, max_inter, wo, pdf_nee);
-            if (!allZero(value, EPS)) {
-                real pv = phaseEval(ph, -p.rd, wo);
-                if (pv > EPS) p.ret = p.ret + p.thr * value * (misPower(pdf_nee, pv) * pv);
-            }
-            wo = phaseSample(ph, -p.rd, r3[0], r3[1]);
-            p.rd = wo;
-            Hit nx;
-            f3 att = lookForEmitter(p.ro, p.rd, false, r, p.med, max_inter, nx, pdf_nee);
-            p.face = nx.face;
-            if (!allZero(att, EPS)) p.ret = p.ret + p.thr * att * misPower(phaseEval(ph, -p.rd, wo), pdf_nee);
-            p.incEmission = false;
+            neePointSetup(p.ro, r2, p.med, max_inter, q);
         } else {
             if (its.face < 0) return false;
             const DShape &sh = shapeOf(S, its.face);
             if (sh.light_id >= 0 && p.incEmission)
                 p.ret = p.ret + p.thr * emitterEval(S.emitters[sh.light_id], its.gn, its.sn, -p.rd);
             if (p.depth >= p.max_depth) return false;
-            const DBsdf &b = bsdfOf(S, sh);
-            Frame shf(its.sn);
-            f3 wo;
-            real pdf_nee;
-            if (b.type != BSDF_NULL) {
-                f3 value = neeSurface(its, shf, r2, r, p.med, max_inter, wo, pdf_nee, false);
-                if (!allZero(value, EPS))
-                    p.ret = p.ret + p.thr * value * bsdfEval(b, its.wi, wo) * misPower(pdf_nee, bsdfPdf(b, its.wi, wo));
+            bt = bsdfOf(S, sh).type;
+            if (bt != BSDF_NULL) neeSurfaceSetup(its, r2, p.med, max_inter, q);
+        }
+        real pdf_nee = 0._r;
+        const real Tq = q.live ? evalTransmittance(q.o, q.d, q.on_surface, q.med, q.dist, r, q.max_inter) : 0._r;
+        const f3 value = neeFinish(q, Tq, pdf_nee);
+        f3 wo;
+        bool first_only = false;
+        real bpdf = 0._r;
+        f3 bw{1, 1, 1};
+        if (inside) {
+            const DPhase &ph = S.phases[S.media[p.med].phase_id];
+            if (!allZero(value, EPS)) {
+                real pv = phaseEval(ph, -p.rd, q.d);
+                if (pv > EPS) p.ret = p.ret + p.thr * value * (misPower(pdf_nee, pv) * pv);
             }
-            real bpdf = 0._r;
-            f3 bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
+            wo = phaseSample(ph, -p.rd, r3[0], r3[1]);
+            p.rd = wo;
+        } else {
+            const DShape &sh = shapeOf(S, its.face);
+            const DBsdf &b = bsdfOf(S, sh);
+            const Frame shf(its.sn);
+            if (!allZero(value, EPS)) {
+                const f3 wl = shf.toLocal(q.d);
+                p.ret = p.ret + p.thr * value * bsdfEval(b, its.wi, wl) * misPower(pdf_nee, bsdfPdf(b, its.wi, wl));
+            }
+            bw = bsdfSample(b, its.wi, r3[0], r3[1], r3[2], wo, bpdf);
             if (allZero(bw, EPS)) return false;
             wo = shf.toWorld(wo);
             p.ro = its.p;
             p.rd = wo;
             if (isTransition(sh)) p.med = targetMedium(sh, its.gn, wo);
-            if (b.type == BSDF_NULL) {
-                Hit nx;
-                rayIntersect(p.ro, p.rd, true, nx);
-                p.face = nx.face;
-            } else {
-                Hit nx;
-                f3 att = lookForEmitter(p.ro, p.rd, true, r, p.med, max_inter, nx, pdf_nee);
-                p.face = nx.face;
-                p.thr = p.thr * bw;
-                if (!allZero(att, 1e-12_r)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
-                p.incEmission = false;
-            }
+            first_only = bt == BSDF_NULL;
+        }
+        Hit nx;
+        const f3 att = lookForEmitter(p.ro, p.rd, !inside, r, p.med, max_inter, nx, pdf_nee, first_only);
+        p.face = nx.face;
+        if (inside) {
+            const DPhase &ph = S.phases[S.media[p.med].phase_id];
+            if (!allZero(att, EPS)) p.ret = p.ret + p.thr * att * misPower(phaseEval(ph, -p.rd, wo), pdf_nee);
+            p.incEmission = false;
+        } else if (!first_only) {
+            p.thr = p.thr * bw;
+            if (!allZero(att, 1e-12_r)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
+            p.incEmission = false;
         }
         p.depth++;
         return p.depth <= p.max_depth;
