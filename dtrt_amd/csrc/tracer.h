@@ -18,6 +18,12 @@ struct LiPath {
     int face;   // face the ray (ro, rd) hits, -1: none
     int depth, max_depth, med;
     bool incEmission;
+    // The walk along (ro, rd) that the last bounce set up, run at the start of the next
+    // step: 0 none, 1 first hit only (a new ray, or through a null surface), 2 emitter
+    // walk after a volume scattering, 3 after a surface scattering. cpdf is the MIS
+    // partner pdf of an emitter the walk ends on.
+    int walk, walk_max_inter;
+    real cpdf;
 };
 
 template <int N> struct Tracer {
@@ -357,6 +363,9 @@ template <int N> struct Tracer {
         p.med = med;
         p.incEmission = incEmission;
         p.face = first.face;
+        p.walk = 0;
+        p.walk_max_inter = 0;
+        p.cpdf = 0._r;
     }
     // The hit record of the path's current ray (shape.cpp:166-208 on the recorded face).
     DI Hit hitOf(const LiPath &p) const {
@@ -364,15 +373,28 @@ template <int N> struct Tracer {
         if (p.face >= 0) refineHit(S, p.face, p.ro, p.rd, h);
         return h;
     }
+    // The first hit (from ShadowEpsilon) is found by the first step's walk.
     DI void liStart(LiPath &p, f3 ro, f3 rd, int max_depth, int med, bool incEmission) {
-        Hit h;
-        rayIntersect(ro, rd, true, h);
-        liStartAt(p, ro, rd, max_depth, med, incEmission, h);
+        Hit none;
+        liStartAt(p, ro, rd, max_depth, med, incEmission, none);
+        p.walk = 1;
     }
 
     // The volume and surface branches share one shadow walk and one continuation walk,
-    // so the lanes of a wave on either kind of event traverse the BVH together.
+    // so the lanes of a wave on either kind of event traverse the BVH together. The
+    // continuation walk of a bounce runs at the start of the next step, where it also
+    // finds the first hit of a new path's ray: paths just taken from the queue trace
+    // together with the others.
     DI bool liStep(LiPath &p, Rng &r) {
+        if (p.walk) {
+            Hit nx;
+            real pdf_nee = 0._r;
+            const f3 att = lookForEmitter(p.ro, p.rd, p.walk != 2, r, p.med, p.walk_max_inter, nx, pdf_nee, p.walk == 1);
+            p.face = nx.face;
+            if (p.walk == 2 ? !allZero(att, EPS) : (p.walk == 3 && !allZero(att, 1e-12_r)))
+                p.ret = p.ret + p.thr * att * misPower(p.cpdf, pdf_nee);
+            p.walk = 0;
+        }
         if (p.depth > p.max_depth) return false;
         const int max_inter = p.max_depth - p.depth - 1;
         const Hit its = hitOf(p);
@@ -432,20 +454,18 @@ template <int N> struct Tracer {
             if (isTransition(sh)) p.med = targetMedium(sh, its.gn, wo);
             first_only = bt == BSDF_NULL;
         }
-        Hit nx;
-        const f3 att = lookForEmitter(p.ro, p.rd, !inside, r, p.med, max_inter, nx, pdf_nee, first_only);
-        p.face = nx.face;
+        p.walk = inside ? 2 : (first_only ? 1 : 3);
+        p.walk_max_inter = max_inter;
         if (inside) {
-            const DPhase &ph = S.phases[S.media[p.med].phase_id];
-            if (!allZero(att, EPS)) p.ret = p.ret + p.thr * att * misPower(phaseEval(ph, -p.rd, wo), pdf_nee);
+            p.cpdf = phaseEval(S.phases[S.media[p.med].phase_id], -wo, wo);
             p.incEmission = false;
         } else if (!first_only) {
             p.thr = p.thr * bw;
-            if (!allZero(att, 1e-12_r)) p.ret = p.ret + p.thr * att * misPower(bpdf, pdf_nee);
+            p.cpdf = bpdf;
             p.incEmission = false;
         }
         p.depth++;
-        return p.depth <= p.max_depth;
+        return true;   // the walk runs in the next step, also when this was the last bounce
     }
 
     // volpath_simple.cpp:10-86 — VolPathTracerSimple's bounce: next-event estimation
