@@ -722,7 +722,7 @@ struct Scene : Obj {
     // the queue an AD chunk aims to fill, and the largest AD chunk
     int tune_child_blocks = 32;
     double tune_fill = 0.75;
-    uint64_t tune_max_chunk = uint64_t(1) << 21;
+    uint64_t tune_max_chunk = uint64_t(1) << 22;
     int tune_ad_block = 64, tune_child_block = 64;   // work-group sizes of the AD-path and child kernels
     DevBuf tunebuf;
 

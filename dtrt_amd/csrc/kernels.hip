@@ -51,13 +51,13 @@ HDI unsigned blockOr64(int b) { return b == 128 || b == 256 ? (unsigned)b : 64u;
 #define CHILD_WAVES_PER_SIMD (kF64 ? CHILD_WAVES_F64 : 4)
 #endif
 #ifndef CHILD_WAVES_F64
-#define CHILD_WAVES_F64 2   // config[1] at nder 3: 1, 2, 4 waves -> 853, 571, 616 ms per render's child kernel
+#define CHILD_WAVES_F64 2   // config[1] at nder 3, child kernel per render: 2 waves 413 ms, 4 waves 464 ms (1 wave: 1.5x 2)
 #endif
 #ifndef AD_WAVES_PER_SIMD
 #define AD_WAVES_PER_SIMD (kF64 ? AD_WAVES_F64 : 2)
 #endif
 #ifndef AD_WAVES_F64
-#define AD_WAVES_F64 1      // 1 wave: 498 ms, 2 waves: 553 ms per render's AD kernel (config[1], nder 3)
+#define AD_WAVES_F64 1      // 1 wave: 394 ms, 2 waves: 514 ms per render's AD kernel (config[1], nder 3)
 #endif
 // Dual widths from which the VolPathTracerAD samples run in k_ad_refill (per-bounce
 // lane refill) at one wave per SIMD, with the whole 512-entry register file, rather
@@ -71,9 +71,10 @@ HDI unsigned blockOr64(int b) { return b == 128 || b == 256 ? (unsigned)b : 64u;
 template <int N> constexpr bool adRefill() { return N >= AD_REFILL_MIN_N; }
 template <int N> constexpr int adWaves() { return adRefill<N>() ? 1 : AD_WAVES_PER_SIMD; }
 // Grid of k_ad_refill: waves per CU sharing a launch's samples (measured on the
-// 100K-face dielectric scene at nder 8: 64 beats 32 and 128).
+// 100K-face dielectric scene at nder 8: in float 64 beats 32 and 128; in double the
+// AD kernel takes 8.16, 7.76, 7.62, 7.48, 7.84, 8.78 s at 4, 8, 16, 32, 64, 128).
 #ifndef AD_REFILL_WAVES_PER_CU
-#define AD_REFILL_WAVES_PER_CU 64
+#define AD_REFILL_WAVES_PER_CU (kF64 ? 32 : 64)
 #endif
 
 DI uint32_t xcdRemap(uint32_t b, uint32_t nb) {
