@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from dtrt_amd import scenes
-from tests.helpers import assert_3sigma, block_z, renders
+from tests.helpers import assert_parity, renders
 from tests.test_gpu_adjoint import INTERIOR, _density_scene, _support_voxels
 from tests.test_gpu_kernels import _global_face, _rays
 
@@ -40,8 +40,7 @@ def test_cfg2_full_mesh_parity(gpu, oracle, cfg2, mode):
     a = renders(gpu, d, range(16), mode=mode, sppe=sppe)
     b = renders(oracle, d, range(1000, 1012), mode=mode, sppe=4 * sppe, spp=4 * d.spp)
     assert np.abs(b[:, 1:].mean(0)).max() > 0   # the term has tangents in the crop
-    z = block_z(a, b, block)
-    assert_3sigma(z, what="cfg2 full mesh, mode %d" % mode, max_mean_z2=1.8)
+    assert_parity(a, b, block, what="cfg2 full mesh, mode %d" % mode, max_mean_z2=1.8)
 
 
 @pytest.mark.parametrize("which", ["cfg2", "cfg4"])

@@ -9,7 +9,7 @@ import pytest
 
 from dtrt_amd import loaders, scenes
 from dtrt_amd.harness import SceneManager, SceneTransform
-from tests.helpers import assert_3sigma, block_z
+from tests.helpers import assert_3sigma, block_z, assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,7 @@ def test_example_scene_ad_parity(gpu, oracle, name):
     a = np.stack([scenes.render(gpu, d, seed=s, scene=sg, **kw) for s in range(16)]).astype(np.float64)
     b = np.stack([scenes.render(oracle, d, seed=100 + s, scene=so, **kw) for s in range(16)]).astype(np.float64)
     assert np.abs(a[:, 1]).mean() > 0
-    assert_3sigma(block_z(a, b, 4), what=name, max_mean_z2=1.8)
+    assert_parity(a, b, 4, what=name, max_mean_z2=1.8)
 
 
 def test_glass_1_primal_parity(gpu, oracle):
@@ -47,7 +47,7 @@ def test_glass_1_primal_parity(gpu, oracle):
     a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, spp=32, max_bounces=4)[None] for s in range(16)])
     b = np.stack([scenes.render_primal(oracle, d, seed=50 + s, scene=so, spp=32, max_bounces=4)[None]
                   for s in range(16)])
-    assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 4), what="glass_1 primal")
+    assert_parity(a.astype(np.float64), b.astype(np.float64), 4, what="glass_1 primal")
 
 
 def test_glass_1_optimisation_follows_the_reference_trace(gpu):

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from dtrt_amd import scenes
-from tests.helpers import assert_3sigma, block_z, renders
+from tests.helpers import assert_3sigma, block_z, renders, assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,7 @@ def test_fp32_leg_matches_the_oracle(gpu, oracle, which):
     with precision(gpu, 32):
         a = renders(gpu, d, range(16), sppe=sppe)
     b = renders(oracle, d, range(1000, 1016), sppe=sppe)
-    assert_3sigma(block_z(a, b, 4), what="fp32 " + which)
+    assert_parity(a, b, 4, what="fp32 " + which)
 
 
 def test_fp64_and_fp32_legs_agree(gpu):
@@ -50,4 +50,4 @@ def test_fp64_and_fp32_legs_agree(gpu):
     a = renders(gpu, d, range(24), sppe=256)
     with precision(gpu, 32):
         b = renders(gpu, d, range(100, 124), sppe=256)
-    assert_3sigma(block_z(a, b, 4), what="fp64 vs fp32")
+    assert_parity(a, b, 4, what="fp64 vs fp32")

@@ -7,7 +7,7 @@ import pytest
 
 from dtrt_amd import scenes
 from tests import scene_fixtures as F
-from tests.helpers import assert_3sigma, block_z
+from tests.helpers import assert_3sigma, block_z, assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -48,7 +48,7 @@ def test_primal_parity(gpu, oracle, cfg, simple):
     k = 32 if cfg == "cfg2" else 16   # the near-specular dielectric gives heavy-tailed samples
     a = _stack(gpu, d, range(k), simple=simple)
     b = _stack(oracle, d, range(100, 100 + k), simple=simple)
-    assert_3sigma(block_z(a, b, 3), what=("VolPathTracerSimple " if simple else "VolPathTracer ") + cfg)
+    assert_parity(a, b, 3, what=("VolPathTracerSimple " if simple else "VolPathTracer ") + cfg)
 
 
 def test_primal_every_pixel_written_and_device_output(gpu):
@@ -99,7 +99,7 @@ def test_surface_integrators_parity(gpu, oracle, cfg, kind):
     sg, so = scenes.instantiate(gpu, d), scenes.instantiate(oracle, d)
     a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, kind=kind)[None] for s in range(k)])
     b = np.stack([scenes.render_primal(oracle, d, seed=100 + s, scene=so, kind=kind)[None] for s in range(k)])
-    assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 3), what="kind %d %s" % (kind, cfg))
+    assert_parity(a.astype(np.float64), b.astype(np.float64), 3, what="kind %d %s" % (kind, cfg))
 
 
 def test_particle_tracer_closed_forms(gpu):
@@ -123,4 +123,4 @@ def test_particle_tracer_parity(gpu, oracle, cfg):
     sg, so = scenes.instantiate(gpu, d), scenes.instantiate(oracle, d)
     a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, kind=5, spp=4)[None] for s in range(16)])
     b = np.stack([scenes.render_primal(oracle, d, seed=100 + s, scene=so, kind=5, spp=4)[None] for s in range(16)])
-    assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 3), what="ParticleTracer " + cfg)
+    assert_parity(a.astype(np.float64), b.astype(np.float64), 3, what="ParticleTracer " + cfg)

@@ -7,7 +7,7 @@ import pytest
 
 from dtrt_amd import scenes
 from tests import scene_fixtures as F
-from tests.helpers import assert_3sigma, block_z, renders
+from tests.helpers import assert_3sigma, block_z, renders, sum_z
 
 pytestmark = pytest.mark.gpu
 
@@ -78,8 +78,11 @@ def test_albedo_tangent_equals_crn_fd(gpu):
 
 
 def _parity(gpu, oracle, d, mode=-1, seeds=16, block=4, **kw):
+    """Block z-scores GPU vs oracle; the image sums of every channel must agree too."""
     a = renders(gpu, d, range(seeds), mode=mode, **kw)
     b = renders(oracle, d, range(1000, 1000 + seeds), mode=mode, **kw)
+    zs = sum_z(a, b)
+    assert (np.abs(zs) <= 4).all(), "%s mode %d: image-sum z-scores %s" % (d.name, mode, np.array2string(zs, precision=2))
     return block_z(a, b, block)
 
 

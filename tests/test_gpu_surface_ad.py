@@ -5,7 +5,7 @@ import pytest
 
 from dtrt_amd import scenes
 from tests import scene_fixtures as F
-from tests.helpers import assert_3sigma, block_z
+from tests.helpers import assert_3sigma, block_z, assert_parity
 
 pytestmark = pytest.mark.gpu
 NAMES = ["path", "direct"]
@@ -54,7 +54,7 @@ def test_parity(gpu, oracle, cfg, name):
     a = np.stack([scenes.render(gpu, d, seed=s, scene=sg, integrator=_integ(gpu, name), **kw) for s in range(k)])
     b = np.stack([scenes.render(oracle, d, seed=1000 + s, scene=so, integrator=_integ(oracle, name), **kw)
                   for s in range(k)])
-    assert_3sigma(block_z(a.astype(np.float64), b.astype(np.float64), 4), what="%s %s" % (name, cfg), max_mean_z2=1.8)
+    assert_parity(a.astype(np.float64), b.astype(np.float64), 4, what="%s %s" % (name, cfg), max_mean_z2=1.8)
 
 
 def test_path_ad_overflow_replay_is_exact(gpu):
