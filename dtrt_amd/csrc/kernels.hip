@@ -581,6 +581,12 @@ template <int N> struct AdState {
     int hface;   // face the ray (ro, rd) hits; the bounce recomputes the hit record from it
     Ds<N> beta;
     int med, depth, mode;
+    // The walk along (ro, rd) still to run at the start of the next step: 0 none, 1 first
+    // hit only from a surface, 2 emitter walk, 3 the camera ray's first hit (from 0);
+    // walk_add: its emitter term counts, with MIS partner pdf walk_cpdf.
+    int walk;
+    bool walk_surf, walk_add;
+    real walk_cpdf;
 };
 
 // adPathSample up to its bounce loop: camera ray and first hit. False when the path
@@ -588,9 +594,11 @@ template <int N> struct AdState {
 template <int N>
 DI bool adBegin(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, AdState<N> &st) {
     adCameraRay<N>(vw, r, pixel, st.ro, st.rd);
-    real tcam;
-    st.hface = T.trace(st.ro.val(), st.rd.val(), 0._r, FINF, tcam);
-    if (st.hface < 0) return false;
+    st.walk = 3;   // the first hit is found by the first step's walk, with the other lanes' walks
+    st.walk_surf = false;
+    st.walk_add = false;
+    st.walk_cpdf = 0._r;
+    st.hface = -1;
     st.med = vw.cam.med_id;
     st.depth = P.max_bounces;
     st.mode = P.mode;
@@ -609,8 +617,22 @@ DI bool adBounce(Tracer<N> &T, const RenderParams &P, Rng &r, int pixel, AdState
     int &hface = st.hface;
     Ds<N> &beta = st.beta;
     int &med = st.med, &depth = st.depth, &mode = st.mode;
+    if (st.walk) {   // the walk the last step handed over (or the camera ray's first hit)
+        real pdf_nee = 0._r;
+        int nface = -1;
+        Ds<N> att = T.lookForEmitterAD(ro, rd, st.walk_surf, r, med, depth, nface, pdf_nee, nullptr, nullptr, nullptr,
+                                       st.walk != 2);
+        hface = nface;
+        if (st.walk_add && !att.isZero(EPS)) L += beta * (att * misPower(st.walk_cpdf, pdf_nee));
+        const bool camera = st.walk == 3;
+        st.walk = 0;
+        if (camera && hface < 0) return false;
+    }
 #define AD_BOUNCE_END return false
+#define AD_WALK_DEFERRED(kind, surf, add, cp) \
+    { st.walk = (kind); st.walk_surf = (surf); st.walk_add = (add); st.walk_cpdf = (cp); }
 #include "ad_bounce.inc"
+#undef AD_WALK_DEFERRED
 #undef AD_BOUNCE_END
     return mode != 0;
 }
