@@ -863,7 +863,7 @@ bool frameRotationReady() {
 // wave the device holds at once. The limits are the largest figures that a kernel has
 // passed the GPU parity suite with, rounded up: 10,360 bytes per lane and 679 MB over the
 // device (a build of the double-precision nder-16 refill kernel: 10,360 B x 64 lanes x 4
-// waves per CU x 256 CUs; the current build needs 7,888 B). A width-16 build of the
+// waves per CU x 256 CUs; the current build needs 7,104 B). A width-16 build of the
 // single-precision AD-path kernel held to two waves per SIMD, whose spills needed 7.9 KB
 // per lane (1.04 GB over the device), rendered empty images: every path ended after two
 // segments. Its cause is not known; per-lane size alone is not it (the 10,360-byte
