@@ -197,8 +197,8 @@ DI int isSilhouette(const DScene &S, int4 e, f3 p, bool hasNormals) {
 // :513-616 (interface). Accepted samples push two primal sub-paths with +/- weights.
 template <int N>
 DI void edgeTerm(Tracer<N> &T, const RenderParams &P, Rng &r, bool volume, const Dv<N> &x, const Hit *its0,
-                 const DBsdf *b, const DPhase *ph, f3 rd_in, f3 thrv, f3 betav, int med, int depth, int pixel,
-                 Pusher &u, unsigned *stats) {
+                 const DBsdf *b, const DPhase *ph, f3 rd_in, f3 thrv, f3 betav, int med, int depth, Pusher &u,
+                 unsigned *stats) {
     const DScene &S = T.S;
     if (!S.edges_valid) return;
     const int ne = P.sppse;
@@ -592,7 +592,7 @@ template <int N> struct AdState {
 // adPathSample up to its bounce loop: camera ray and first hit. False when the path
 // has nothing to do.
 template <int N>
-DI bool adBegin(Tracer<N> &T, const RenderParams &P, const DView &vw, Rng &r, int pixel, AdState<N> &st) {
+DI bool adBegin(const RenderParams &P, const DView &vw, Rng &r, int pixel, AdState<N> &st) {
     adCameraRay<N>(vw, r, pixel, st.ro, st.rd);
     st.walk = 3;   // the first hit is found by the first step's walk, with the other lanes' walks
     st.walk_surf = false;
@@ -707,7 +707,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, Ren
                 r.init(vw.seed, ls);
                 L = Ds<N>();
                 have = true;
-                alive = adBegin<N>(T, P, vw, r, pixel, st);
+                alive = adBegin<N>(P, vw, r, pixel, st);
             } else {
                 exhausted = true;
             }
