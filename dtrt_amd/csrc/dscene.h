@@ -71,7 +71,27 @@ struct TraceHit {
     uint32_t nbox, ntri;
 };
 
-DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3 o_, f3 d_, real tnear_, real tfar_) {
+// A small scene's tree (the nodes, then the leaf-ordered triangle records) can live in
+// LDS after the traversal stack: sceneLdsBytes bytes, 0 when it is larger than
+// LDS_TREE_MAX (traversal then reads global memory).
+constexpr size_t LDS_TREE_MAX = 32 << 10;
+HDI size_t sceneLdsBytes(const DScene &S) {
+    const size_t b = (size_t)S.nnodes * 64 + (size_t)S.ntris * 48;
+    return S.nnodes > 0 && b <= LDS_TREE_MAX ? b : 0;
+}
+DI const float4 *ldsTree() { return reinterpret_cast<const float4 *>(s_bvh_stack + DTRT_BVH_STACK * blockDim.x); }
+// Copies the tree into LDS; called by every thread of the block, at the top of a kernel
+// launched with sceneLdsBytes(S) more bytes of dynamic LDS.
+DI void treeToLds(const DScene &S) {
+    float4 *dst = reinterpret_cast<float4 *>(s_bvh_stack + DTRT_BVH_STACK * blockDim.x);
+    const int nn = 4 * S.nnodes, nt = 3 * S.ntris;
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) dst[i] = S.nodes[i];
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) dst[nn + i] = S.tris[i];
+    __syncthreads();
+}
+
+DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3 o_, f3 d_, real tnear_, real tfar_,
+                         bool lds_tree = false) {
     const fv3 o{(float)o_.x, (float)o_.y, (float)o_.z}, d{(float)d_.x, (float)d_.y, (float)d_.z};
     const float tnear = (float)tnear_, tfar = (float)tfar_;
     TraceHit res{-1, tfar, 0u, 0u};
@@ -90,8 +110,14 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
         // inner nodes until this lane reaches a leaf or runs out of nodes; the wave then
         // tests the leaves it reached together
         while (node >= 0) {
-            const float4 *n = nodes + 4 * (size_t)node;
-            float4 a = n[0], b = n[1], c = n[2], e = n[3];
+            float4 a, b, c, e;
+            if (lds_tree) {
+                const float4 *n = ldsTree() + 4 * node;
+                a = n[0]; b = n[1]; c = n[2]; e = n[3];
+            } else {
+                const float4 *n = nodes + 4 * (size_t)node;
+                a = n[0]; b = n[1]; c = n[2]; e = n[3];
+            }
             float tl, tr;
             nbox += 2;
             bool hl = boxHit(a, b, o, inv, tnear, best, tl);
@@ -114,8 +140,14 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
         int first = enc >> 4, cnt = enc & 15;
         ntri += cnt;
         for (int k = first; k < first + cnt; ++k) {
-            const float4 *tp = tris + 3 * (size_t)k;
-            float4 v0 = tp[0], e1 = tp[1], e2 = tp[2];
+            float4 v0, e1, e2;
+            if (lds_tree) {
+                const float4 *tp = ldsTree() + 4 * nnodes + 3 * k;
+                v0 = tp[0]; e1 = tp[1]; e2 = tp[2];
+            } else {
+                const float4 *tp = tris + 3 * (size_t)k;
+                v0 = tp[0]; e1 = tp[1]; e2 = tp[2];
+            }
             fv3 E1{e1.x, e1.y, e1.z}, E2{e2.x, e2.y, e2.z};
             fv3 pv = fcross(d, E2);
             float det = fdot(E1, pv);

@@ -723,7 +723,9 @@ struct Scene : Obj {
     int tune_child_blocks = 32;
     double tune_fill = 0.75;
     uint64_t tune_max_chunk = uint64_t(1) << 22;
-    int tune_ad_block = 64, tune_child_block = 64;   // work-group sizes of the AD-path and child kernels
+    // work-group sizes of the AD-path and child kernels; the child kernel's 256 lets a
+    // small scene's tree sit in LDS at its full occupancy (treeLdsFor in kernels.hip)
+    int tune_ad_block = 64, tune_child_block = 256;
     DevBuf tunebuf;
 
     template <class T> const T *upload(const std::vector<T> &v) {

@@ -525,6 +525,7 @@ __global__ __launch_bounds__(MAX_BLOCK, adWaves<N>()) void k_ad_paths(DScene S, 
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * blockDim.x + threadIdx.x;
     Tracer<N> T(S);
+    T.useLdsTree(P.lds_tree);
     Ds<N> L;
     const bool active = li < nsamp;
     uint64_t sid = s0 + li;
@@ -655,6 +656,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, Ren
     const uint64_t begin = (uint64_t)blk * per_wave;
     const uint64_t end = begin < nsamp ? min(nsamp, begin + per_wave) : begin;
     Tracer<N> T(S);
+    T.useLdsTree(P.lds_tree);
     AdState<N> st;
     Ds<N> L;
     Rng r;
@@ -943,6 +945,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
     Tracer<N> T(S);
+    T.useLdsTree(P.lds_tree);
     const uint64_t sid = s0 + li;
     const int v = viewOf(S, sid);
     const DView &vw = S.views[v];
@@ -1138,6 +1141,7 @@ template <int N>
 __global__ __launch_bounds__(MAX_BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
                                                     unsigned *qnext, real *acc, real *acc_pe, unsigned long long *work) {
     Tracer<N> T(S);
+    T.useLdsTree(P.lds_tree);
     const unsigned n = min(*qcount, P.qcap);
     const int lane = threadIdx.x & 63;
     uint32_t nlk = 0;
@@ -1204,6 +1208,7 @@ __global__ __launch_bounds__(MAX_BLOCK, CHILD_WAVES_PER_SIMD) void k_children_pa
                                                                             real *acc, real *acc_pe,
                                                                             unsigned long long *work) {
     Tracer<N> T(S);
+    T.useLdsTree(P.lds_tree);
     const unsigned n = min(*qcount, P.qcap);
     const int lane = threadIdx.x & 63;
     uint32_t nlk = 0;
@@ -1336,6 +1341,7 @@ __global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, int
                                                    uint64_t total, char *q, unsigned *qcount, unsigned *stats,
                                                    unsigned long long *work) {
     Tracer<N> T(S);
+    T.useLdsTree(P.lds_tree);
     const uint64_t li = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t sid = s0 + li;
     Rng r;
@@ -1472,6 +1478,21 @@ static hipError_t addKernelInfo(F fn, const char *name, int block, size_t lds, K
     return hipSuccess;
 }
 
+// LDS bytes for the scene's tree in a launch of `kern` with blocks of `blk`: the
+// tree's size when it fits (sceneLdsBytes) and a CU then holds as many blocks as
+// with the traversal stacks alone, else 0 (the launch reads the tree from HBM).
+template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &S) {
+    const size_t tb = sceneLdsBytes(S);
+    if (tb == 0) return 0;
+    int base = 0, with = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, kern, (int)blk, traceLds(blk)) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kern, (int)blk, traceLds(blk) + tb) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return base > 0 && with >= base ? tb : 0;
+}
+
 template <class Sym> static hipError_t setRotation(const Sym &sym, const double *m9) {
     FrameRot r;
     for (int i = 0; i < 9; ++i) r.m[i] = (real)m9[i];
@@ -1487,13 +1508,18 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
                                       unsigned long long *wk, hipStream_t sm) {
     const unsigned blk = blockOr64(P.ad_block);
     unsigned nb = (unsigned)((n + blk - 1) / blk);
-    if (P.ad_kind == AD_PATH)
-        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(blk), traceLds(blk), sm, S, P, s0, n, replay, acc, q, qc,
-                           st, ovf, ovf_count, wk);
-    else if (P.ad_kind == AD_DIRECT)
-        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(blk), traceLds(blk), sm, S, P, s0, n, replay, acc, q,
-                           qc, st, ovf, ovf_count, wk);
-    else if constexpr (adRefill<N>()) {
+    RenderParams Pl = P;
+    if (P.ad_kind == AD_PATH) {
+        const size_t tl = treeLdsFor(k_ad_paths<N, AD_PATH>, blk, S);
+        Pl.lds_tree = tl > 0;
+        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay, acc,
+                           q, qc, st, ovf, ovf_count, wk);
+    } else if (P.ad_kind == AD_DIRECT) {
+        const size_t tl = treeLdsFor(k_ad_paths<N, AD_DIRECT>, blk, S);
+        Pl.lds_tree = tl > 0;
+        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay,
+                           acc, q, qc, st, ovf, ovf_count, wk);
+    } else if constexpr (adRefill<N>()) {
         // one wave per block, AD_REFILL_WAVES_PER_CU of them per CU share the samples
         int dev = 0, ncu = 0;
         (void)hipGetDevice(&dev);
@@ -1501,28 +1527,43 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
         const uint64_t waves = (uint64_t)std::max(ncu, 1) * AD_REFILL_WAVES_PER_CU;
         const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, waves));
         const uint64_t per_wave = (n + grid - 1) / grid;
+        const size_t tl = treeLdsFor(k_ad_refill<N>, BLOCK, S);
+        Pl.lds_tree = tl > 0;
         // one wave per block: each owns a contiguous range of samples
-        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, per_wave,
-                           replay, acc, q, qc, st, ovf, ovf_count, wk);
+        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, s0, n,
+                           per_wave, replay, acc, q, qc, st, ovf, ovf_count, wk);
     } else {
-        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(blk), traceLds(blk), sm, S, P, s0, n, replay, acc, q,
-                           qc, st, ovf, ovf_count, wk);
+        const size_t tl = treeLdsFor(k_ad_paths<N, AD_VOLPATH>, blk, S);
+        Pl.lds_tree = tl > 0;
+        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay,
+                           acc, q, qc, st, ovf, ovf_count, wk);
     }
 }
 template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderParams &P, const char *q, unsigned *qc,
                                             real *acc, real *acc_pe, unsigned long long *wk, unsigned grid, hipStream_t sm) {
     const unsigned blk = blockOr64(P.child_block);   // grid: blocks of `blk` threads
-    if (P.ad_kind == AD_VOLPATH)
-        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(blk), traceLds(blk), sm, S, P, q, qc, qc + 2, acc, acc_pe, wk);
-    else
-        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(blk), traceLds(blk), sm, S, P, q, qc, qc + 2, acc, acc_pe,
+    RenderParams Pl = P;
+    if (P.ad_kind == AD_VOLPATH) {
+        const size_t tl = treeLdsFor(k_children<N>, blk, S);
+        Pl.lds_tree = tl > 0;
+        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(blk), traceLds(blk) + tl, sm, S, Pl, q, qc, qc + 2, acc, acc_pe,
                            wk);
+    } else {
+        const size_t tl = treeLdsFor(k_children_path<N>, blk, S);
+        Pl.lds_tree = tl > 0;
+        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(blk), traceLds(blk) + tl, sm, S, Pl, q, qc, qc + 2, acc,
+                           acc_pe, wk);
+    }
 }
 template <int N> void WidthOps<N>::launchPrimary(const DScene &S, const RenderParams &P, int view, uint64_t s0, uint64_t n,
                                            uint64_t total, char *q, unsigned *qc, unsigned *st, unsigned long long *wk,
                                            hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, view, s0, n, total, q, qc, st, wk);
+    RenderParams Pl = P;
+    const size_t tl = treeLdsFor(k_primary<N>, BLOCK, S);
+    Pl.lds_tree = tl > 0;
+    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, view, s0, n, total, q, qc, st,
+                       wk);
 }
 template <int N> void WidthOps<N>::launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const real *acc,
                                             const real *acc_pe, float *out, unsigned *st, hipStream_t sm) {
@@ -1536,7 +1577,11 @@ template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderPa
                                                  const float *dimg, float *grad, int pmed, unsigned *st,
                                                  unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, dimg, grad, pmed, st, wk);
+    RenderParams Pl = P;
+    const size_t tl = treeLdsFor(k_adjoint<N>, BLOCK, S);
+    Pl.lds_tree = tl > 0;
+    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, s0, n, dimg, grad, pmed, st,
+                       wk);
 }
 template <int N> hipError_t WidthOps<N>::kernelInfo(KernelInfo *out, int cap, int *count) {
     char nm[40];

@@ -29,11 +29,19 @@ struct LiPath {
 template <int N> struct Tracer {
     const DScene &S;
     uint32_t nbox = 0, ntri = 0, nseg = 0;   // work counters: box tests, triangle tests, traced segments
+    bool lds_tree = false;                   // the BVH is in LDS (treeToLds)
     DI explicit Tracer(const DScene &s) : S(s) {}
+    // At the top of a kernel, by every thread: copy the tree into LDS if the launch made room.
+    DI void useLdsTree(int on) {
+        if (on) {
+            treeToLds(S);
+            lds_tree = true;
+        }
+    }
 
     DI int trace(f3 o, f3 d, real tnear, real tfar, real &t) {
         nseg++;
-        TraceHit h = traceClosest(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar);
+        TraceHit h = traceClosest(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar, lds_tree);
         nbox += h.nbox;
         ntri += h.ntri;
         t = h.t;
