@@ -228,6 +228,24 @@ template <int N> DI void refineHitAD(const DScene &S, int face, const Dv<N> &o, 
     h.wi = FrameAD<N>(sn).toLocal(-d);
 }
 
+// The distance of refineHitAD alone, with the geometric normal's value oriented as
+// there (towards the shading normal's side): what a walk through a null interface needs.
+template <int N> DI Df<N> hitDistanceAD(const DScene &S, int face, const Dv<N> &o, const Dv<N> &d, f3 &gn) {
+    const int nd = N;
+    const int *fi = S.F + 3 * (size_t)face;
+    int i0 = __ldg(fi), i1 = __ldg(fi + 1), i2 = __ldg(fi + 2);
+    Df<N> u, v, t;
+    rayTriangleUVT_AD<N>(ldDv<N>(S.V + 3 * (size_t)i0, S.Vd + (size_t)i0 * nd * 3), ldDv<N>(S.V + 3 * (size_t)i1, S.Vd + (size_t)i1 * nd * 3),
+                         ldDv<N>(S.V + 3 * (size_t)i2, S.Vd + (size_t)i2 * nd * 3), o, d, u, v, t);
+    gn = ldg3(S.FN + 3 * (size_t)face);
+    if (shapeOf(S, face).has_normals) {
+        const real w = 1.0_r - (u.v + v.v);
+        const f3 nn = ldg3(S.Nrm + 3 * (size_t)i0) * w + ldg3(S.Nrm + 3 * (size_t)i1) * u.v + ldg3(S.Nrm + 3 * (size_t)i2) * v.v;
+        if (dot(gn, nn) < 0._r) gn = -gn;
+    }
+    return t;
+}
+
 DI int targetMedium(const DShape &sh, f3 gn, f3 d) { return dot(d, gn) > 0._r ? sh.med_ext : sh.med_int; }
 DI bool isTransition(const DShape &sh) { return sh.med_ext >= 0 || sh.med_int >= 0; }
 DI const DBsdf &bsdfOf(const DScene &S, const DShape &sh) { return S.bsdfs[sh.bsdf_id]; }
