@@ -448,14 +448,31 @@ bool buildBvh(const std::vector<std::array<float, 9>> &tri, const std::vector<in
         out.depth = 1;
         return true;
     }
-    std::vector<int> stack{0};
-    while (!stack.empty()) {   // DFS preorder numbering of inner nodes
-        int b = stack.back();
-        stack.pop_back();
+    // Inner nodes are numbered breadth-first for the first BVH_TOP_NODES (the top levels,
+    // which a launch may keep in LDS as a prefix of the node array), then depth-first
+    // (preorder) below each remaining subtree root in that order, which keeps a subtree
+    // together in memory.
+    constexpr size_t BVH_TOP_NODES = 1024;
+    std::vector<int> level{0};
+    size_t head = 0;
+    while (head < level.size() && order.size() < BVH_TOP_NODES) {
+        const int b = level[head++];
         gpuIndex[b] = (int)order.size();
         order.push_back(b);
-        for (int c : {bn[b].right, bn[b].left})
-            if (bn[c].left >= 0) stack.push_back(c);
+        for (int c : {bn[b].left, bn[b].right})
+            if (bn[c].left >= 0) level.push_back(c);
+    }
+    std::vector<int> stack;
+    for (size_t k = head; k < level.size(); ++k) {
+        stack.push_back(level[k]);
+        while (!stack.empty()) {
+            const int b = stack.back();
+            stack.pop_back();
+            gpuIndex[b] = (int)order.size();
+            order.push_back(b);
+            for (int c : {bn[b].right, bn[b].left})
+                if (bn[c].left >= 0) stack.push_back(c);
+        }
     }
     out.nodes.resize(order.size() * 4);
     auto enc = [&](int c) {

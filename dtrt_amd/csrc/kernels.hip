@@ -525,7 +525,7 @@ __global__ __launch_bounds__(MAX_BLOCK, adWaves<N>()) void k_ad_paths(DScene S, 
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * blockDim.x + threadIdx.x;
     Tracer<N> T(S);
-    T.useLdsTree(P.lds_tree);
+    T.useLdsTree(P);
     Ds<N> L;
     const bool active = li < nsamp;
     uint64_t sid = s0 + li;
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, Ren
     const uint64_t begin = (uint64_t)blk * per_wave;
     const uint64_t end = begin < nsamp ? min(nsamp, begin + per_wave) : begin;
     Tracer<N> T(S);
-    T.useLdsTree(P.lds_tree);
+    T.useLdsTree(P);
     AdState<N> st;
     Ds<N> L;
     Rng r;
@@ -945,7 +945,7 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_adjoint(DScene S, Rende
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
     Tracer<N> T(S);
-    T.useLdsTree(P.lds_tree);
+    T.useLdsTree(P);
     const uint64_t sid = s0 + li;
     const int v = viewOf(S, sid);
     const DView &vw = S.views[v];
@@ -1141,7 +1141,7 @@ template <int N>
 __global__ __launch_bounds__(MAX_BLOCK, CHILD_WAVES_PER_SIMD) void k_children(DScene S, RenderParams P, const char *q, const unsigned *qcount,
                                                     unsigned *qnext, real *acc, real *acc_pe, unsigned long long *work) {
     Tracer<N> T(S);
-    T.useLdsTree(P.lds_tree);
+    T.useLdsTree(P);
     const unsigned n = min(*qcount, P.qcap);
     const int lane = threadIdx.x & 63;
     uint32_t nlk = 0;
@@ -1208,7 +1208,7 @@ __global__ __launch_bounds__(MAX_BLOCK, CHILD_WAVES_PER_SIMD) void k_children_pa
                                                                             real *acc, real *acc_pe,
                                                                             unsigned long long *work) {
     Tracer<N> T(S);
-    T.useLdsTree(P.lds_tree);
+    T.useLdsTree(P);
     const unsigned n = min(*qcount, P.qcap);
     const int lane = threadIdx.x & 63;
     uint32_t nlk = 0;
@@ -1341,7 +1341,7 @@ __global__ __launch_bounds__(BLOCK) void k_primary(DScene S, RenderParams P, int
                                                    uint64_t total, char *q, unsigned *qcount, unsigned *stats,
                                                    unsigned long long *work) {
     Tracer<N> T(S);
-    T.useLdsTree(P.lds_tree);
+    T.useLdsTree(P);
     const uint64_t li = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t sid = s0 + li;
     Rng r;
@@ -1478,19 +1478,49 @@ static hipError_t addKernelInfo(F fn, const char *name, int block, size_t lds, K
     return hipSuccess;
 }
 
-// LDS bytes for the scene's tree in a launch of `kern` with blocks of `blk`: the
-// tree's size when it fits (sceneLdsBytes) and a CU then holds as many blocks as
-// with the traversal stacks alone, else 0 (the launch reads the tree from HBM).
-template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &S) {
-    const size_t tb = sceneLdsBytes(S);
-    if (tb == 0) return 0;
-    int base = 0, with = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, kern, (int)blk, traceLds(blk)) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kern, (int)blk, traceLds(blk) + tb) != hipSuccess) {
+// The part of the scene's tree a launch of `kern` with blocks of `blk` keeps in LDS
+// (Pl.lds_nodes, Pl.lds_tris), chosen so that a CU holds as many blocks as with the
+// traversal stacks alone: all nodes and triangle records when they fit, else, in a
+// kernel built to read a prefix (`prefix`: treePrefixInLds), the first nodes (the top
+// levels), at least 64 of them. Returns the extra LDS bytes.
+template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &S, RenderParams &Pl, bool prefix) {
+    Pl.lds_nodes = 0;
+    Pl.lds_tris = 0;
+    if (S.nnodes <= 0) return 0;
+    int dev = 0, per_cu = 0, per_block = 0, base = 0, with = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, kern, (int)blk, traceLds(blk)) != hipSuccess || base <= 0) {
         (void)hipGetLastError();
         return 0;
     }
-    return base > 0 && with >= base ? tb : 0;
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern)) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const size_t used = traceLds(blk) + fa.sharedSizeBytes;
+    const size_t room = std::min<size_t>((size_t)per_cu / (size_t)base, (size_t)per_block);
+    if (room <= used) return 0;
+    const size_t spare = room - used;
+    if (treeLdsBytes(S, S.nnodes, true) <= spare) {
+        Pl.lds_nodes = S.nnodes;
+        Pl.lds_tris = 1;
+    } else if (prefix) {
+        Pl.lds_nodes = (int)std::min<size_t>((size_t)S.nnodes, spare / 64);
+        if (Pl.lds_nodes < 64) Pl.lds_nodes = 0;
+    }
+    const size_t tb = treeLdsBytes(S, Pl.lds_nodes, Pl.lds_tris != 0);
+    if (tb == 0 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kern, (int)blk, traceLds(blk) + tb) != hipSuccess ||
+        with < base) {
+        (void)hipGetLastError();
+        Pl.lds_nodes = 0;
+        Pl.lds_tris = 0;
+        return 0;
+    }
+    return tb;
 }
 
 template <class Sym> static hipError_t setRotation(const Sym &sym, const double *m9) {
@@ -1510,13 +1540,11 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
     unsigned nb = (unsigned)((n + blk - 1) / blk);
     RenderParams Pl = P;
     if (P.ad_kind == AD_PATH) {
-        const size_t tl = treeLdsFor(k_ad_paths<N, AD_PATH>, blk, S);
-        Pl.lds_tree = tl > 0;
+        const size_t tl = treeLdsFor(k_ad_paths<N, AD_PATH>, blk, S, Pl, treePrefixInLds<N>());
         hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay, acc,
                            q, qc, st, ovf, ovf_count, wk);
     } else if (P.ad_kind == AD_DIRECT) {
-        const size_t tl = treeLdsFor(k_ad_paths<N, AD_DIRECT>, blk, S);
-        Pl.lds_tree = tl > 0;
+        const size_t tl = treeLdsFor(k_ad_paths<N, AD_DIRECT>, blk, S, Pl, treePrefixInLds<N>());
         hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay,
                            acc, q, qc, st, ovf, ovf_count, wk);
     } else if constexpr (adRefill<N>()) {
@@ -1527,14 +1555,12 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
         const uint64_t waves = (uint64_t)std::max(ncu, 1) * AD_REFILL_WAVES_PER_CU;
         const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, waves));
         const uint64_t per_wave = (n + grid - 1) / grid;
-        const size_t tl = treeLdsFor(k_ad_refill<N>, BLOCK, S);
-        Pl.lds_tree = tl > 0;
+        const size_t tl = treeLdsFor(k_ad_refill<N>, BLOCK, S, Pl, treePrefixInLds<N>());
         // one wave per block: each owns a contiguous range of samples
         hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, s0, n,
                            per_wave, replay, acc, q, qc, st, ovf, ovf_count, wk);
     } else {
-        const size_t tl = treeLdsFor(k_ad_paths<N, AD_VOLPATH>, blk, S);
-        Pl.lds_tree = tl > 0;
+        const size_t tl = treeLdsFor(k_ad_paths<N, AD_VOLPATH>, blk, S, Pl, treePrefixInLds<N>());
         hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay,
                            acc, q, qc, st, ovf, ovf_count, wk);
     }
@@ -1544,13 +1570,11 @@ template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderP
     const unsigned blk = blockOr64(P.child_block);   // grid: blocks of `blk` threads
     RenderParams Pl = P;
     if (P.ad_kind == AD_VOLPATH) {
-        const size_t tl = treeLdsFor(k_children<N>, blk, S);
-        Pl.lds_tree = tl > 0;
+        const size_t tl = treeLdsFor(k_children<N>, blk, S, Pl, treePrefixInLds<N>());
         hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(blk), traceLds(blk) + tl, sm, S, Pl, q, qc, qc + 2, acc, acc_pe,
                            wk);
     } else {
-        const size_t tl = treeLdsFor(k_children_path<N>, blk, S);
-        Pl.lds_tree = tl > 0;
+        const size_t tl = treeLdsFor(k_children_path<N>, blk, S, Pl, treePrefixInLds<N>());
         hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(blk), traceLds(blk) + tl, sm, S, Pl, q, qc, qc + 2, acc,
                            acc_pe, wk);
     }
@@ -1560,8 +1584,7 @@ template <int N> void WidthOps<N>::launchPrimary(const DScene &S, const RenderPa
                                            hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     RenderParams Pl = P;
-    const size_t tl = treeLdsFor(k_primary<N>, BLOCK, S);
-    Pl.lds_tree = tl > 0;
+    const size_t tl = treeLdsFor(k_primary<N>, BLOCK, S, Pl, treePrefixInLds<N>());
     hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, view, s0, n, total, q, qc, st,
                        wk);
 }
@@ -1578,8 +1601,7 @@ template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderPa
                                                  unsigned long long *wk, hipStream_t sm) {
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     RenderParams Pl = P;
-    const size_t tl = treeLdsFor(k_adjoint<N>, BLOCK, S);
-    Pl.lds_tree = tl > 0;
+    const size_t tl = treeLdsFor(k_adjoint<N>, BLOCK, S, Pl, treePrefixInLds<N>());
     hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, s0, n, dimg, grad, pmed, st,
                        wk);
 }

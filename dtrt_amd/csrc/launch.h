@@ -19,7 +19,7 @@ struct RenderParams {
     int rec_stride;          // bytes per child record (ChildHdr + 12*nder, 16-byte aligned)
     unsigned qcap;           // child records the queue can hold
     int ad_block, child_block;   // work-group sizes of the AD-path and child kernels (64, 128, 256; 0 = 64)
-    int lds_tree;                // set per launch: the kernel copies the BVH into LDS (treeToLds)
+    int lds_nodes, lds_tris;     // set per launch: the part of the BVH the kernel copies into LDS (treeToLds)
 };
 
 // Counters written by the kernels (warnings the reference prints, as counts).

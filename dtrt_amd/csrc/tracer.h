@@ -5,6 +5,7 @@
 // (scene.cpp:617-793) and next-event estimation (scene.cpp:399-524).
 #pragma once
 #include "dscene.h"
+#include "launch.h"
 
 namespace dt {
 namespace DT_PNS {
@@ -29,19 +30,27 @@ struct LiPath {
 template <int N> struct Tracer {
     const DScene &S;
     uint32_t nbox = 0, ntri = 0, nseg = 0;   // work counters: box tests, triangle tests, traced segments
-    bool lds_tree = false;                   // the BVH is in LDS (treeToLds)
+    int lds_nodes = 0;                       // nodes [0, lds_nodes) are in LDS (treeToLds)
+    bool lds_tris = false;                   // all nodes and the triangle records are
     DI explicit Tracer(const DScene &s) : S(s) {}
-    // At the top of a kernel, by every thread: copy the tree into LDS if the launch made room.
-    DI void useLdsTree(int on) {
-        if (on) {
-            treeToLds(S);
-            lds_tree = true;
+    // At the top of a kernel, by every thread: copy what the launch made room for into
+    // LDS (a kernel that reads no prefix is given the whole tree or nothing).
+    DI void useLdsTree(const RenderParams &P) {
+        if constexpr (treePrefixInLds<N>()) {
+            if (P.lds_nodes > 0) {
+                treeToLds(S, P.lds_nodes, P.lds_tris != 0);
+                lds_nodes = P.lds_nodes;
+                lds_tris = P.lds_tris != 0;
+            }
+        } else if (P.lds_tris) {
+            treeToLds(S, S.nnodes, true);
+            lds_tris = true;
         }
     }
 
     DI int trace(f3 o, f3 d, real tnear, real tfar, real &t) {
         nseg++;
-        TraceHit h = traceClosest(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar, lds_tree);
+        TraceHit h = traceClosest<treePrefixInLds<N>()>(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar, lds_nodes, lds_tris);
         nbox += h.nbox;
         ntri += h.ntri;
         t = h.t;
