@@ -34,9 +34,11 @@ DI f3 xform33(const float *m, f3 p) {
 
 // Per-thread traversal stack in LDS, laid out [depth][thread of the block] so that the
 // lanes of a wave touch consecutive banks. Its size follows the work-group size: every
-// kernel that traces rays is launched with traceLds(block) bytes of dynamic LDS.
+// kernel that traces rays is launched with traceLds(block, entries) bytes of dynamic
+// LDS, entries = DTRT_BVH_STACK or, in the kernels that may keep the tree in LDS after
+// the stacks, the scene's stack_depth.
 constexpr int TRACE_BLOCK = 64;   // the default work-group size (one wave)
-constexpr size_t traceLds(unsigned block) { return (size_t)DTRT_BVH_STACK * block * sizeof(int); }
+constexpr size_t traceLds(unsigned block, int entries = DTRT_BVH_STACK) { return (size_t)entries * block * sizeof(int); }
 extern __shared__ int s_bvh_stack[];
 
 // ---------------------------------------------------------------- BVH (fp32)
@@ -85,11 +87,12 @@ template <int N> constexpr bool treePrefixInLds() { return N >= 8; }
 HDI size_t treeLdsBytes(const DScene &S, int lds_nodes, bool lds_tris) {
     return (size_t)lds_nodes * 64 + (lds_tris ? (size_t)S.ntris * 48 : 0);
 }
-DI const float4 *ldsTree() { return reinterpret_cast<const float4 *>(s_bvh_stack + DTRT_BVH_STACK * blockDim.x); }
+// The tree's LDS copy starts after stack_depth stack entries per thread.
+DI const float4 *ldsTree(int stack_depth) { return reinterpret_cast<const float4 *>(s_bvh_stack + stack_depth * blockDim.x); }
 // Copies that part into LDS; called by every thread of the block, at the top of a
 // kernel launched with treeLdsBytes more bytes of dynamic LDS.
 DI void treeToLds(const DScene &S, int lds_nodes, bool lds_tris) {
-    float4 *dst = reinterpret_cast<float4 *>(s_bvh_stack + DTRT_BVH_STACK * blockDim.x);
+    float4 *dst = reinterpret_cast<float4 *>(s_bvh_stack + S.stack_depth * blockDim.x);
     const int nn = 4 * lds_nodes, nt = lds_tris ? 3 * S.ntris : 0;
     for (int i = threadIdx.x; i < nn; i += blockDim.x) dst[i] = S.nodes[i];
     for (int i = threadIdx.x; i < nt; i += blockDim.x) dst[nn + i] = S.tris[i];
@@ -98,7 +101,7 @@ DI void treeToLds(const DScene &S, int lds_nodes, bool lds_tris) {
 
 template <bool PREFIX = false>
 DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3 o_, f3 d_, real tnear_, real tfar_,
-                         int lds_nodes = 0, bool lds_tris = false) {
+                         int lds_nodes = 0, bool lds_tris = false, int stack_depth = DTRT_BVH_STACK) {
     const fv3 o{(float)o_.x, (float)o_.y, (float)o_.z}, d{(float)d_.x, (float)d_.y, (float)d_.z};
     const float tnear = (float)tnear_, tfar = (float)tfar_;
     TraceHit res{-1, tfar, 0u, 0u};
@@ -119,7 +122,7 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
         while (node >= 0) {
             float4 a, b, c, e;
             if (PREFIX ? node < lds_nodes : lds_tris) {
-                const float4 *n = ldsTree() + 4 * node;
+                const float4 *n = ldsTree(stack_depth) + 4 * node;
                 a = n[0]; b = n[1]; c = n[2]; e = n[3];
             } else {
                 const float4 *n = nodes + 4 * (size_t)node;
@@ -149,7 +152,7 @@ DI TraceHit traceClosest(const float4 *nodes, const float4 *tris, int nnodes, f3
         for (int k = first; k < first + cnt; ++k) {
             float4 v0, e1, e2;
             if (lds_tris) {
-                const float4 *tp = ldsTree() + 4 * nnodes + 3 * k;
+                const float4 *tp = ldsTree(stack_depth) + 4 * nnodes + 3 * k;
                 v0 = tp[0]; e1 = tp[1]; e2 = tp[2];
             } else {
                 const float4 *tp = tris + 3 * (size_t)k;

@@ -740,14 +740,11 @@ struct Scene : Obj {
     int tune_child_blocks = 32;
     double tune_fill = 0.75;
     uint64_t tune_max_chunk = uint64_t(1) << 22;
-    // work-group sizes of the AD-path and child kernels (child: 0 = by precision)
-    int tune_ad_block = 64, tune_child_block = 0;
-    // The child kernel's default work-group: 256 in double, where a small scene's tree
-    // then fits in LDS beside the traversal stacks at the kernel's two waves per SIMD
-    // (treeLdsFor in kernels.hip; benchmark scene 390 -> 317 ms); 64 in float, whose
-    // four waves per SIMD leave no room for it at any size and which measured fastest
-    // at 64.
-    int childBlock() const { return tune_child_block ? tune_child_block : (S.f64 ? 256 : 64); }
+    // work-group sizes of the AD-path and child kernels. The child kernel's 256 lets a
+    // small scene's tree sit in LDS beside the traversal stacks at the kernel's full
+    // occupancy (treeLdsFor in kernels.hip): benchmark scene 390 -> 318 ms in double,
+    // 262 -> 220 ms in float, against 64.
+    int tune_ad_block = 64, tune_child_block = 256;
     DevBuf tunebuf;
 
     template <class T> const T *upload(const std::vector<T> &v) {
@@ -1010,6 +1007,8 @@ bool Scene::build() {
     if (!buildBvh(tri, fshape, bvh)) return false;
     S.nnodes = (int)(bvh.nodes.size() / 4);
     S.ntris = (int)tri.size();
+    // a lane's stack holds at most one entry per inner node on its path (depth - 1)
+    S.stack_depth = std::min(DTRT_BVH_STACK, std::max(1, bvh.depth));
     S.nodes = upload(bvh.nodes);
     S.tris = upload(bvh.tris);
     S.V = upload(V); S.Vd = upload(Vd); S.Nrm = upload(Nrm); S.Nd = upload(Nd);
@@ -1383,7 +1382,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     if (P.max_bounces < 0 || P.max_bounces > 200) return fail("render: max_bounces out of range");
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     P.ad_block = tune_ad_block;
-    P.child_block = childBlock();
+    P.child_block = tune_child_block;
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total, pool)) return false;
@@ -1574,7 +1573,7 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     P.ad_block = tune_ad_block;
-    P.child_block = childBlock();
+    P.child_block = tune_child_block;
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
@@ -1751,7 +1750,7 @@ bool Scene::captureStep(StepArgs &a) {
     P.mode = o0.mode >= 0 ? o0.mode : 0x3F; P.ddist = o0.ddist_coeff;
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     P.ad_block = tune_ad_block;
-    P.child_block = childBlock();
+    P.child_block = tune_child_block;
     std::vector<DView> hvf, hva;
     std::vector<uint64_t> pe_total, pe_unused;
     if (!setupViews(nv, cp.data(), a.fopts.data(), hvf, pe_total)) return false;
@@ -2078,7 +2077,6 @@ bool Scene::autotune(const dtrt_render_options &o, int ad_kind, double *best_ms,
         if (!tryKnob(queue_budget, q)) return false;
     for (int g : groups)
         if (!tryKnob(tune_ad_block, g)) return false;
-    tune_child_block = childBlock();   // the default made explicit, so it is not tried twice
     for (int g : groups)
         if (!tryKnob(tune_child_block, g)) return false;
     for (int b : blocks) {
@@ -2680,7 +2678,7 @@ int dtrt_scene_launch_settings(dtrt_object *scene, int64_t *out6) {
     out6[2] = (int64_t)s->tune_max_chunk;
     out6[3] = (int64_t)s->queue_budget;
     out6[4] = s->tune_ad_block;
-    out6[5] = s->childBlock();
+    out6[5] = s->tune_child_block;
     return 0;
 }
 int dtrt_scene_set_launch_settings(dtrt_object *scene, const int64_t *in6) {

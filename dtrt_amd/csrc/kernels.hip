@@ -1491,7 +1491,7 @@ template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
         hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, kern, (int)blk, traceLds(blk)) != hipSuccess || base <= 0) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, kern, (int)blk, traceLds(blk, S.stack_depth)) != hipSuccess || base <= 0) {
         (void)hipGetLastError();
         return 0;
     }
@@ -1500,7 +1500,7 @@ template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &
         (void)hipGetLastError();
         return 0;
     }
-    const size_t used = traceLds(blk) + fa.sharedSizeBytes;
+    const size_t used = traceLds(blk, S.stack_depth) + fa.sharedSizeBytes;
     const size_t room = std::min<size_t>((size_t)per_cu / (size_t)base, (size_t)per_block);
     if (room <= used) return 0;
     const size_t spare = room - used;
@@ -1513,7 +1513,7 @@ template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &
     }
     const size_t tb = treeLdsBytes(S, Pl.lds_nodes, Pl.lds_tris != 0);
     if (tb == 0 ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kern, (int)blk, traceLds(blk) + tb) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kern, (int)blk, traceLds(blk, S.stack_depth) + tb) != hipSuccess ||
         with < base) {
         (void)hipGetLastError();
         Pl.lds_nodes = 0;
@@ -1541,11 +1541,11 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
     RenderParams Pl = P;
     if (P.ad_kind == AD_PATH) {
         const size_t tl = treeLdsFor(k_ad_paths<N, AD_PATH>, blk, S, Pl, treePrefixInLds<N>());
-        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay, acc,
+        hipLaunchKernelGGL((k_ad_paths<N, AD_PATH>), dim3(nb), dim3(blk), traceLds(blk, S.stack_depth) + tl, sm, S, Pl, s0, n, replay, acc,
                            q, qc, st, ovf, ovf_count, wk);
     } else if (P.ad_kind == AD_DIRECT) {
         const size_t tl = treeLdsFor(k_ad_paths<N, AD_DIRECT>, blk, S, Pl, treePrefixInLds<N>());
-        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay,
+        hipLaunchKernelGGL((k_ad_paths<N, AD_DIRECT>), dim3(nb), dim3(blk), traceLds(blk, S.stack_depth) + tl, sm, S, Pl, s0, n, replay,
                            acc, q, qc, st, ovf, ovf_count, wk);
     } else if constexpr (adRefill<N>()) {
         // one wave per block, AD_REFILL_WAVES_PER_CU of them per CU share the samples
@@ -1557,11 +1557,11 @@ template <int N> void WidthOps<N>::launchAd(const DScene &S, const RenderParams 
         const uint64_t per_wave = (n + grid - 1) / grid;
         const size_t tl = treeLdsFor(k_ad_refill<N>, BLOCK, S, Pl, treePrefixInLds<N>());
         // one wave per block: each owns a contiguous range of samples
-        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, s0, n,
+        hipLaunchKernelGGL(k_ad_refill<N>, dim3((unsigned)grid), dim3(BLOCK), traceLds(BLOCK, S.stack_depth) + tl, sm, S, Pl, s0, n,
                            per_wave, replay, acc, q, qc, st, ovf, ovf_count, wk);
     } else {
         const size_t tl = treeLdsFor(k_ad_paths<N, AD_VOLPATH>, blk, S, Pl, treePrefixInLds<N>());
-        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(blk), traceLds(blk) + tl, sm, S, Pl, s0, n, replay,
+        hipLaunchKernelGGL((k_ad_paths<N, AD_VOLPATH>), dim3(nb), dim3(blk), traceLds(blk, S.stack_depth) + tl, sm, S, Pl, s0, n, replay,
                            acc, q, qc, st, ovf, ovf_count, wk);
     }
 }
@@ -1571,11 +1571,11 @@ template <int N> void WidthOps<N>::launchChildren(const DScene &S, const RenderP
     RenderParams Pl = P;
     if (P.ad_kind == AD_VOLPATH) {
         const size_t tl = treeLdsFor(k_children<N>, blk, S, Pl, treePrefixInLds<N>());
-        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(blk), traceLds(blk) + tl, sm, S, Pl, q, qc, qc + 2, acc, acc_pe,
+        hipLaunchKernelGGL(k_children<N>, dim3(grid), dim3(blk), traceLds(blk, S.stack_depth) + tl, sm, S, Pl, q, qc, qc + 2, acc, acc_pe,
                            wk);
     } else {
         const size_t tl = treeLdsFor(k_children_path<N>, blk, S, Pl, treePrefixInLds<N>());
-        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(blk), traceLds(blk) + tl, sm, S, Pl, q, qc, qc + 2, acc,
+        hipLaunchKernelGGL(k_children_path<N>, dim3(grid), dim3(blk), traceLds(blk, S.stack_depth) + tl, sm, S, Pl, q, qc, qc + 2, acc,
                            acc_pe, wk);
     }
 }
@@ -1585,7 +1585,7 @@ template <int N> void WidthOps<N>::launchPrimary(const DScene &S, const RenderPa
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     RenderParams Pl = P;
     const size_t tl = treeLdsFor(k_primary<N>, BLOCK, S, Pl, treePrefixInLds<N>());
-    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, view, s0, n, total, q, qc, st,
+    hipLaunchKernelGGL(k_primary<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK, S.stack_depth) + tl, sm, S, Pl, view, s0, n, total, q, qc, st,
                        wk);
 }
 template <int N> void WidthOps<N>::launchFinalize(const RenderParams &P, int npix, int pix0, int spp, int has_pe, const real *acc,
@@ -1602,7 +1602,7 @@ template <int N> void WidthOps<N>::launchAdjoint(const DScene &S, const RenderPa
     unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
     RenderParams Pl = P;
     const size_t tl = treeLdsFor(k_adjoint<N>, BLOCK, S, Pl, treePrefixInLds<N>());
-    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK) + tl, sm, S, Pl, s0, n, dimg, grad, pmed, st,
+    hipLaunchKernelGGL(k_adjoint<N>, dim3(nb), dim3(BLOCK), traceLds(BLOCK, S.stack_depth) + tl, sm, S, Pl, s0, n, dimg, grad, pmed, st,
                        wk);
 }
 template <int N> hipError_t WidthOps<N>::kernelInfo(KernelInfo *out, int cap, int *count) {

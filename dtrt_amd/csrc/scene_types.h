@@ -82,6 +82,7 @@ struct DScene {
     int nder, nshapes, nlights, nmedia;
     int f64;                    // estimator precision of the scene's kernels: 1 double, 0 float
     int NV, NF, ntris, nnodes;
+    int stack_depth;            // traversal stack entries a lane can need: the tree's depth (<= DTRT_BVH_STACK)
     const float4 *nodes;        // 4 float4 per node
     const float4 *tris;         // 3 float4 per triangle, leaf order
     const float *V;             // [NV][3]

@@ -50,7 +50,8 @@ template <int N> struct Tracer {
 
     DI int trace(f3 o, f3 d, real tnear, real tfar, real &t) {
         nseg++;
-        TraceHit h = traceClosest<treePrefixInLds<N>()>(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar, lds_nodes, lds_tris);
+        TraceHit h = traceClosest<treePrefixInLds<N>()>(S.nodes, S.tris, S.nnodes, o, d, tnear, tfar, lds_nodes, lds_tris,
+                                                        S.stack_depth);
         nbox += h.nbox;
         ntri += h.ntri;
         t = h.t;
