@@ -735,6 +735,7 @@ struct Scene : Obj {
     uint64_t work_counts[12] = {0};
     int edge_ebase_total = 0;
     size_t queue_budget = size_t(6) << 30;   // bytes of queued child records
+    bool tree_in_hbm = false;                 // test setting: traversal reads the BVH from HBM only
     // launch settings chosen by autotune(): persistent child blocks per CU, the share of
     // the queue an AD chunk aims to fill, and the largest AD chunk
     int tune_child_blocks = 32;
@@ -1383,6 +1384,7 @@ bool Scene::renderViews(int nv, const Camera *const *cams, const dtrt_render_opt
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     P.ad_block = tune_ad_block;
     P.child_block = tune_child_block;
+    P.tree_in_hbm = tree_in_hbm ? 1 : 0;
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total, pool)) return false;
@@ -1574,6 +1576,7 @@ bool Scene::captureViews(int nv, const Camera *const *cams, const dtrt_render_op
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     P.ad_block = tune_ad_block;
     P.child_block = tune_child_block;
+    P.tree_in_hbm = tree_in_hbm ? 1 : 0;
     std::vector<DView> hv;
     std::vector<uint64_t> pe_total;
     if (!setupViews(nv, cams, opts, hv, pe_total)) return false;
@@ -1751,6 +1754,7 @@ bool Scene::captureStep(StepArgs &a) {
     P.rec_stride = childRecordStride(nder, S.f64 != 0);
     P.ad_block = tune_ad_block;
     P.child_block = tune_child_block;
+    P.tree_in_hbm = tree_in_hbm ? 1 : 0;
     std::vector<DView> hvf, hva;
     std::vector<uint64_t> pe_total, pe_unused;
     if (!setupViews(nv, cp.data(), a.fopts.data(), hvf, pe_total)) return false;
@@ -1942,6 +1946,7 @@ bool Scene::renderAdjointViews(int nv, const Camera *const *cams, const dtrt_ren
     RenderParams P{};
     P.max_bounces = o.max_bounces; P.sppse = 0;
     P.mode = (o.mode >= 0 ? o.mode : 0x3F) & (1 | 2 | 8);
+    P.tree_in_hbm = tree_in_hbm ? 1 : 0;
     P.ddist = 0.f;
     std::vector<dtrt_render_options> ov(opts, opts + nv);
     for (auto &x : ov) {
@@ -2770,6 +2775,15 @@ int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes) {
         Scene *s = as<Scene>(scene, T_SCENE);
         if (!s) return -1;
         s->queue_budget = bytes ? (size_t)bytes : (size_t(6) << 30);
+        return 0;
+    });
+}
+
+int dtrt_test_set_tree_in_hbm(dtrt_object *scene, int on) {
+    return guarded<int>(-1, [&]() -> int {
+        Scene *s = as<Scene>(scene, T_SCENE);
+        if (!s) return -1;
+        s->tree_in_hbm = on != 0;
         return 0;
     });
 }

@@ -1486,7 +1486,7 @@ static hipError_t addKernelInfo(F fn, const char *name, int block, size_t lds, K
 template <class K> static size_t treeLdsFor(K kern, unsigned blk, const DScene &S, RenderParams &Pl, bool prefix) {
     Pl.lds_nodes = 0;
     Pl.lds_tris = 0;
-    if (S.nnodes <= 0) return 0;
+    if (S.nnodes <= 0 || Pl.tree_in_hbm) return 0;
     int dev = 0, per_cu = 0, per_block = 0, base = 0, with = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||

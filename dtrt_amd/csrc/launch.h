@@ -20,6 +20,7 @@ struct RenderParams {
     unsigned qcap;           // child records the queue can hold
     int ad_block, child_block;   // work-group sizes of the AD-path and child kernels (64, 128, 256; 0 = 64)
     int lds_nodes, lds_tris;     // set per launch: the part of the BVH the kernel copies into LDS (treeToLds)
+    int tree_in_hbm;             // nonzero: no part of the BVH goes into LDS (a test setting)
 };
 
 // Counters written by the kernels (warnings the reference prints, as counts).

@@ -87,6 +87,7 @@ _P = C.c_void_p
 _lib.dtrt_test_kernel_times.argtypes = [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
 _lib.dtrt_test_scene_info.argtypes = [_P, C.POINTER(C.c_int64)]
 _lib.dtrt_test_set_queue_budget.argtypes = [_P, C.c_uint64]
+_lib.dtrt_test_set_tree_in_hbm.argtypes = [_P, C.c_int]
 _lib.dtrt_test_set_voxel_lanes.argtypes = [_P, C.c_int, C.POINTER(C.c_int), C.c_int]
 _lib.dtrt_scene_set_density.argtypes = [_P, C.c_int, _P, C.c_int]
 _lib.dtrt_scene_capture_views.argtypes = [_P, C.c_int, C.POINTER(_P), C.POINTER(_m.fns["render"].argtypes[2]._type_),
@@ -328,6 +329,11 @@ def set_voxel_lanes(scene, medium, voxels):
 def set_queue_budget(scene, nbytes):
     """Child-record queue bytes for this scene's renders (0 restores the default)."""
     _chk(_lib.dtrt_test_set_queue_budget(scene._h, int(nbytes)), "set_queue_budget")
+
+
+def set_tree_in_hbm(scene, on=True):
+    """Traversal reads the BVH from HBM only (True) or from LDS where a launch has room."""
+    _chk(_lib.dtrt_test_set_tree_in_hbm(scene._h, 1 if on else 0), "set_tree_in_hbm")
 
 
 def render_stats(scene):

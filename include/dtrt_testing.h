@@ -31,6 +31,10 @@ int dtrt_test_set_voxel_lanes(dtrt_object *scene, int medium, const int *voxels,
 /* Bytes of child-record queue the scene's renders may use (0 = default 6 GiB).
  * A small budget forces the exact overflow-replay path. */
 int dtrt_test_set_queue_budget(dtrt_object *scene, uint64_t bytes);
+/* on != 0: the scene's kernels read the BVH from HBM only; 0 (the default): a launch
+ * keeps the tree, or its top levels, in LDS where that costs no occupancy. Images are
+ * the same either way up to summation order. */
+int dtrt_test_set_tree_in_hbm(dtrt_object *scene, int on);
 /* [0] BVH nodes, [1] triangles, [2] edges, [3] primary edges, [4] primary-edge
  * length sum (double bits), [5] secondary edges usable, [6] vertices, [7] faces. */
 int dtrt_test_scene_info(dtrt_object *scene, int64_t *out8);
