@@ -43,6 +43,9 @@ __constant__ FrameRot DT_FRAME_ROT;
 
 constexpr int BLOCK = TRACE_BLOCK;
 constexpr int MAX_BLOCK = 256;   // largest work-group size of the AD-path and child kernels
+// Work-group size of the primal path and particle kernels: four waves, so that at their
+// occupancy a small scene's tree fits in LDS beside the traversal stacks (treeLdsFor).
+constexpr int PRIMAL_BLOCK = 256;
 HDI unsigned blockOr64(int b) { return b == 128 || b == 256 ? (unsigned)b : 64u; }
 // Minimum resident waves per SIMD requested from the register allocator (HIP's second
 // __launch_bounds__ argument): bounds the VGPRs of the two path kernels. Per
@@ -730,17 +733,18 @@ __global__ __launch_bounds__(BLOCK, adWaves<N>()) void k_ad_refill(DScene S, Ren
 //                   51-85, max depth 1): pathStep, camera ray from tnear ShadowEpsilon.
 // Radiance sums go to acc[pixel][3].
 template <int KIND>
-__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DScene S, RenderParams P, uint64_t s0,
-                                                                              uint64_t nsamp, real *acc,
-                                                                              unsigned long long *work) {
+__global__ __launch_bounds__(PRIMAL_BLOCK, CHILD_WAVES_PER_SIMD) void k_primal_paths(DScene S, RenderParams P, uint64_t s0,
+                                                                                     uint64_t nsamp, real *acc,
+                                                                                     unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
-    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    const uint64_t li = (uint64_t)blk * PRIMAL_BLOCK + threadIdx.x;
     const bool active = li < nsamp;
     const DView &vw = S.views[0];
     const DCamera &cam = vw.cam;
     const uint64_t ls = s0 + li;
     const int pixel = (int)(ls / (uint64_t)vw.spp);
     Tracer<1> T(S);
+    T.useLdsTree(P);
     Rng r;
     r.init(vw.seed, ls);
     f3 L{0, 0, 0};
@@ -914,12 +918,13 @@ DI void traceParticle(Tracer<1> &T, const DCamera &cam, int max_bounces, Rng &r,
 #ifndef DT_N
 // ParticleTracer::render (ptracer.cpp:10-55): particles s0 .. s0+n-1, each its own
 // stream keyed (seed, particle index), splatted into acc[pixel][3] of view 0's window.
-__global__ __launch_bounds__(BLOCK, CHILD_WAVES_PER_SIMD) void k_particles(DScene S, RenderParams P, uint64_t s0,
-                                                                           uint64_t n, real *acc,
-                                                                           unsigned long long *work) {
+__global__ __launch_bounds__(PRIMAL_BLOCK, CHILD_WAVES_PER_SIMD) void k_particles(DScene S, RenderParams P, uint64_t s0,
+                                                                                  uint64_t n, real *acc,
+                                                                                  unsigned long long *work) {
     const uint32_t blk = xcdRemap(blockIdx.x, gridDim.x);
-    const uint64_t li = (uint64_t)blk * BLOCK + threadIdx.x;
+    const uint64_t li = (uint64_t)blk * PRIMAL_BLOCK + threadIdx.x;
     Tracer<1> T(S);
+    T.useLdsTree(P);
     Rng r;
     r.init(S.views[0].seed, s0 + li);
     if (li < n) traceParticle(T, S.views[0].cam, P.max_bounces, r, acc);
@@ -1723,28 +1728,36 @@ hipError_t launch_trace(const DScene &S, const float *rays, int n, int *face, fl
 }
 hipError_t launch_primal_paths(const DScene &S, const RenderParams &P, int kind, uint64_t s0, uint64_t n, void *acc,
                               unsigned long long *work, hipStream_t sm) {
-    const uint64_t nb = (n + BLOCK - 1) / BLOCK;
+    const uint64_t nb = (n + PRIMAL_BLOCK - 1) / PRIMAL_BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
     real *A = static_cast<real *>(acc);
-    if (kind == PRIMAL_SIMPLE)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n,
+    RenderParams Pl = P;
+    const size_t lds = traceLds(PRIMAL_BLOCK, S.stack_depth);
+    if (kind == PRIMAL_SIMPLE) {
+        const size_t tl = treeLdsFor(k_primal_paths<PRIMAL_SIMPLE>, PRIMAL_BLOCK, S, Pl, false);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_SIMPLE>, dim3((unsigned)nb), dim3(PRIMAL_BLOCK), lds + tl, sm, S, Pl, s0,
+                           n, A, work);
+    } else if (kind == PRIMAL_PATH) {
+        const size_t tl = treeLdsFor(k_primal_paths<PRIMAL_PATH>, PRIMAL_BLOCK, S, Pl, false);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(PRIMAL_BLOCK), lds + tl, sm, S, Pl, s0, n,
                            A, work);
-    else if (kind == PRIMAL_PATH)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_PATH>, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n, A,
-                           work);
-    else if (kind == PRIMAL_VOLPATH)
-        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n,
-                           A, work);
-    else
+    } else if (kind == PRIMAL_VOLPATH) {
+        const size_t tl = treeLdsFor(k_primal_paths<PRIMAL_VOLPATH>, PRIMAL_BLOCK, S, Pl, false);
+        hipLaunchKernelGGL(k_primal_paths<PRIMAL_VOLPATH>, dim3((unsigned)nb), dim3(PRIMAL_BLOCK), lds + tl, sm, S, Pl, s0,
+                           n, A, work);
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 hipError_t launch_particles(const DScene &S, const RenderParams &P, uint64_t s0, uint64_t n, void *acc,
                             unsigned long long *work, hipStream_t sm) {
-    const uint64_t nb = (n + BLOCK - 1) / BLOCK;
+    const uint64_t nb = (n + PRIMAL_BLOCK - 1) / PRIMAL_BLOCK;
     if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(BLOCK), traceLds(BLOCK), sm, S, P, s0, n,
-                       static_cast<real *>(acc), work);
+    RenderParams Pl = P;
+    const size_t tl = treeLdsFor(k_particles, PRIMAL_BLOCK, S, Pl, false);
+    hipLaunchKernelGGL(k_particles, dim3((unsigned)nb), dim3(PRIMAL_BLOCK), traceLds(PRIMAL_BLOCK, S.stack_depth) + tl, sm, S,
+                       Pl, s0, n, static_cast<real *>(acc), work);
     return hipGetLastError();
 }
 hipError_t launch_scale(const DScene &S, const void *acc, float *out, uint64_t n, double scale, hipStream_t sm) {
@@ -1777,10 +1790,10 @@ hipError_t kernel_info(const DScene &S, KernelInfo *out, int cap, int *count) {
 #undef DT_INFO_N
         default: return hipErrorInvalidValue;
     }
-    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_VOLPATH>, "k_primal_paths/volpath", BLOCK, traceLds(BLOCK), out, cap, count);
-    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_SIMPLE>, "k_primal_paths/simple", BLOCK, traceLds(BLOCK), out, cap, count);
-    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_PATH>, "k_primal_paths/path", BLOCK, traceLds(BLOCK), out, cap, count);
-    if (e == hipSuccess) e = addKernelInfo(k_particles, "k_particles", BLOCK, traceLds(BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_VOLPATH>, "k_primal_paths/volpath", PRIMAL_BLOCK, traceLds(PRIMAL_BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_SIMPLE>, "k_primal_paths/simple", PRIMAL_BLOCK, traceLds(PRIMAL_BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_primal_paths<PRIMAL_PATH>, "k_primal_paths/path", PRIMAL_BLOCK, traceLds(PRIMAL_BLOCK), out, cap, count);
+    if (e == hipSuccess) e = addKernelInfo(k_particles, "k_particles", PRIMAL_BLOCK, traceLds(PRIMAL_BLOCK), out, cap, count);
     if (e == hipSuccess) e = addKernelInfo(k_refit, "k_refit", 256, 0, out, cap, count);
     return e;
 }
