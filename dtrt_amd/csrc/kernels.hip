@@ -54,13 +54,13 @@ HDI unsigned blockOr64(int b) { return b == 128 || b == 256 ? (unsigned)b : 64u;
 #define CHILD_WAVES_PER_SIMD (kF64 ? CHILD_WAVES_F64 : 4)
 #endif
 #ifndef CHILD_WAVES_F64
-#define CHILD_WAVES_F64 2   // config[1] at nder 3, child kernel per render: 2 waves 413 ms, 4 waves 464 ms (1 wave: 1.5x 2)
+#define CHILD_WAVES_F64 2   // heterogeneous-medium scene, 512^2, 128 spp, nder 3, child kernel per render: 2 waves 413 ms, 4 waves 464 ms (1 wave: 1.5x 2)
 #endif
 #ifndef AD_WAVES_PER_SIMD
 #define AD_WAVES_PER_SIMD (kF64 ? AD_WAVES_F64 : 2)
 #endif
 #ifndef AD_WAVES_F64
-#define AD_WAVES_F64 1      // 1 wave: 394 ms, 2 waves: 514 ms per render's AD kernel (config[1], nder 3)
+#define AD_WAVES_F64 1      // 1 wave: 394 ms, 2 waves: 514 ms per render's AD kernel (same scene and size)
 #endif
 // Dual widths from which the VolPathTracerAD samples run in k_ad_refill (per-bounce
 // lane refill) at one wave per SIMD, with the whole 512-entry register file, rather
