@@ -61,7 +61,7 @@ def cpu_baseline(desc, target_s=15.0):
     o.set_threads(threads)
 
     def run(side):
-        d = scenes.config1(nder=desc.nder, res=desc.width, spp=desc.spp,
+        d = scenes.cube_medium(nder=desc.nder, res=desc.width, spp=desc.spp,
                            crop=((desc.width - side) // 2, (desc.height - side) // 2, side, side))
         sc = scenes.instantiate(o, d)
         t = time.perf_counter()
@@ -84,7 +84,7 @@ def gpu_crop_rate(desc, side, reps=5):
     import torch
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
-    d = scenes.config1(nder=desc.nder, res=desc.width, spp=desc.spp,
+    d = scenes.cube_medium(nder=desc.nder, res=desc.width, spp=desc.spp,
                        crop=((desc.width - side) // 2, (desc.height - side) // 2, side, side))
     sc = scenes.instantiate(G, d)
     out = torch.zeros(d.out_shape(), dtype=torch.float32, device="cuda")
@@ -143,7 +143,7 @@ def adjoint_leg(spp=64, res=512, grid_res=128):
     interior = 1 | 2 | 8
 
     def desc(nder, seed):
-        d = scenes.config1(nder=nder, res=res, spp=spp, grid_res=grid_res, seed=seed)
+        d = scenes.cube_medium(nder=nder, res=res, spp=spp, grid_res=grid_res, seed=seed)
         for sh in d.shapes:
             if sh.velocities is not None:
                 sh.velocities = np.zeros_like(sh.velocities)
@@ -201,7 +201,7 @@ def views_leg(nviews=32):
     import torch
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
-    d = scenes.config1(nder=3, res=512, spp=64)
+    d = scenes.cube_medium(nder=3, res=512, spp=64)
     sc = scenes.instantiate(G, d)
     views = scenes.orbit_views(nviews)
     cams = [G.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
@@ -242,7 +242,7 @@ def views_leg(nviews=32):
     t_graph = (time.perf_counter() - t) / reps
     # several small scenes (own density grids) in one call, each on its own stream
     nsc = 8
-    mds = [scenes.config1(nder=3, res=128, spp=16, seed=20 + i) for i in range(nsc)]
+    mds = [scenes.cube_medium(nder=3, res=128, spp=16, seed=20 + i) for i in range(nsc)]
     mscs = [scenes.instantiate(G, m) for m in mds]
     mopts = [[G.RenderOptions(400 + i, m.spp, m.max_bounces, m.sppe, m.sppse, True, -1, m.ddist)] for i, m in enumerate(mds)]
     mouts = [[torch.zeros(m.out_shape(), device="cuda")] for m in mds]
@@ -278,7 +278,7 @@ def primal_leg(res=512, spp=128, reps=2):
     import torch
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
-    d = scenes.config1(nder=3, res=res, spp=spp)
+    d = scenes.cube_medium(nder=3, res=res, spp=spp)
     sc = scenes.instantiate(G, d)
     out = torch.zeros((d.height, d.width, 3), device="cuda")
     rec = {"workload": "primal render of the heterogeneous-medium scene, %d^2, %d spp, depth %d" % (res, spp, d.max_bounces)}
@@ -348,8 +348,8 @@ def configs_leg():
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
     rec = {}
-    for name, d in (("surface_icosphere_256sq_64spp_nder1", scenes.config0()),
-                    ("dielectric_sphere_in_medium_512sq_256spp_nder8", scenes.config2())):
+    for name, d in (("surface_icosphere_256sq_64spp_nder1", scenes.ico_surface()),
+                    ("dielectric_sphere_in_medium_512sq_256spp_nder8", scenes.dielectric_medium())):
         sc = scenes.instantiate(G, d)
         out = torch.zeros(d.out_shape(), device="cuda")
         opts = G.RenderOptions(1, d.spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
@@ -383,7 +383,7 @@ def inverse_leg(spp_divisor=16):
     import dtrt_amd.dtrt as G
     from dtrt_amd import scenes
     t0 = time.perf_counter()
-    d = scenes.config4()
+    d = scenes.displaced_views()
     views = scenes.orbit_views(32, seed=5, spp_choices=(16, 32, 64, 128, 256), radius=3.2)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -477,7 +477,7 @@ def main():
     from dtrt_amd import scenes
 
     G.set_precision(64)   # the reference's Float (include/fwd.h:10-13); every leg below runs at it
-    desc = scenes.config1(nder=3, res=args.res, spp=args.spp)
+    desc = scenes.cube_medium(nder=3, res=args.res, spp=args.spp)
     scene = scenes.instantiate(G, desc)
     out = torch.zeros(desc.out_shape(), dtype=torch.float32, device="cuda")
     opts = G.RenderOptions(0, desc.spp, desc.max_bounces, desc.sppe, desc.sppse, True, -1, desc.ddist)

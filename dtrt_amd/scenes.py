@@ -392,8 +392,8 @@ def render_primal(mod, d: SceneDesc, seed=0, spp=None, max_bounces=None, scene=N
     return out
 
 
-# ------------------------------------------------------------- configs
-def config0(nder=1, res=256, spp=64, crop=None):
+# ------------------------------------------------------------- benchmark scenes
+def ico_surface(nder=1, res=256, spp=64, crop=None):
     """Ground quad + icosphere (subdiv 3, r 0.5) + area-light quad; no media; sphere x-translation."""
     g_v, g_f = quad((0, 0, 0), 4.0, 1, +1)
     s_v, s_f = icosphere(3, 0.5, (0, 0.5, 0))
@@ -407,10 +407,10 @@ def config0(nder=1, res=256, spp=64, crop=None):
                      shapes=shapes, bsdfs=[("diffuse", (0.5, 0.5, 0.5)), ("diffuse", (0.8, 0.8, 0.8)),
                                            ("diffuse", (0.0, 0.0, 0.0))],
                      lights=[("area", 2, (10.0, 10.0, 10.0), False)], spp=spp, max_bounces=5, sppe=4096, sppse=10,
-                     crop=crop, name="cfg0")
+                     crop=crop, name="ico_surface")
 
 
-def config1(nder=3, res=512, spp=128, grid_res=64, crop=None, seed=7):
+def cube_medium(nder=3, res=512, spp=128, grid_res=64, crop=None, seed=7):
     """Null cube holding a 64^3 fBm heterogeneous medium, HG 0.3; quad light + enclosing emissive sphere."""
     c_v, c_f = cube()
     l_v, l_f = quad((0, 1.5, 0), 1.0, 1, -1)
@@ -431,7 +431,7 @@ def config1(nder=3, res=512, spp=128, grid_res=64, crop=None, seed=7):
                      shapes=shapes, bsdfs=[("null",), ("diffuse", (0.0, 0.0, 0.0))],
                      lights=[("area", 1, (20.0, 20.0, 20.0), False), ("area", 2, (0.5, 0.5, 0.5), True)],
                      phases=[("hg", 0.3)], media=[med], edge_weights=np.array([1.0, 1.0, 0.0], np.float32),
-                     spp=spp, max_bounces=10, sppe=4096, sppse=10, ddist=0.0, crop=crop, name="cfg1")
+                     spp=spp, max_bounces=10, sppe=4096, sppse=10, ddist=0.0, crop=crop, name="cube_medium")
 
 
 def orbit_views(nviews=32, seed=5, res_choices=tuple(range(128, 769, 64)), spp_choices=(16, 32, 64), radius=3.0):
@@ -450,7 +450,7 @@ def orbit_views(nviews=32, seed=5, res_choices=tuple(range(128, 769, 64)), spp_c
     return views
 
 
-def config2(nder=8, res=512, spp=256, freq=71, grid_res=128, crop=None):
+def dielectric_medium(nder=8, res=512, spp=256, freq=71, grid_res=128, crop=None):
     """Displaced geodesic sphere (rough dielectric, IOR 1.5) inside a 128^3 fBm medium; 8 deformation lanes."""
     s_v, s_f = geodesic_sphere(freq, 1.0)
     dirs = s_v.astype(np.float64)
@@ -476,16 +476,16 @@ def config2(nder=8, res=512, spp=256, freq=71, grid_res=128, crop=None):
                                            ("roughdielectric", 0.01, 1.5, 1.0)],
                      lights=[("area", 2, (15.0, 15.0, 15.0), False), ("area", 3, (8.0, 8.0, 8.0), False)],
                      phases=[("hg", 0.3)], media=[med], spp=spp, max_bounces=12, sppe=4096, sppse=10, crop=crop,
-                     name="cfg2")
+                     name="dielectric_medium")
 
 
-def config4(nder=8, freq=224, grid_res=256):
-    """The batched inverse-rendering scene: config2's construction at scale, a displaced
+def displaced_views(nder=8, freq=224, grid_res=256):
+    """The batched inverse-rendering scene: dielectric_medium's construction at scale, a displaced
     geodesic sphere of frequency 224 (1,003,520 faces, rough dielectric) inside a 256^3
     fBm medium (64 MiB), 8 deformation lanes. Views come from orbit_views(32, seed 5,
     spp 16..256, radius 3.2)."""
-    d = config2(nder=nder, res=512, spp=64, freq=freq, grid_res=grid_res)
-    d.name = "cfg4"
+    d = dielectric_medium(nder=nder, res=512, spp=64, freq=freq, grid_res=grid_res)
+    d.name = "displaced_views"
     return d
 
 

@@ -108,17 +108,17 @@ def absorbing_slab(heterogeneous=False, sigma=1.3, res=16, spp=4, Le=2.0):
     return d, L, -(sigma if heterogeneous else 1.0) * length * L
 
 
-def cfg0_reflectance_lane(res=24, spp=8, rho=0.8):
-    """cfg0 with its single lane on the sphere's diffuse reflectance (nothing moves)."""
-    d = scenes.config0(nder=1, res=res, spp=spp)
+def ico_surface_reflectance_lane(res=24, spp=8, rho=0.8):
+    """ico_surface with its single lane on the sphere's diffuse reflectance (nothing moves)."""
+    d = scenes.ico_surface(nder=1, res=res, spp=spp)
     d.shapes[1].velocities = None
     d.bsdfs[1] = ("diffuse", (rho, rho, rho), np.ones((1, 3), np.float32))
     return d
 
 
-def cfg1_albedo_lane(res=16, spp=8, albedo=0.8, grid_res=16, crop=None):
-    """cfg1 with its single lane on the medium's albedo (nothing moves)."""
-    d = scenes.config1(nder=1, res=res, spp=spp, grid_res=grid_res, crop=crop)
+def cube_medium_albedo_lane(res=16, spp=8, albedo=0.8, grid_res=16, crop=None):
+    """cube_medium with its single lane on the medium's albedo (nothing moves)."""
+    d = scenes.cube_medium(nder=1, res=res, spp=spp, grid_res=grid_res, crop=crop)
     d.shapes[0].velocities = None
     m = d.media[0]
     m.trans_vel = None
@@ -243,8 +243,8 @@ def feature_sweep(nder=4, res=40, spp=16, crop=(8, 8, 24, 24)):
 def rotating_grid(nder=2, res=48, spp=16, grid_res=16, crop=(12, 12, 24, 24)):
     """The heterogeneous-medium scene with the grid (and its null cube) rotating about y
     in lane 0 and the density scale as lane 1."""
-    from dtrt_amd.scenes import config1
-    d = config1(nder=nder, res=res, spp=spp, grid_res=grid_res, crop=crop)
+    from dtrt_amd.scenes import cube_medium
+    d = cube_medium(nder=nder, res=res, spp=spp, grid_res=grid_res, crop=crop)
     c_v = d.shapes[0].vertices
     vel = np.zeros((nder, len(c_v), 3), np.float32)
     vel[0] = np.cross(np.array([0.0, 1.0, 0.0]), c_v)   # omega x p, centre at the origin

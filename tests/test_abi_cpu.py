@@ -55,7 +55,7 @@ def test_host_edges_match_oracle(oracle, which):
     G.set_nder(1)
     s = G.Shape(v, f, None, None, len(v), len(f), -1, 0, -1, -1, None)
     e, length = G.shape_edges(s)
-    d = scenes.config0(nder=1, res=8, spp=1)
+    d = scenes.ico_surface(nder=1, res=8, spp=1)
     d.shapes[1] = scenes.ShapeDesc(v, f, bsdf_id=1)
     sc = scenes.instantiate(oracle, d)
     n, total = oracle.scene_query(sc, "num_edges", 1)
@@ -102,7 +102,7 @@ def test_bvh_builder_splits_the_enclosing_shape_off_first():
     (320). The sphere's box holds the others with far more area, so the root's first
     child holds the 14 contents and the second the sphere; without shape ids SAH mixes
     them."""
-    d = scenes.config1(nder=1, res=8, spp=1, grid_res=4)
+    d = scenes.cube_medium(nder=1, res=8, spp=1, grid_res=4)
     tris = _tris(d)
     shapes = np.concatenate([np.full(len(s.indices), i) for i, s in enumerate(d.shapes)])
     b = _bvh(tris, shapes)

@@ -18,7 +18,7 @@ INTERIOR = 1 | 2 | 8   # emission, surface interior, volume interior (integrator
 
 def _density_scene(nder, res=64, crop=(16, 16, 32, 32), spp=8, grid_res=24):
     """The heterogeneous-medium scene with every motion tangent removed."""
-    d = scenes.config1(nder=nder, res=res, spp=spp, grid_res=grid_res, crop=crop)
+    d = scenes.cube_medium(nder=nder, res=res, spp=spp, grid_res=grid_res, crop=crop)
     for s in d.shapes:
         if s.velocities is not None:
             s.velocities = np.zeros_like(s.velocities)

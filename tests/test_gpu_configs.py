@@ -16,23 +16,23 @@ EMISSION, SURF_MAIN, SURF_BOUNDARY, VOL_MAIN, VOL_BOUNDARY1, VOL_BOUNDARY2 = 1, 
 
 
 @pytest.fixture(scope="module")
-def cfg2():
+def dielectric_medium():
     """The dielectric configuration at full size: 100,820-face displaced sphere with
     shading normals, 128^3 fBm grid, depth 12, nder 8; a 24^2 crop across the sphere's
     right silhouette (the sphere spans x ~ 145..367 at y = 256)."""
-    return scenes.config2(nder=8, res=512, spp=8, crop=(355, 244, 24, 24))
+    return scenes.dielectric_medium(nder=8, res=512, spp=8, crop=(355, 244, 24, 24))
 
 
 @pytest.fixture(scope="module")
-def cfg4():
-    return scenes.config4()
+def displaced_views():
+    return scenes.displaced_views()
 
 
 @pytest.mark.parametrize("mode", [-1, EMISSION | SURF_MAIN | VOL_MAIN, VOL_BOUNDARY1, VOL_BOUNDARY2, 0])
-def test_cfg2_full_mesh_parity(gpu, oracle, cfg2, mode):
+def test_dielectric_medium_full_mesh_parity(gpu, oracle, dielectric_medium, mode):
     """Per EMode term (0: the primary-edge pass alone). The oracle side renders 4x the
     GPU's spp per image, so its estimate is the closer to converged of the two."""
-    d = cfg2
+    d = dielectric_medium
     assert sum(len(s.indices) for s in d.shapes) >= 100820 and d.media[0].density.shape == (128, 128, 128)
     assert d.max_bounces == 12 and d.nder == 8
     # the primary-edge pass alone is sparse (silhouette pixels only): more samples, 6x6 blocks
@@ -40,15 +40,15 @@ def test_cfg2_full_mesh_parity(gpu, oracle, cfg2, mode):
     a = renders(gpu, d, range(16), mode=mode, sppe=sppe)
     b = renders(oracle, d, range(1000, 1012), mode=mode, sppe=4 * sppe, spp=4 * d.spp)
     assert np.abs(b[:, 1:].mean(0)).max() > 0   # the term has tangents in the crop
-    assert_parity(a, b, block, what="cfg2 full mesh, mode %d" % mode, max_mean_z2=1.8)
+    assert_parity(a, b, block, what="dielectric_medium full mesh, mode %d" % mode, max_mean_z2=1.8)
 
 
-@pytest.mark.parametrize("which", ["cfg2", "cfg4"])
-def test_bvh_parity_on_config_meshes(gpu, oracle, cfg2, cfg4, which):
+@pytest.mark.parametrize("which", ["dielectric_medium", "displaced_views"])
+def test_bvh_parity_on_config_meshes(gpu, oracle, dielectric_medium, displaced_views, which):
     """100K rays from around the displaced sphere towards it: the GPU's fp32 closest hit
     names the same face as the oracle's double-precision BVH for >= 99.99% of rays,
     and matched hits agree on t within 1e-5 relative."""
-    d = cfg2 if which == "cfg2" else cfg4
+    d = dielectric_medium if which == "dielectric_medium" else displaced_views
     rng = np.random.default_rng(11)
     rays = _rays(100000, rng, -0.95, 0.95, -0.3, 0.3)   # from inside the medium cube
     sg = scenes.instantiate(gpu, d)
@@ -126,11 +126,11 @@ def test_adjoint_full_size_matches_crn_finite_differences(gpu):
     gpu.set_density(sc, 0, dens)
 
 
-def test_cfg4_views_equal_single_renders(gpu, cfg4):
+def test_displaced_views_views_equal_single_renders(gpu, displaced_views):
     """The batched inverse-rendering scene (1,003,520 faces, 256^3 grid, nder 8): one
     dtrt_render_views call over three uneven views gives each view's single render."""
     import torch
-    d = cfg4
+    d = displaced_views
     sc = scenes.instantiate(gpu, d)
     views = scenes.orbit_views(32, seed=5, spp_choices=(16, 32, 64, 128, 256), radius=3.2)[:3]
     cams = [gpu.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)

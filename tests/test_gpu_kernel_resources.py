@@ -61,7 +61,7 @@ def test_every_kernel_within_the_scratch_limits(lib):
 
 
 def test_scene_beyond_a_lowered_limit_is_refused(gpu, lib):
-    d = scenes.config1(nder=16, res=16, spp=1, grid_res=8)
+    d = scenes.cube_medium(nder=16, res=16, spp=1, grid_res=8)
     try:
         assert lib.dtrt_test_set_scratch_limits(1024, 0) == 0
         with pytest.raises(RuntimeError, match="bytes of scratch per lane"):

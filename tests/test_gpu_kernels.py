@@ -32,26 +32,26 @@ def _global_face(desc, shape, tri):
     return np.where(shape >= 0, base[np.maximum(shape, 0)] + tri, -1)
 
 
-@pytest.mark.parametrize("which", ["cfg0", "cfg1", "cfg1_outward", "displaced_sphere"])
+@pytest.mark.parametrize("which", ["ico_surface", "cube_medium", "cube_medium_outward", "displaced_sphere"])
 def test_bvh_closest_hit_parity(gpu, oracle, which):
-    """cfg1_outward: rays from around the medium cube in all directions, most of them
+    """cube_medium_outward: rays from around the medium cube in all directions, most of them
     ending on the enclosing emissive sphere, whose triangles the builder splits off
     first and then splits by direction from the cube's centre."""
     rng = np.random.default_rng(7)
-    if which == "cfg0":
-        d = scenes.config0(nder=1, res=16, spp=1)
+    if which == "ico_surface":
+        d = scenes.ico_surface(nder=1, res=16, spp=1)
         rays = _rays(100000, rng, -3, 3, -0.6, 0.6)
         brute = True
-    elif which == "cfg1":
-        d = scenes.config1(nder=3, res=16, spp=1, grid_res=8)
+    elif which == "cube_medium":
+        d = scenes.cube_medium(nder=3, res=16, spp=1, grid_res=8)
         rays = _rays(100000, rng, -2, 2, -0.6, 0.6)
         brute = True
-    elif which == "cfg1_outward":
-        d = scenes.config1(nder=3, res=16, spp=1, grid_res=8)
+    elif which == "cube_medium_outward":
+        d = scenes.cube_medium(nder=3, res=16, spp=1, grid_res=8)
         rays = _rays(100000, rng, -3, 3, -30, 30)
         brute = True
     else:
-        d = scenes.config2(nder=1, res=16, spp=1, freq=30, grid_res=8)
+        d = scenes.dielectric_medium(nder=1, res=16, spp=1, freq=30, grid_res=8)
         rays = _rays(100000, rng, -2, 2, -0.5, 0.5)
         brute = False
     sg = scenes.instantiate(gpu, d)
@@ -62,7 +62,7 @@ def test_bvh_closest_hit_parity(gpu, oracle, which):
     # device's fp32 traversal and refinement, not the rounding of the input rays
     sh, tr, to = oracle.intersect(so, r32.astype(np.float64), brute=brute)
     fo = _global_face(d, sh, tr)
-    if which == "cfg1_outward":
+    if which == "cube_medium_outward":
         assert (sh == 2).mean() > 0.5   # most rays end on the enclosing sphere
     assert (fg == fo).mean() >= 0.9999
     both = (fg == fo) & (fo >= 0)
@@ -73,7 +73,7 @@ def test_bvh_closest_hit_parity(gpu, oracle, which):
 
 
 def test_grid_lookup_parity(gpu, oracle):
-    d = scenes.config1(nder=3, res=16, spp=1, grid_res=32)
+    d = scenes.cube_medium(nder=3, res=16, spp=1, grid_res=32)
     d.media[0].rot_vel = np.array([[0.2, 0.0, 0.1], [0.0, 0.3, 0.0], [0.0, 0.0, 0.0]], np.float32)
     sg = scenes.instantiate(gpu, d)
     so = scenes.instantiate(oracle, d)
@@ -103,7 +103,7 @@ def test_medium_estimators_closed_form(gpu):
 
 
 def test_primal_li_parity(gpu, oracle):
-    d = scenes.config1(nder=3, res=16, spp=1, grid_res=16)
+    d = scenes.cube_medium(nder=3, res=16, spp=1, grid_res=16)
     sg = scenes.instantiate(gpu, d)
     so = scenes.instantiate(oracle, d)
     rng = np.random.default_rng(5)

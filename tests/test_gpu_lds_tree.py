@@ -31,7 +31,7 @@ def test_whole_tree_in_lds_renders_as_from_hbm(gpu, bits):
     """The benchmark scene's 194-node tree fits whole: the AD-path kernel (64-lane
     groups) and the child kernel (256-lane groups) read nodes and triangles from LDS."""
     with precision(gpu, bits):
-        d = scenes.config1(nder=3, res=48, spp=32, grid_res=16)
+        d = scenes.cube_medium(nder=3, res=48, spp=32, grid_res=16)
         assert gpu.scene_info(scenes.instantiate(gpu, d))["nodes"] == 194
         _same_image(gpu, d, rtol=1e-5 if bits == 64 else 1e-3)
 
@@ -39,6 +39,6 @@ def test_whole_tree_in_lds_renders_as_from_hbm(gpu, bits):
 def test_tree_top_in_lds_renders_as_from_hbm(gpu):
     """The 100,820-face dielectric mesh at nder 8: the refill and child kernels keep
     the first few hundred nodes (the top of the tree) in LDS and read the rest from HBM."""
-    d = scenes.config2(nder=8, res=512, spp=4, crop=(355, 244, 24, 24))
+    d = scenes.dielectric_medium(nder=8, res=512, spp=4, crop=(355, 244, 24, 24))
     assert sum(len(s.indices) for s in d.shapes) >= 100820 and d.nder == 8
     _same_image(gpu, d, rtol=1e-5)

@@ -14,7 +14,7 @@ INTERIOR, LINS = 1 | 2 | 8, 16
 
 
 def _scene():
-    return scenes.config1(nder=3, res=32, spp=8, grid_res=16)
+    return scenes.cube_medium(nder=3, res=32, spp=8, grid_res=16)
 
 
 def test_vol_file_renders_like_the_in_memory_grid(gpu, tmp_path):

@@ -24,20 +24,20 @@ def precision(gpu, bits):
 
 def test_default_precision_is_double(gpu):
     assert gpu.precision() == 64
-    sc = scenes.instantiate(gpu, scenes.config0(nder=1, res=8, spp=1))
+    sc = scenes.instantiate(gpu, scenes.ico_surface(nder=1, res=8, spp=1))
     assert gpu.scene_precision(sc) == 64
     with precision(gpu, 32):
-        assert gpu.scene_precision(scenes.instantiate(gpu, scenes.config0(nder=1, res=8, spp=1))) == 32
+        assert gpu.scene_precision(scenes.instantiate(gpu, scenes.ico_surface(nder=1, res=8, spp=1))) == 32
     with pytest.raises(ValueError):
         gpu.set_precision(16)
 
 
-@pytest.mark.parametrize("which", ["cfg0", "cfg1"])
+@pytest.mark.parametrize("which", ["ico_surface", "cube_medium"])
 def test_fp32_leg_matches_the_oracle(gpu, oracle, which):
-    if which == "cfg0":
-        d, sppe = scenes.config0(nder=1, res=24, spp=32), 1024
+    if which == "ico_surface":
+        d, sppe = scenes.ico_surface(nder=1, res=24, spp=32), 1024
     else:
-        d, sppe = scenes.config1(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24)), 256
+        d, sppe = scenes.cube_medium(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24)), 256
     with precision(gpu, 32):
         a = renders(gpu, d, range(16), sppe=sppe)
     b = renders(oracle, d, range(1000, 1016), sppe=sppe)
@@ -46,7 +46,7 @@ def test_fp32_leg_matches_the_oracle(gpu, oracle, which):
 
 def test_fp64_and_fp32_legs_agree(gpu):
     """Same scene, both kernel sets, 24 renders each: block means agree at 3 sigma."""
-    d = scenes.config1(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
+    d = scenes.cube_medium(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
     a = renders(gpu, d, range(24), sppe=256)
     with precision(gpu, 32):
         b = renders(gpu, d, range(100, 124), sppe=256)

@@ -36,16 +36,16 @@ def test_primal_direct_lighting_matches_lambert_polygon(gpu, simple):
 
 
 @pytest.mark.parametrize("simple", [False, True])
-@pytest.mark.parametrize("cfg", ["cfg0", "cfg1", "cfg2"])
+@pytest.mark.parametrize("cfg", ["ico_surface", "cube_medium", "dielectric_medium"])
 def test_primal_parity(gpu, oracle, cfg, simple):
     """VolPathTracer (simple: VolPathTracerSimple) against the oracle's restatement."""
-    if cfg == "cfg0":
-        d = scenes.config0(nder=1, res=24, spp=32)
-    elif cfg == "cfg1":
-        d = scenes.config1(nder=1, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
+    if cfg == "ico_surface":
+        d = scenes.ico_surface(nder=1, res=24, spp=32)
+    elif cfg == "cube_medium":
+        d = scenes.cube_medium(nder=1, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
     else:
-        d = scenes.config2(nder=1, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
-    k = 32 if cfg == "cfg2" else 16   # the near-specular dielectric gives heavy-tailed samples
+        d = scenes.dielectric_medium(nder=1, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
+    k = 32 if cfg == "dielectric_medium" else 16   # the near-specular dielectric gives heavy-tailed samples
     a = _stack(gpu, d, range(k), simple=simple)
     b = _stack(oracle, d, range(100, 100 + k), simple=simple)
     assert_parity(a, b, 3, what=("VolPathTracerSimple " if simple else "VolPathTracer ") + cfg)
@@ -53,7 +53,7 @@ def test_primal_parity(gpu, oracle, cfg, simple):
 
 def test_primal_every_pixel_written_and_device_output(gpu):
     import torch
-    d = scenes.config1(nder=1, res=40, spp=8, grid_res=16)
+    d = scenes.cube_medium(nder=1, res=40, spp=8, grid_res=16)
     sc = scenes.instantiate(gpu, d)
     host = np.full((40, 40, 3), -7.0, np.float32)   # overwritten, as the reference writes every pixel
     opts = gpu.RenderOptions(5, 8, d.max_bounces, 0, 0, True)
@@ -84,18 +84,18 @@ def test_surface_integrators_direct_lighting_matches_lambert_polygon(gpu, kind):
 
 
 @pytest.mark.parametrize("kind", [3, 4])
-@pytest.mark.parametrize("cfg", ["cfg0", "cfg2"])
+@pytest.mark.parametrize("cfg", ["ico_surface", "dielectric_medium"])
 def test_surface_integrators_parity(gpu, oracle, cfg, kind):
     """PathTracer / DirectIntegrator against the oracle's restatement of path.cpp /
     direct.cpp (media are ignored by both)."""
-    if cfg == "cfg0":
-        d = scenes.config0(nder=1, res=24, spp=32)
+    if cfg == "ico_surface":
+        d = scenes.ico_surface(nder=1, res=24, spp=32)
     else:
-        d = scenes.config2(nder=1, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
+        d = scenes.dielectric_medium(nder=1, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
         # a rougher dielectric: without media the near-specular glass turns every light
         # path through it into a caustic reached by BSDF sampling alone (fireflies)
         d.bsdfs = [("roughdielectric", 0.3) + tuple(b[2:]) if b[0] == "roughdielectric" else b for b in d.bsdfs]
-    k = 32 if cfg == "cfg2" else 16
+    k = 32 if cfg == "dielectric_medium" else 16
     sg, so = scenes.instantiate(gpu, d), scenes.instantiate(oracle, d)
     a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, kind=kind)[None] for s in range(k)])
     b = np.stack([scenes.render_primal(oracle, d, seed=100 + s, scene=so, kind=kind)[None] for s in range(k)])
@@ -113,13 +113,13 @@ def test_particle_tracer_closed_forms(gpu):
     assert abs(v.mean() / (2.0 * sum(0.5 ** k for k in range(6))) - 1) < 3e-3
 
 
-@pytest.mark.parametrize("cfg", ["cfg0", "cfg1"])
+@pytest.mark.parametrize("cfg", ["ico_surface", "cube_medium"])
 def test_particle_tracer_parity(gpu, oracle, cfg):
-    """ParticleTracer against the oracle's restatement of ptracer.cpp (cfg1: media)."""
-    if cfg == "cfg0":
-        d = scenes.config0(nder=1, res=24, spp=32)
+    """ParticleTracer against the oracle's restatement of ptracer.cpp (cube_medium: media)."""
+    if cfg == "ico_surface":
+        d = scenes.ico_surface(nder=1, res=24, spp=32)
     else:
-        d = scenes.config1(nder=1, res=24, spp=32, grid_res=16)
+        d = scenes.cube_medium(nder=1, res=24, spp=32, grid_res=16)
     sg, so = scenes.instantiate(gpu, d), scenes.instantiate(oracle, d)
     a = np.stack([scenes.render_primal(gpu, d, seed=s, scene=sg, kind=5, spp=4)[None] for s in range(16)])
     b = np.stack([scenes.render_primal(oracle, d, seed=100 + s, scene=so, kind=5, spp=4)[None] for s in range(16)])

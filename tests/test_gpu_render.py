@@ -66,13 +66,13 @@ def _crn_fd(gpu, make, h, seed=5):
 
 
 def test_reflectance_tangent_equals_crn_fd(gpu):
-    ad, fd = _crn_fd(gpu, lambda e: F.cfg0_reflectance_lane(res=32, spp=16, rho=0.8 + e), 1e-2)
+    ad, fd = _crn_fd(gpu, lambda e: F.ico_surface_reflectance_lane(res=32, spp=16, rho=0.8 + e), 1e-2)
     assert np.abs(fd).max() > 1e-2
     assert np.allclose(ad, fd, rtol=2e-3, atol=2e-3)
 
 
 def test_albedo_tangent_equals_crn_fd(gpu):
-    ad, fd = _crn_fd(gpu, lambda e: F.cfg1_albedo_lane(res=24, spp=16, albedo=0.8 + e), 1e-2)
+    ad, fd = _crn_fd(gpu, lambda e: F.cube_medium_albedo_lane(res=24, spp=16, albedo=0.8 + e), 1e-2)
     assert np.abs(fd).max() > 1e-2
     assert np.allclose(ad, fd, rtol=5e-3, atol=5e-3)
 
@@ -87,17 +87,17 @@ def _parity(gpu, oracle, d, mode=-1, seeds=16, block=4, **kw):
 
 
 @pytest.mark.parametrize("mode", [-1, EMISSION | SURF_MAIN, SURF_BOUNDARY])
-def test_cfg0_parity(gpu, oracle, mode):
-    d = scenes.config0(nder=1, res=24, spp=32)
+def test_ico_surface_parity(gpu, oracle, mode):
+    d = scenes.ico_surface(nder=1, res=24, spp=32)
     z = _parity(gpu, oracle, d, mode=mode, sppe=1024)
-    assert_3sigma(z, what="cfg0 mode %d" % mode)
+    assert_3sigma(z, what="ico_surface mode %d" % mode)
 
 
 @pytest.mark.parametrize("mode", [-1, EMISSION | VOL_MAIN, VOL_BOUNDARY1, VOL_BOUNDARY2])
-def test_cfg1_parity(gpu, oracle, mode):
-    d = scenes.config1(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
+def test_cube_medium_parity(gpu, oracle, mode):
+    d = scenes.cube_medium(nder=3, res=64, spp=32, grid_res=32, crop=(20, 20, 24, 24))
     z = _parity(gpu, oracle, d, mode=mode, sppe=256)
-    assert_3sigma(z, what="cfg1 mode %d" % mode)
+    assert_3sigma(z, what="cube_medium mode %d" % mode)
 
 
 @pytest.mark.parametrize("move", ["light", "box"])
@@ -109,16 +109,16 @@ def test_medium_box_parity(gpu, oracle, move):
     assert_3sigma(z, what="medium box " + move)
 
 
-def test_cfg2_small_parity(gpu, oracle):
+def test_dielectric_medium_small_parity(gpu, oracle):
     """Rough dielectric displaced sphere with shading normals in a medium, 8 lanes."""
-    d = scenes.config2(nder=8, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
+    d = scenes.dielectric_medium(nder=8, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
     z = _parity(gpu, oracle, d, seeds=12, block=4, sppe=128, max_bounces=6)
-    assert_3sigma(z, what="cfg2 small", max_mean_z2=1.8)
+    assert_3sigma(z, what="dielectric_medium small", max_mean_z2=1.8)
 
 
 def test_render_into_device_tensor(gpu):
     import torch
-    d = scenes.config0(nder=1, res=16, spp=8)
+    d = scenes.ico_surface(nder=1, res=16, spp=8)
     sc = scenes.instantiate(gpu, d)
     host = scenes.render(gpu, d, seed=2, scene=sc)
     out = torch.zeros(d.out_shape(), dtype=torch.float32, device="cuda")
@@ -135,7 +135,7 @@ def test_queue_overflow_replay_is_exact(gpu, nder):
     """A queue far smaller than one chunk's children forces many overflow/replay rounds;
     every child and every sample must still be counted exactly once. At nder 8 the AD
     samples run in the per-bounce refill kernel (k_ad_refill), at nder 3 in k_ad_paths."""
-    d = scenes.config1(nder=nder, res=64, spp=64, grid_res=16, crop=(16, 16, 32, 32))
+    d = scenes.cube_medium(nder=nder, res=64, spp=64, grid_res=16, crop=(16, 16, 32, 32))
     sc = scenes.instantiate(gpu, d)
     ref = scenes.render(gpu, d, seed=4, scene=sc, sppe=64).astype(np.float64)
     st_ref = gpu.render_stats(sc)
@@ -158,7 +158,7 @@ def test_value_lane_independent_of_lane_count(gpu):
     gives the same image at every compiled lane count (all kernel instantiations)."""
     ref = None
     for n in (1, 2, 3, 4, 8, 16):
-        d = scenes.config1(nder=n, res=64, spp=8, grid_res=24, crop=(16, 16, 32, 32))
+        d = scenes.cube_medium(nder=n, res=64, spp=8, grid_res=24, crop=(16, 16, 32, 32))
         for s in d.shapes:
             if s.velocities is not None:
                 s.velocities = np.zeros_like(s.velocities)
@@ -189,7 +189,7 @@ def test_rotating_grid_parity(gpu, oracle):
 def test_autotune_keeps_results_and_picks_listed_settings(gpu):
     """dtrt_scene_autotune only changes launch settings: the same seed renders the same
     image (up to float summation order) before and after tuning."""
-    d = scenes.config1(nder=3, res=64, spp=32, grid_res=16)
+    d = scenes.cube_medium(nder=3, res=64, spp=32, grid_res=16)
     sc = scenes.instantiate(gpu, d)
     before = scenes.render(gpu, d, seed=9, scene=sc).astype(np.float64)
     opts = gpu.RenderOptions(3, d.spp, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)
@@ -209,7 +209,7 @@ def test_autotune_keeps_results_and_picks_listed_settings(gpu):
 def test_workgroup_sizes_render_the_same_image(gpu, groups):
     """The AD-path and child kernels at other work-group sizes (the traversal stack in
     LDS follows the block size) and a small queue budget: same seed, same image."""
-    d = scenes.config1(nder=3, res=48, spp=32, grid_res=16)
+    d = scenes.cube_medium(nder=3, res=48, spp=32, grid_res=16)
     sc = scenes.instantiate(gpu, d)
     ref = scenes.render(gpu, d, seed=5, scene=sc).astype(np.float64)
     gpu.set_launch_settings(sc, ad_workgroup=groups[0], child_workgroup=groups[1], queue_budget_bytes=1 << 24)

@@ -12,7 +12,7 @@ INTERIOR = 1 | 2 | 8
 
 def _setup(gpu, nviews=3):
     import torch
-    d = scenes.config2(nder=8, res=48, spp=4, freq=10, grid_res=16)
+    d = scenes.dielectric_medium(nder=8, res=48, spp=4, freq=10, grid_res=16)
     sc = scenes.instantiate(gpu, d)
     views = scenes.orbit_views(nviews, seed=3, res_choices=(32, 40, 48), spp_choices=(2, 4), radius=3.2)
     cams = [gpu.Camera(w, h, c2w.astype(np.float32), d.cam_to_ndc.astype(np.float32), d.clip_near, -1, None)
@@ -101,7 +101,7 @@ def test_shape_lane_gradient_matches_crn_finite_differences(gpu, lane):
     terminators where the view direction crosses the interpolated normal, facet edges of
     flat shading), so its shape derivative differs from finite differences there."""
     import torch
-    d = scenes.config2(nder=8, res=48, spp=256, freq=10, grid_res=16)
+    d = scenes.dielectric_medium(nder=8, res=48, spp=256, freq=10, grid_res=16)
     d.bsdfs[2] = ("diffuse", (0.0, 0.0, 0.0))
     d.shapes[0].velocities[0] = np.array([1.0, 0.0, 0.0], np.float32)
     sc = scenes.instantiate(gpu, d)

@@ -41,12 +41,12 @@ def test_direct_lighting_value(gpu, name):
 
 
 @pytest.mark.parametrize("name", NAMES)
-@pytest.mark.parametrize("cfg", ["cfg0", "cfg2"])
+@pytest.mark.parametrize("cfg", ["ico_surface", "dielectric_medium"])
 def test_parity(gpu, oracle, cfg, name):
-    if cfg == "cfg0":
-        d, kw = scenes.config0(nder=1, res=24, spp=32), dict(sppe=1024)
+    if cfg == "ico_surface":
+        d, kw = scenes.ico_surface(nder=1, res=24, spp=32), dict(sppe=1024)
     else:
-        d = scenes.config2(nder=8, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
+        d = scenes.dielectric_medium(nder=8, res=48, spp=16, freq=10, grid_res=16, crop=(14, 14, 20, 20))
         d.bsdfs = [("roughdielectric", 0.3) + tuple(b[2:]) if b[0] == "roughdielectric" else b for b in d.bsdfs]
         kw = dict(sppe=128, max_bounces=6)
     sg, so = scenes.instantiate(gpu, d), scenes.instantiate(oracle, d)
@@ -59,7 +59,7 @@ def test_parity(gpu, oracle, cfg, name):
 
 def test_path_ad_overflow_replay_is_exact(gpu):
     """Edge sub-path records (CHILD_PATH) through the exact overflow replay."""
-    d = scenes.config0(nder=1, res=32, spp=64)
+    d = scenes.ico_surface(nder=1, res=32, spp=64)
     sc = scenes.instantiate(gpu, d)
     ref = scenes.render(gpu, d, seed=4, scene=sc, sppe=64, integrator=gpu.PathTracerAD()).astype(np.float64)
     st_ref = gpu.render_stats(sc)

@@ -30,7 +30,7 @@ def _same_images(a, b):
 
 @pytest.mark.parametrize("which", ["sphere", "light"])
 def test_updated_scene_renders_like_a_rebuilt_one(gpu, which):
-    d = scenes.config0(nder=1, res=32, spp=16)
+    d = scenes.ico_surface(nder=1, res=32, spp=16)
     si = 1 if which == "sphere" else 2   # ground, icosphere, light quad
     new = _moved(d, si, 1.15, (0.12, 0.05, -0.08))
     sc = scenes.instantiate(gpu, d)
@@ -47,7 +47,7 @@ def test_updated_scene_renders_like_a_rebuilt_one(gpu, which):
 
 
 def test_refitted_bvh_traces_like_the_oracle(gpu, oracle):
-    d = scenes.config2(nder=1, res=16, spp=1, freq=30, grid_res=8)
+    d = scenes.dielectric_medium(nder=1, res=16, spp=1, freq=30, grid_res=8)
     new = _moved(d, 0, 0.9, (0.05, -0.1, 0.02))
     sc = scenes.instantiate(gpu, d)
     gpu.update_shape(sc, 0, new.shapes[0].vertices, new.shapes[0].normals)
@@ -65,7 +65,7 @@ def test_refitted_bvh_traces_like_the_oracle(gpu, oracle):
 
 def test_update_releases_captured_graphs(gpu):
     import torch
-    d = scenes.config0(nder=1, res=16, spp=4)
+    d = scenes.ico_surface(nder=1, res=16, spp=4)
     sc = scenes.instantiate(gpu, d)
     out = [torch.zeros(d.out_shape(), device="cuda")]
     opts = [gpu.RenderOptions(1, 4, d.max_bounces, d.sppe, d.sppse, True, -1, d.ddist)]

@@ -23,7 +23,7 @@ def test_views_equal_single_renders(gpu, nder):
     """At nder 8 the AD samples run in the per-bounce refill kernel, whose waves mix
     samples of several pixels and views."""
     import torch
-    d = scenes.config0(nder=nder, res=32, spp=16)
+    d = scenes.ico_surface(nder=nder, res=32, spp=16)
     sc = scenes.instantiate(gpu, d)
     cams = [None, _camera(gpu, d, (1.5, 1.2, 3.0), (48, 40)), _camera(gpu, d, (-2.0, 2.0, 2.0), (64, 64), crop=(16, 8, 24, 32))]
     shapes = [(nder + 1, 32, 32, 3), (nder + 1, 40, 48, 3), (nder + 1, 32, 24, 3)]
@@ -48,7 +48,7 @@ def test_views_equal_single_renders(gpu, nder):
 
 def test_views_reject_mixed_settings(gpu):
     import torch
-    d = scenes.config0(nder=1, res=16, spp=4)
+    d = scenes.ico_surface(nder=1, res=16, spp=4)
     sc = scenes.instantiate(gpu, d)
     outs = [torch.zeros((2, 16, 16, 3), device="cuda") for _ in range(2)]
     with pytest.raises(RuntimeError, match="share"):
@@ -59,7 +59,7 @@ def test_graph_replay_equals_eager(gpu):
     """Replays of the captured views with new seeds, with eager renders in between,
     each equal to an eager render with those seeds."""
     import torch
-    d = scenes.config0(nder=1, res=32, spp=16)
+    d = scenes.ico_surface(nder=1, res=32, spp=16)
     sc = scenes.instantiate(gpu, d)
     cams = [None, _camera(gpu, d, (1.5, 1.2, 3.0), (48, 40))]
     shapes = [(2, 32, 32, 3), (2, 40, 48, 3)]
@@ -82,7 +82,7 @@ def test_graph_replay_equals_eager(gpu):
 
 def test_graph_capture_refuses_overflowing_schedule(gpu):
     import torch
-    d = scenes.config0(nder=1, res=32, spp=16)
+    d = scenes.ico_surface(nder=1, res=32, spp=16)
     sc = scenes.instantiate(gpu, d)
     gpu.set_queue_budget(sc, 64 * 128)
     with pytest.raises(RuntimeError, match="overflowed"):
@@ -95,8 +95,8 @@ def test_scenes_render_together_as_alone(gpu):
     dielectric in a medium), each on its own stream and host thread, gives every view
     the image of that scene's render_views alone; a scene listed twice is refused."""
     import torch
-    ds = [scenes.config0(nder=4, res=32, spp=16), scenes.config1(nder=4, res=40, spp=8, grid_res=16),
-          scenes.config2(nder=4, res=32, spp=8, freq=8, grid_res=16)]
+    ds = [scenes.ico_surface(nder=4, res=32, spp=16), scenes.cube_medium(nder=4, res=40, spp=8, grid_res=16),
+          scenes.dielectric_medium(nder=4, res=32, spp=8, freq=8, grid_res=16)]
     scs = [scenes.instantiate(gpu, d) for d in ds]
     cams = [[None], [None, _camera(gpu, ds[1], (1.5, 1.2, 3.0), (48, 40))], [None]]
     opts = [[gpu.RenderOptions(1, 16, 5, 256, 4, True)],

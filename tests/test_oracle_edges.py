@@ -94,7 +94,7 @@ def test_path_ad_agrees_with_volpath_ad_on_surfaces(oracle):
     """Without media or null surfaces PathTracerAD and VolPathTracerAD estimate the same
     image and tangents (all EMode terms, both edge passes)."""
     from tests.helpers import assert_3sigma, block_z
-    d = scenes.config0(nder=1, res=16, spp=8)
+    d = scenes.ico_surface(nder=1, res=16, spp=8)
     sc = scenes.instantiate(oracle, d)
     a = np.stack([scenes.render(oracle, d, seed=s, scene=sc, sppe=256) for s in range(24)]).astype(np.float64)
     b = np.stack([scenes.render(oracle, d, seed=100 + s, scene=sc, sppe=256, integrator=oracle.PathTracerAD())

@@ -25,7 +25,7 @@ def test_icosphere_edge_count_is_euler(oracle, sub):
     v, f = scenes.icosphere(sub, 0.5)
     g = GOLD["icosphere_subdiv%d" % sub]
     assert (len(v), len(f)) == (g["vertices"], g["faces"])
-    d = scenes.config0(nder=1, res=16, spp=1)
+    d = scenes.ico_surface(nder=1, res=16, spp=1)
     d.shapes[1] = scenes.ShapeDesc(v, f, bsdf_id=1)
     sc = scenes.instantiate(oracle, d)
     n, length = oracle.scene_query(sc, "num_edges", 1)
@@ -36,14 +36,14 @@ def test_icosphere_edge_count_is_euler(oracle, sub):
 
 
 def test_coplanar_diagonals_are_not_edges(oracle):
-    d = scenes.config0(nder=1, res=16, spp=1)
+    d = scenes.ico_surface(nder=1, res=16, spp=1)
     sc = scenes.instantiate(oracle, d)
     n, length = oracle.scene_query(sc, "num_edges", 0)     # 4x4 ground quad: 4 boundary edges
     assert int(n) == 4 and abs(length - 16.0) < 1e-6
 
 
 def test_bvh_matches_brute_force(oracle):
-    d = scenes.config0(nder=1, res=16, spp=1)
+    d = scenes.ico_surface(nder=1, res=16, spp=1)
     sc = scenes.instantiate(oracle, d)
     rng = np.random.default_rng(0)
     n = 5000
@@ -60,7 +60,7 @@ def test_bvh_matches_brute_force(oracle):
 
 
 def test_primary_edges_are_projected_vertices(oracle):
-    d = scenes.config0(nder=1, res=64, spp=1)
+    d = scenes.ico_surface(nder=1, res=64, spp=1)
     sc = scenes.instantiate(oracle, d)
     cnt, total = oracle.scene_query(sc, "primary_edges")
     assert int(cnt) > 0
@@ -96,7 +96,7 @@ def test_primary_edges_are_projected_vertices(oracle):
 
 
 def test_light_pmf_proportional_to_power(oracle):
-    d = scenes.config1(nder=3, res=16, spp=1, grid_res=8)
+    d = scenes.cube_medium(nder=3, res=16, spp=1, grid_res=8)
     sc = scenes.instantiate(oracle, d)
     pmf = oracle.scene_query(sc, "light_pmf")
 

@@ -50,13 +50,13 @@ def _fd_crn(mod, make, h, seed=5):
 
 
 def test_reflectance_tangent_equals_crn_finite_difference(oracle):
-    ad, fd = _fd_crn(oracle, lambda e: F.cfg0_reflectance_lane(res=16, spp=4, rho=0.8 + e), 1e-3)
+    ad, fd = _fd_crn(oracle, lambda e: F.ico_surface_reflectance_lane(res=16, spp=4, rho=0.8 + e), 1e-3)
     assert np.abs(fd).max() > 1e-2
     assert np.allclose(ad, fd, rtol=2e-3, atol=2e-4)
 
 
 def test_medium_albedo_tangent_equals_crn_finite_difference(oracle):
-    ad, fd = _fd_crn(oracle, lambda e: F.cfg1_albedo_lane(res=12, spp=4, albedo=0.8 + e, grid_res=12), 1e-3)
+    ad, fd = _fd_crn(oracle, lambda e: F.cube_medium_albedo_lane(res=12, spp=4, albedo=0.8 + e, grid_res=12), 1e-3)
     assert np.abs(fd).max() > 1e-2
     assert np.allclose(ad, fd, rtol=2e-3, atol=2e-3)
 
@@ -80,7 +80,7 @@ def test_primal_direct_lighting_matches_lambert_polygon(oracle):
 def test_primal_equals_ad_value_channel(oracle):
     """VolPathTracer and the value channel of VolPathTracerAD's interior terms estimate
     the same radiance (volpath.cpp:10-103 vs volpathAD.cpp:145-620 on the value lane)."""
-    d = scenes.config1(nder=1, res=32, spp=16, grid_res=16, crop=(10, 10, 12, 12))
+    d = scenes.cube_medium(nder=1, res=32, spp=16, grid_res=16, crop=(10, 10, 12, 12))
     sc = scenes.instantiate(oracle, d)
     a = np.stack([scenes.render_primal(oracle, d, seed=s, scene=sc)[None] for s in range(48)]).astype(np.float64)
     b = np.stack([scenes.render(oracle, d, seed=1000 + s, scene=sc, mode=1 | 2 | 8, sppe=0, sppse=0)[:1]
@@ -106,7 +106,7 @@ def test_primal_simple_direct_lighting_matches_lambert_polygon(oracle):
 
 def test_primal_simple_equals_mis_estimator(oracle):
     """NEE-only and NEE+MIS primal estimators have the same expectation."""
-    d = scenes.config0(nder=1, res=24, spp=8)
+    d = scenes.ico_surface(nder=1, res=24, spp=8)
     sc = scenes.instantiate(oracle, d)
     a = np.stack([scenes.render_primal(oracle, d, seed=s, scene=sc, simple=True)[None] for s in range(32)])
     b = np.stack([scenes.render_primal(oracle, d, seed=500 + s, scene=sc)[None] for s in range(32)])

@@ -40,8 +40,8 @@ def test_noalloc_equals_render_device_twice(gpu, bits):
     gpu.set_precision(bits)
     try:
         # first a larger medium scene, then a smaller surface scene in the same scratch area
-        for d, seed in ((scenes.config1(nder=3, res=96, spp=16, grid_res=32), 5),
-                        (scenes.config0(nder=1, res=40, spp=16), 6)):
+        for d, seed in ((scenes.cube_medium(nder=3, res=96, spp=16, grid_res=32), 5),
+                        (scenes.ico_surface(nder=1, res=40, spp=16), 6)):
             ref, got = _pair(gpu, d, seed)
             assert np.abs(ref[0]).max() > 0 and np.abs(ref[1:]).max() > 0
             assert np.allclose(got, ref, rtol=1e-4, atol=1e-5 * np.abs(ref).max())
@@ -51,7 +51,7 @@ def test_noalloc_equals_render_device_twice(gpu, bits):
 
 def test_noalloc_refuses_images_larger_than_the_scratch_area(gpu):
     import torch
-    big = scenes.config0(nder=16, res=1400, spp=1)   # 17 x 1400^2 x 3 doubles of accumulators alone: 800 MB
+    big = scenes.ico_surface(nder=16, res=1400, spp=1)   # 17 x 1400^2 x 3 doubles of accumulators alone: 800 MB
     sc = scenes.instantiate(gpu, big)
     out = torch.zeros(8, dtype=torch.float32, device="cuda")   # never written: refused before any launch
     opts = gpu.RenderOptions(1, 1, big.max_bounces, 0, 0, True, -1, big.ddist)
