@@ -42,3 +42,22 @@ def test_tree_top_in_lds_renders_as_from_hbm(gpu):
     d = scenes.dielectric_medium(nder=8, res=512, spp=4, crop=(355, 244, 24, 24))
     assert sum(len(s.indices) for s in d.shapes) >= 100820 and d.nder == 8
     _same_image(gpu, d, rtol=1e-5)
+
+
+def test_deep_tree_renders_as_from_hbm(gpu):
+    """A comb of 2,000 slivers spaced geometrically along x, in front of the icosphere
+    scene, makes a 24-level tree: the traversal stacks, sized to the tree's depth, then
+    hold 24 entries per lane, and the top of the tree follows them in LDS (nder 8). A
+    stack deeper than its size would overwrite the tree's copy."""
+    base = scenes.ico_surface(nder=8, res=48, spp=16)
+    n = 2000
+    x = (np.cumsum(1.02 ** np.arange(n)) * 1e-3).astype(np.float32) - 1.2
+    v = np.zeros((3 * n, 3), np.float32)
+    v[0::3, 0], v[1::3, 0], v[2::3, 0] = x, x + 1e-3, x
+    v[:, 1] = 0.1
+    v[2::3, 1] = 0.6
+    v[:, 2] = 0.8
+    comb = scenes.ShapeDesc(v, np.arange(3 * n, dtype=np.int32).reshape(-1, 3), bsdf_id=1)
+    base.shapes = list(base.shapes) + [comb]
+    assert gpu.scene_info(scenes.instantiate(gpu, base))["nodes"] == 1931
+    _same_image(gpu, base, rtol=1e-5)
